@@ -1,0 +1,244 @@
+/* oppx: MI355X-native (gfx950) particle hot path behind OP-PIC's opp_lib API.
+ *
+ * This is the C-ABI layer: extern "C", plain pointers and sizes. Device pointers are
+ * HIP device allocations; `stream` is a hipStream_t passed as void* (NULL = default
+ * stream). Every entry point returns an int status (OPPX_OK or an OPPX_ERR_* code);
+ * oppx_last_error() gives the message of the last failure on the calling thread.
+ *
+ * Data layout on the device (chosen for HBM3E streaming + L2-friendly gathers):
+ *   particle dats : SoA, component d of particle i at base[d * stride + i], stride = set capacity
+ *   mesh dats     : AoS, cell c's record contiguous (c_det = 16 doubles = one 128 B line)
+ *   tet aux       : per cell {c2 = (1/6)/volume, c2c[4]} packed in 32 B (oppx_tet_aux)
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the
+ * OP-PIC source tree).
+ */
+#ifndef OPPX_H
+#define OPPX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OPPX_OK 0
+#define OPPX_ERR_ARG 1    /* invalid argument (null pointer, size out of range, ...) */
+#define OPPX_ERR_HIP 2    /* a HIP runtime call failed */
+#define OPPX_ERR_NOMEM 3  /* device or host allocation failed, or admission budget exceeded */
+#define OPPX_ERR_STATE 4  /* call out of order (library not initialised, handle busy, ...) */
+
+/* MAX_CELL_INDEX of opp_lib/include/opp_defs.h:150: the p2c value of a removed particle. */
+#define OPPX_MAX_CELL_INDEX 2147483647
+
+/* Fill policy of opp_lib/include/opp_defs.h:252-258 (opp_fill_type). */
+#define OPPX_FILL_HOLEFILL_ALL 0
+#define OPPX_FILL_SORT_ALL 1
+#define OPPX_FILL_SHUFFLE_ALL 2
+#define OPPX_FILL_SORT_PERIODIC 3
+#define OPPX_FILL_SHUFFLE_PERIODIC 4
+
+/* ---------------------------------------------------------------------------------------
+ * Device-side records
+ * ------------------------------------------------------------------------------------- */
+
+/* Per-tet data the multi-hop walk reads besides c_det: the mover's coefficient
+ * (1/6)/volume (apps/app_fempic/kernels.h:110) and the face neighbours c2c (-1 = boundary). */
+typedef struct oppx_tet_aux {
+    double c2;
+    int32_t c2c[4];
+    int32_t pad[2];
+} oppx_tet_aux;
+
+/* Tet mesh as the hot path sees it (all device pointers). */
+typedef struct oppx_tet_mesh {
+    int64_t n_cells;
+    int64_t n_nodes;
+    const double* det;         /* [16*n_cells] alpha,beta,gamma,delta per vertex (c_det) */
+    const oppx_tet_aux* aux;   /* [n_cells] */
+    const int32_t* c2n;        /* [4*n_cells] cell -> node map */
+    const double* ef;          /* [3*n_cells] cell electric field (c_ef), AoS */
+} oppx_tet_mesh;
+
+/* Direct-hop overlay (opp_lib/include/opp_direct_hop_core.h:264-454, CellMapper). */
+typedef struct oppx_dh_grid {
+    const int32_t* cell;       /* device [dims[0]*dims[1]*dims[2]] owning cell or MAX_CELL_INDEX */
+    int64_t dims[3];
+    double min[3];             /* global minimum corner of the bounding box */
+    double inv_gs;             /* 1/grid_spacing */
+} oppx_dh_grid;
+
+/* Particle state of one FEM-PIC instance (device pointers, SoA with `stride`). */
+typedef struct oppx_fempic_parts {
+    int64_t stride;            /* set capacity */
+    double* pos;               /* [3*stride] */
+    double* vel;               /* [3*stride] */
+    double* lc;                /* [4*stride] */
+    int32_t* p2c;              /* [stride]   */
+} oppx_fempic_parts;
+
+/* Device counters a move pass maintains (one per particle set). */
+typedef struct oppx_move_counters {
+    int32_t size;              /* live particles before the move (input), after finalize (output) */
+    int32_t removed;           /* particles removed by this move */
+    int32_t holes_low;         /* holes below the new size = records moved by hole-fill */
+    int32_t max_hops;          /* longest walk of the pass */
+    int64_t total_hops;        /* kernel evaluations of the pass */
+    int32_t faults;            /* particles whose walk hit the hop cap or an invalid cell index */
+    int32_t pad;
+} oppx_move_counters;
+
+/* Walks longer than this are stopped and counted in oppx_move_counters.faults (the
+ * reference would loop forever on a cycling walk). */
+#define OPPX_MAX_HOPS 4096
+
+/* One particle dat for compaction/permutation: SoA base, components, element bytes (4 or 8). */
+typedef struct oppx_dat_desc {
+    void* base;
+    int32_t dim;
+    int32_t elem_bytes;
+} oppx_dat_desc;
+
+/* ---------------------------------------------------------------------------------------
+ * Library / device
+ * ------------------------------------------------------------------------------------- */
+
+/* opp_init's device part (opp_lib/src/hip/opp_hip.cpp:221-271, opp_device_init). */
+int oppx_init(int device);
+const char* oppx_last_error(void);
+int oppx_device_malloc(void** ptr, size_t bytes);
+int oppx_device_free(void* ptr);
+int oppx_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
+int oppx_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
+int oppx_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
+int oppx_memset_zero(void* dst, size_t bytes, void* stream);
+int oppx_stream_sync(void* stream);
+
+/* AoS (host layout, doc/api.rst:115-118) <-> SoA (device layout) transposes on the device,
+ * replacing the host transposes of opp_lib/src/hip/opp_hip_utils.cpp:35-153. */
+int oppx_aos_to_soa(const void* aos, void* soa, int64_t n, int dim, int64_t stride, int elem_bytes, void* stream);
+int oppx_soa_to_aos(const void* soa, void* aos, int64_t n, int dim, int64_t stride, int elem_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Host builders (CPU only; no device needed). Inputs of the hot path.
+ * ------------------------------------------------------------------------------------- */
+
+/* Box mesh of nx*ny*nz cubes x 6 tets; geometry per apps/app_fempic/minifempic_funcs.h:284-443. */
+int oppx_box_mesh_sizes(int nx, int ny, int nz, int64_t* n_nodes, int64_t* n_cells);
+int oppx_box_mesh_build(int nx, int ny, int nz, double lx, double ly, double lz,
+                        double* node_pos, int* c2n, int* c2c, double* c_det, double* c_vol);
+/* c_ef = -grad of the P1 interpolant of sin(pi x) sin(pi y) sin(pi z). */
+int oppx_box_field_sines(int64_t n_nodes, int64_t n_cells, const double* node_pos, const int* c2n,
+                         const double* c_det, const double* c_vol, double* c_ef);
+/* Seeded uniform positions, N(0,sigma^2) velocities, containing tet (AoS host arrays). */
+int oppx_box_particles_uniform(int64_t n_part, uint64_t seed, double sigma, int nx, int ny, int nz,
+                               double lx, double ly, double lz, double* pos, double* vel, int* p2c);
+/* Overlay dims of CellMapper::init_host (opp_lib/include/opp_direct_hop_core.h:317-336). */
+int oppx_dh_grid_dims(const double bmin[3], const double bmax[3], double gs, int64_t dims[3]);
+/* Overlay contents of CellMapper::generateStructuredMesh (opp_lib/src/seq/opp_direct_hop_seq.cpp:134-265),
+ * built with a candidate-restricted search that returns the reference's first-in-index-order owner. */
+int oppx_dh_build_overlay_nodes(int64_t n_cells, const double* c_det, const double* c_vol, const int* c_gbl_id,
+                                const int* c2n, const double* node_pos, const double bmin[3],
+                                const double bmax[3], double gs, int* overlay);
+
+/* NESO-advection 2-D periodic quad mesh (apps/app_neso_advection/advec_misc.h:115-143):
+ * c_pos_ll [2*nx*ny] lower-left corners, c2c [4*nx*ny] = {x-1, x+1, y-1, y+1} wrapped. */
+int oppx_neso_mesh(int nx, int ny, double cell_width, double* c_pos_ll, int* c2c);
+/* Seeded uniform positions in the periodic box, velocities uniform in [-vmax, vmax]^2, containing cell. */
+int oppx_neso_particles(int64_t n_part, uint64_t seed, int nx, int ny, double cell_width, double vmax, double* pos,
+                        double* vel, int* p2c);
+
+/* ---------------------------------------------------------------------------------------
+ * FEM-PIC hot path (device)
+ * ------------------------------------------------------------------------------------- */
+
+/* Build the per-tet aux records from device c_volume [n] and c2c [4n]. */
+int oppx_tet_aux_build(const double* c_vol, const int32_t* c2c, int64_t n_cells, oppx_tet_aux* aux, void* stream);
+
+/* opp_par_loop_all__calculate_new_pos_vel_kernel
+ * (apps_codegen/app_fempic_cg/hip/calculate_new_pos_vel_kernel_loop.hpp:31-100; elemental
+ * apps/app_fempic/kernels.h:93-103): vel += k*E[p2c]; pos += vel*dt with k = (charge/mass)*dt,
+ * over the counters->size live particles (n_max bounds the grid). */
+int oppx_fempic_push(int64_t n_max, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh,
+                     double k, double dt, oppx_move_counters* counters, void* stream);
+
+/* opp_particle_move__move_kernel (apps_codegen/app_fempic_cg/hip/move_kernel_loop.hpp:142-407):
+ * optional direct-hop relocation (dh may be NULL; opp_lib/include/device_kernels/hip_inline_kernels.h:330-393)
+ * then the multi-hop barycentric walk (apps/app_fempic/kernels.h:106-161). Reads counters->size,
+ * writes counters->max_hops/total_hops/faults and the removal bitmap hole_bits
+ * [oppx_hole_bits_words(n_max)] used by oppx_holefill (which sets counters->removed). `hops` (nullable) receives per-particle hop counts.
+ * n_max bounds the live count (grid size); the live count itself is read on the device. */
+int oppx_fempic_move(int64_t n_max, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh,
+                     const oppx_dh_grid* dh, oppx_move_counters* counters, uint64_t* hole_bits,
+                     int32_t* hops, void* stream);
+
+/* opp_par_loop_all__deposit_charge_on_nodes_kernel
+ * (apps_codegen/app_fempic_cg/hip/deposit_charge_on_nodes_kernel_loop.hpp:39-95, 208-222;
+ * elemental apps/app_fempic/kernels.h:164-175): node_charge[c2n[p2c][k]] += lc[k], FP64 atomics.
+ * Reads the live count from counters->size. */
+int oppx_fempic_deposit_atomic(int64_t n_max, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh,
+                               double* node_charge, oppx_move_counters* counters, void* stream);
+
+/* Fused step: push + (direct hop) + walk + FP64 atomic deposit in one pass over the particles.
+ * Same per-particle results as the three loops above in sequence; node_charge (nullable)
+ * is accumulated, not cleared. */
+int oppx_fempic_step_fused(int64_t n_max, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh,
+                           const oppx_dh_grid* dh, double k, double dt, oppx_move_counters* counters,
+                           uint64_t* hole_bits, int32_t* hops, double* node_charge, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * NESO-advection hot path (device)
+ * ------------------------------------------------------------------------------------- */
+
+/* Per-quad record of the 2-D walk: lower-left corner (c_pos_ll) and neighbours
+ * {x-1, x+1, y-1, y+1} (c2c), 32 B. */
+typedef struct oppx_quad_cell {
+    double llx, lly;
+    int32_t c2c[4];
+} oppx_quad_cell;
+
+/* NESO particle state (device pointers, SoA with `stride`). */
+typedef struct oppx_neso_parts {
+    int64_t stride;
+    double* pos;               /* [2*stride] */
+    const double* vel;         /* [2*stride] */
+    int32_t* mdir;             /* [2*stride] move direction (OPP_RW dat p_mdir) */
+    int32_t* p2c;              /* [stride] */
+} oppx_neso_parts;
+
+/* Pack c_pos_ll [2n] and c2c [4n] (device) into quad records. */
+int oppx_quad_cells_build(const double* c_pos_ll, const int32_t* c2c, int64_t n_cells, oppx_quad_cell* cells,
+                          void* stream);
+
+/* opp_par_loop_all__update_pos_kernel (do_update != 0; apps/app_neso_advection/kernels.h:41-54) then
+ * opp_particle_move__move_kernel (apps/app_neso_advection/kernels.h:56-113;
+ * apps_codegen/app_neso_advection_mdir_cg/hip/move_kernel_loop.hpp:166-420) over counters->size particles.
+ * Hop statistics go to counters; `hops` (nullable) receives per-particle hop counts. */
+int oppx_neso_step(int64_t n_max, const oppx_neso_parts* parts, const oppx_quad_cell* cells, int64_t n_cells,
+                   double dt, double cell_width, double ext_x, double ext_y, int do_update,
+                   oppx_move_counters* counters, int32_t* hops, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Compaction
+ * ------------------------------------------------------------------------------------- */
+
+/* 64-bit words the removal bitmap of a move over n_max particles needs (whole blocks of 256). */
+int64_t oppx_hole_bits_words(int64_t n_max);
+
+/* Scratch needed by oppx_holefill for n_max particles (bytes). */
+size_t oppx_holefill_scratch_bytes(int64_t n_max);
+
+/* Hole-fill of opp_finalize_particle_move (opp_lib/src/core/opp_particle_organize_core.cpp:124-218;
+ * omp opp_lib/src/omp/opp_particle_mover.cpp:153-218; hip opp_lib/src/hip/opp_particle_sorter.cpp:271-359):
+ * the k-th hole (ascending) below the new size receives the k-th live record (descending)
+ * above it, for every dat in `dats` (host array of n_dats <= 16 SoA descriptors sharing `stride`). Updates
+ * counters->size -= removed and counters->holes_low. */
+int oppx_holefill(int64_t n_max, int64_t stride, oppx_move_counters* counters, const uint64_t* hole_bits,
+                  const oppx_dat_desc* dats, int n_dats, void* scratch, size_t scratch_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPPX_H */

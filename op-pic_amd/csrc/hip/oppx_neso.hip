@@ -1,0 +1,186 @@
+// NESO-advection hot path on gfx950: periodic drift + axis-wise multi-hop walk over a 2-D quad
+// mesh, one lane per particle.
+//
+// Elemental math: apps/app_neso_advection/kernels.h:41-113; loops:
+// apps_codegen/app_neso_advection_mdir_cg/{seq,hip}/{update_pos,move}_kernel_loop.hpp. The reference's
+// GCC build contracts pos += vel*dt, pos + n*extent and FMOD's a - b*trunc(a/b) into FMAs; they
+// are explicit fma() here (file built with -ffp-contract=off). OPP_DO_ONCE holds on the first
+// hop only.
+
+#include "oppx_internal.h"
+
+namespace oppx {
+namespace {
+
+constexpr int kBlock = 256;
+
+struct NesoArgs {
+    int64_t n_cells;
+    const oppx_quad_cell* __restrict__ cells;
+    int64_t stride;
+    double* __restrict__ pos;
+    const double* __restrict__ vel;
+    int32_t* __restrict__ mdir;
+    int32_t* __restrict__ p2c;
+    double dt, cw, ext_x, ext_y;
+    int64_t n_max;
+    oppx_move_counters* ctr;
+    int32_t* __restrict__ hops;
+};
+
+__device__ __forceinline__ double periodic_update(double p, double v, double dt, double ext) {
+    p = fma(v, dt, p);
+    const int off = (int)(fabs(p) + 2.0);
+    const double t = fma((double)off, ext, p);
+    return fma(-ext, trunc(t / ext), t);
+}
+
+template <bool STATS>
+__global__ void __launch_bounds__(kBlock) neso_step_kernel(NesoArgs P, uint32_t nblocks, int do_update) {
+    const uint32_t b = xcd_block(blockIdx.x, nblocks);
+    const int64_t i = (int64_t)b * kBlock + threadIdx.x;
+    int64_t n = P.ctr->size;
+    n = n < 0 ? 0 : (n > P.n_max ? P.n_max : n);
+    const bool live = i < n;
+    const int64_t S = P.stride;
+    int h = 0;
+    bool fault = false;
+    if (live) {
+        double x = P.pos[i], y = P.pos[S + i];
+        if (do_update) {
+            x = periodic_update(x, P.vel[i], P.dt, P.ext_x);
+            y = periodic_update(y, P.vel[S + i], P.dt, P.ext_y);
+            P.pos[i] = x;
+            P.pos[S + i] = y;
+        }
+        int cell = P.p2c[i];
+        if (cell != kMaxCell) {
+            if (!((unsigned)cell < (unsigned)P.n_cells)) {
+                fault = true;
+            } else {
+                int mx = P.mdir[i], my = P.mdir[S + i];
+                const double cw = P.cw;
+                for (;;) {
+                    const oppx_quad_cell q = P.cells[cell];
+                    const double dx = x - q.llx, dy = y - q.lly;
+                    h++;
+                    if (h == 1) {
+                        const bool xp = ((dx > cw) && (P.ext_x > 2 * dx - cw)) || ((dx < 0) && (P.ext_x < cw - 2 * dx));
+                        const bool yp = ((dy > cw) && (P.ext_y > 2 * dy - cw)) || ((dy < 0) && (P.ext_y < cw - 2 * dy));
+                        mx = xp ? 1 : -1;
+                        my = yp ? 1 : -1;
+                    }
+                    int next = 0;
+                    bool hop = false;
+                    if ((dx >= 0.0) && (dx <= cw)) mx = 0;
+                    else if (mx > 0) { next = q.c2c[1]; hop = true; }
+                    else if (mx < 0) { next = q.c2c[0]; hop = true; }
+                    if (!hop) {
+                        if ((dy >= 0.0) && (dy <= cw)) my = 0;
+                        else if (my > 0) { next = q.c2c[3]; hop = true; }
+                        else if (my < 0) { next = q.c2c[2]; hop = true; }
+                    }
+                    if (!hop) break;  // MOVE_DONE
+                    if (!((unsigned)next < (unsigned)P.n_cells) || h >= OPPX_MAX_HOPS) {
+                        fault = true;
+                        break;
+                    }
+                    cell = next;
+                }
+                P.mdir[i] = mx;
+                P.mdir[S + i] = my;
+                if (!fault) P.p2c[i] = cell;
+            }
+        }
+        if (P.hops) P.hops[i] = h;
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const unsigned long long fmask = __ballot(fault);
+    if (lane == 0 && fmask) atomicAdd(&P.ctr->faults, (int)__popcll(fmask));
+    if (STATS) {
+        __shared__ long long s_sum[kBlock / kWave];
+        __shared__ int s_max[kBlock / kWave];
+        const long long hs = wave_sum_i64(h);
+        const int hm = wave_max_i32(h);
+        const int w = threadIdx.x / kWave;
+        if (lane == 0) { s_sum[w] = hs; s_max[w] = hm; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long t = 0;
+            int m = 0;
+            for (int q = 0; q < kBlock / kWave; q++) { t += s_sum[q]; m = m > s_max[q] ? m : s_max[q]; }
+            if (t) atomicAdd((unsigned long long*)&P.ctr->total_hops, (unsigned long long)t);
+            if (m) atomicMax(&P.ctr->max_hops, m);
+        }
+    }
+}
+
+__global__ void neso_reset_kernel(oppx_move_counters* ctr) {
+    ctr->removed = 0;
+    ctr->holes_low = 0;
+    ctr->max_hops = 0;
+    ctr->total_hops = 0;
+}
+
+__global__ void quad_cells_kernel(const double* __restrict__ ll, const int32_t* __restrict__ c2c, int64_t n,
+                                  oppx_quad_cell* __restrict__ out) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    oppx_quad_cell q;
+    q.llx = ll[2 * c];
+    q.lly = ll[2 * c + 1];
+    for (int k = 0; k < 4; k++) {
+        const int v = c2c[4 * c + k];
+        q.c2c[k] = (v >= 0 && v < n) ? v : -2;  // invalid neighbours are rejected by the walk as faults
+    }
+    out[c] = q;
+}
+
+}  // namespace
+}  // namespace oppx
+
+using namespace oppx;
+
+extern "C" {
+
+int oppx_quad_cells_build(const double* c_pos_ll, const int32_t* c2c, int64_t n_cells, oppx_quad_cell* cells,
+                          void* stream) {
+    OPPX_ARG_CHECK(c_pos_ll && c2c && cells && n_cells > 0 && n_cells < kMaxCell, "oppx_quad_cells_build: bad args");
+    quad_cells_kernel<<<grid_for(n_cells, 256), 256, 0, as_stream(stream)>>>(c_pos_ll, c2c, n_cells, cells);
+    OPPX_LAUNCH_CHECK("quad_cells");
+    return OPPX_OK;
+}
+
+int oppx_neso_step(int64_t n_max, const oppx_neso_parts* parts, const oppx_quad_cell* cells, int64_t n_cells,
+                   double dt, double cell_width, double ext_x, double ext_y, int do_update,
+                   oppx_move_counters* counters, int32_t* hops, void* stream) {
+    OPPX_ARG_CHECK(parts && cells && counters, "oppx_neso_step: null pointer");
+    OPPX_ARG_CHECK(n_cells > 0 && n_cells < kMaxCell, "oppx_neso_step: n_cells out of range");
+    OPPX_ARG_CHECK(n_max >= 0 && n_max <= parts->stride, "oppx_neso_step: n_max beyond capacity");
+    OPPX_ARG_CHECK(cell_width > 0 && ext_x > 0 && ext_y > 0, "oppx_neso_step: widths must be positive");
+    hipStream_t s = as_stream(stream);
+    neso_reset_kernel<<<1, 1, 0, s>>>(counters);
+    OPPX_LAUNCH_CHECK("neso_reset");
+    if (n_max == 0) return OPPX_OK;
+    NesoArgs P{};
+    P.n_cells = n_cells;
+    P.cells = cells;
+    P.stride = parts->stride;
+    P.pos = parts->pos;
+    P.vel = parts->vel;
+    P.mdir = parts->mdir;
+    P.p2c = parts->p2c;
+    P.dt = dt;
+    P.cw = cell_width;
+    P.ext_x = ext_x;
+    P.ext_y = ext_y;
+    P.n_max = n_max;
+    P.ctr = counters;
+    P.hops = hops;
+    const unsigned g = grid_for(n_max, kBlock);
+    neso_step_kernel<true><<<g, kBlock, 0, s>>>(P, g, do_update);
+    OPPX_LAUNCH_CHECK("neso_step");
+    return OPPX_OK;
+}
+
+}  // extern "C"

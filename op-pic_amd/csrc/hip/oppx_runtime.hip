@@ -1,0 +1,134 @@
+// Device runtime of the oppx C-ABI: initialisation, errors, memory, AoS<->SoA transposes.
+// Replaces opp_lib/src/hip/opp_hip.cpp:221-271 (device init) and the host-side layout
+// conversions of opp_lib/src/hip/opp_hip_utils.cpp:35-153, which here run on the device.
+
+#include "oppx_internal.h"
+
+namespace oppx {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+
+// One thread per (element, component); reads/writes coalesced on the SoA side and
+// contiguous on the AoS side.
+template <typename T>
+__global__ void aos_to_soa_kernel(const T* __restrict__ aos, T* __restrict__ soa, int64_t n, int dim,
+                                  int64_t stride) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * dim) return;
+    const int64_t i = t / dim, d = t % dim;
+    soa[d * stride + i] = aos[t];
+}
+
+template <typename T>
+__global__ void soa_to_aos_kernel(const T* __restrict__ soa, T* __restrict__ aos, int64_t n, int dim,
+                                  int64_t stride) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * dim) return;
+    const int64_t i = t / dim, d = t % dim;
+    aos[t] = soa[d * stride + i];
+}
+
+}  // namespace
+}  // namespace oppx
+
+using namespace oppx;
+
+extern "C" {
+
+const char* oppx_last_error(void) { return g_last_error.c_str(); }
+
+int oppx_init(int device) {
+    int count = 0;
+    OPPX_HIP_TRY(hipGetDeviceCount(&count));
+    OPPX_ARG_CHECK(count > 0, "oppx_init: no HIP device visible");
+    OPPX_ARG_CHECK(device >= 0 && device < count, "oppx_init: device index out of range");
+    OPPX_HIP_TRY(hipSetDevice(device));
+    return OPPX_OK;
+}
+
+int oppx_device_malloc(void** ptr, size_t bytes) {
+    OPPX_ARG_CHECK(ptr != nullptr, "oppx_device_malloc: null out pointer");
+    *ptr = nullptr;
+    if (bytes == 0) return OPPX_OK;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        *ptr = nullptr;
+        return OPPX_ERR_NOMEM;
+    }
+    return OPPX_OK;
+}
+
+int oppx_device_free(void* ptr) {
+    if (ptr) OPPX_HIP_TRY(hipFree(ptr));
+    return OPPX_OK;
+}
+
+int oppx_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes == 0) return OPPX_OK;
+    OPPX_ARG_CHECK(dst && src, "oppx_memcpy_htod: null pointer");
+    OPPX_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    OPPX_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return OPPX_OK;
+}
+
+int oppx_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes == 0) return OPPX_OK;
+    OPPX_ARG_CHECK(dst && src, "oppx_memcpy_dtoh: null pointer");
+    OPPX_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+    OPPX_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return OPPX_OK;
+}
+
+int oppx_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes == 0) return OPPX_OK;
+    OPPX_ARG_CHECK(dst && src, "oppx_memcpy_dtod: null pointer");
+    OPPX_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+    return OPPX_OK;
+}
+
+int oppx_memset_zero(void* dst, size_t bytes, void* stream) {
+    if (bytes == 0) return OPPX_OK;
+    OPPX_ARG_CHECK(dst, "oppx_memset_zero: null pointer");
+    OPPX_HIP_TRY(hipMemsetAsync(dst, 0, bytes, as_stream(stream)));
+    return OPPX_OK;
+}
+
+int oppx_stream_sync(void* stream) {
+    OPPX_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return OPPX_OK;
+}
+
+int oppx_aos_to_soa(const void* aos, void* soa, int64_t n, int dim, int64_t stride, int elem_bytes, void* stream) {
+    OPPX_ARG_CHECK(n >= 0 && dim > 0 && stride >= n, "oppx_aos_to_soa: bad sizes");
+    OPPX_ARG_CHECK(elem_bytes == 4 || elem_bytes == 8, "oppx_aos_to_soa: elem_bytes must be 4 or 8");
+    if (n == 0) return OPPX_OK;
+    OPPX_ARG_CHECK(aos && soa, "oppx_aos_to_soa: null pointer");
+    const unsigned g = grid_for(n * dim, 256);
+    if (elem_bytes == 8)
+        aos_to_soa_kernel<double><<<g, 256, 0, as_stream(stream)>>>((const double*)aos, (double*)soa, n, dim, stride);
+    else
+        aos_to_soa_kernel<int32_t><<<g, 256, 0, as_stream(stream)>>>((const int32_t*)aos, (int32_t*)soa, n, dim, stride);
+    OPPX_LAUNCH_CHECK("aos_to_soa");
+    return OPPX_OK;
+}
+
+int oppx_soa_to_aos(const void* soa, void* aos, int64_t n, int dim, int64_t stride, int elem_bytes, void* stream) {
+    OPPX_ARG_CHECK(n >= 0 && dim > 0 && stride >= n, "oppx_soa_to_aos: bad sizes");
+    OPPX_ARG_CHECK(elem_bytes == 4 || elem_bytes == 8, "oppx_soa_to_aos: elem_bytes must be 4 or 8");
+    if (n == 0) return OPPX_OK;
+    OPPX_ARG_CHECK(aos && soa, "oppx_soa_to_aos: null pointer");
+    const unsigned g = grid_for(n * dim, 256);
+    if (elem_bytes == 8)
+        soa_to_aos_kernel<double><<<g, 256, 0, as_stream(stream)>>>((const double*)soa, (double*)aos, n, dim, stride);
+    else
+        soa_to_aos_kernel<int32_t><<<g, 256, 0, as_stream(stream)>>>((const int32_t*)soa, (int32_t*)aos, n, dim, stride);
+    OPPX_LAUNCH_CHECK("soa_to_aos");
+    return OPPX_OK;
+}
+
+}  // extern "C"

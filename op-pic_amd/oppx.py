@@ -1,0 +1,410 @@
+"""Python binding of liboppx (the C-ABI in include/oppx.h) over ctypes.
+
+Host builders return numpy arrays in the reference host layout (AoS). Device state lives in
+torch tensors (torch is used only for device memory, streams and events); every compute call
+goes to a gfx950 kernel in liboppx.so, and a missing library raises instead of falling back.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, Structure, c_double, c_int, c_int32, c_int64, c_size_t, c_uint64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboppx.so")
+
+MAX_CELL_INDEX = 2147483647
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE = 0, 1, 2, 3, 4
+
+
+class TetAux(Structure):
+    _fields_ = [("c2", c_double), ("c2c", c_int32 * 4), ("pad", c_int32 * 2)]
+
+
+class TetMesh(Structure):
+    _fields_ = [("n_cells", c_int64), ("n_nodes", c_int64), ("det", c_void_p), ("aux", c_void_p),
+                ("c2n", c_void_p), ("ef", c_void_p)]
+
+
+class DhGrid(Structure):
+    _fields_ = [("cell", c_void_p), ("dims", c_int64 * 3), ("min", c_double * 3), ("inv_gs", c_double)]
+
+
+class FempicParts(Structure):
+    _fields_ = [("stride", c_int64), ("pos", c_void_p), ("vel", c_void_p), ("lc", c_void_p), ("p2c", c_void_p)]
+
+
+class MoveCounters(Structure):
+    _fields_ = [("size", c_int32), ("removed", c_int32), ("holes_low", c_int32), ("max_hops", c_int32),
+                ("total_hops", c_int64), ("faults", c_int32), ("pad", c_int32)]
+
+
+class QuadCell(Structure):
+    _fields_ = [("llx", c_double), ("lly", c_double), ("c2c", c_int32 * 4)]
+
+
+class NesoParts(Structure):
+    _fields_ = [("stride", c_int64), ("pos", c_void_p), ("vel", c_void_p), ("mdir", c_void_p), ("p2c", c_void_p)]
+
+
+class DatDesc(Structure):
+    _fields_ = [("base", c_void_p), ("dim", c_int32), ("elem_bytes", c_int32)]
+
+
+assert ctypes.sizeof(TetAux) == 32 and ctypes.sizeof(MoveCounters) == 32
+
+# name -> (restype, argtypes): every symbol include/oppx.h declares
+_SIGS = {
+    "oppx_init": (c_int, [c_int]),
+    "oppx_last_error": (ctypes.c_char_p, []),
+    "oppx_device_malloc": (c_int, [POINTER(c_void_p), c_size_t]),
+    "oppx_device_free": (c_int, [c_void_p]),
+    "oppx_memcpy_htod": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
+    "oppx_memcpy_dtoh": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
+    "oppx_memcpy_dtod": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
+    "oppx_memset_zero": (c_int, [c_void_p, c_size_t, c_void_p]),
+    "oppx_stream_sync": (c_int, [c_void_p]),
+    "oppx_aos_to_soa": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int64, c_int, c_void_p]),
+    "oppx_soa_to_aos": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int64, c_int, c_void_p]),
+    "oppx_box_mesh_sizes": (c_int, [c_int, c_int, c_int, POINTER(c_int64), POINTER(c_int64)]),
+    "oppx_box_mesh_build": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double,
+                                    c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "oppx_box_field_sines": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "oppx_box_particles_uniform": (c_int, [c_int64, c_uint64, c_double, c_int, c_int, c_int,
+                                           c_double, c_double, c_double, c_void_p, c_void_p, c_void_p]),
+    "oppx_dh_grid_dims": (c_int, [c_void_p, c_void_p, c_double, c_void_p]),
+    "oppx_dh_build_overlay_nodes": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                            c_void_p, c_void_p, c_double, c_void_p]),
+    "oppx_neso_mesh": (c_int, [c_int, c_int, c_double, c_void_p, c_void_p]),
+    "oppx_neso_particles": (c_int, [c_int64, c_uint64, c_int, c_int, c_double, c_double, c_void_p, c_void_p,
+                                    c_void_p]),
+    "oppx_tet_aux_build": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
+    "oppx_fempic_push": (c_int, [c_int64, POINTER(FempicParts), POINTER(TetMesh), c_double, c_double,
+                                 c_void_p, c_void_p]),
+    "oppx_fempic_move": (c_int, [c_int64, POINTER(FempicParts), POINTER(TetMesh), POINTER(DhGrid),
+                                 c_void_p, c_void_p, c_void_p, c_void_p]),
+    "oppx_fempic_deposit_atomic": (c_int, [c_int64, POINTER(FempicParts), POINTER(TetMesh), c_void_p,
+                                           c_void_p, c_void_p]),
+    "oppx_fempic_step_fused": (c_int, [c_int64, POINTER(FempicParts), POINTER(TetMesh), POINTER(DhGrid),
+                                       c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "oppx_quad_cells_build": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
+    "oppx_neso_step": (c_int, [c_int64, POINTER(NesoParts), c_void_p, c_int64, c_double, c_double, c_double, c_double,
+                               c_int, c_void_p, c_void_p, c_void_p]),
+    "oppx_hole_bits_words": (c_int64, [c_int64]),
+    "oppx_holefill_scratch_bytes": (c_size_t, [c_int64]),
+    "oppx_holefill": (c_int, [c_int64, c_int64, c_void_p, c_void_p, POINTER(DatDesc), c_int, c_void_p,
+                              c_size_t, c_void_p]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load liboppx.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        # One HIP runtime per process: torch bundles libamdhip64 (SONAME libamdhip64.so.7). Loading
+        # torch first makes liboppx.so's libamdhip64.so.7 dependency bind to that same copy.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    return list(_SIGS)
+
+
+class OppxError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str = ""):
+    if rc != OK:
+        msg = lib().oppx_last_error().decode(errors="replace")
+        raise OppxError(f"{what} failed (rc={rc}): {msg}")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(c_void_p)
+
+
+# ---------------------------------------------------------------------------------------------
+# host builders
+# ---------------------------------------------------------------------------------------------
+
+def box_mesh(nx: int, ny: int, nz: int, lx=1.0, ly=1.0, lz=1.0) -> dict:
+    """Box of nx*ny*nz cubes x 6 tets with the FEM geometry the mover consumes."""
+    L = lib()
+    nn, nc = c_int64(), c_int64()
+    check(L.oppx_box_mesh_sizes(nx, ny, nz, ctypes.byref(nn), ctypes.byref(nc)), "box_mesh_sizes")
+    nn, nc = nn.value, nc.value
+    m = dict(nx=nx, ny=ny, nz=nz, lx=lx, ly=ly, lz=lz,
+             node_pos=np.zeros((nn, 3)), c2n=np.zeros((nc, 4), np.int32), c2c=np.zeros((nc, 4), np.int32),
+             c_det=np.zeros((nc, 16)), c_vol=np.zeros(nc))
+    check(L.oppx_box_mesh_build(nx, ny, nz, lx, ly, lz, _ptr(m["node_pos"]), _ptr(m["c2n"]), _ptr(m["c2c"]),
+                                _ptr(m["c_det"]), _ptr(m["c_vol"])), "box_mesh_build")
+    m["n_nodes"], m["n_cells"] = nn, nc
+    return m
+
+
+def box_field_sines(m: dict) -> np.ndarray:
+    ef = np.zeros((m["n_cells"], 3))
+    check(lib().oppx_box_field_sines(m["n_nodes"], m["n_cells"], _ptr(m["node_pos"]), _ptr(m["c2n"]),
+                                     _ptr(m["c_det"]), _ptr(m["c_vol"]), _ptr(ef)), "box_field_sines")
+    return ef
+
+
+def box_particles(n: int, seed: int, sigma: float, m: dict):
+    pos, vel = np.zeros((n, 3)), np.zeros((n, 3))
+    p2c = np.zeros(n, np.int32)
+    check(lib().oppx_box_particles_uniform(n, seed, sigma, m["nx"], m["ny"], m["nz"], m["lx"], m["ly"], m["lz"],
+                                           _ptr(pos), _ptr(vel), _ptr(p2c)), "box_particles_uniform")
+    return pos, vel, p2c
+
+
+def dh_dims(bmin, bmax, gs):
+    bmin = np.ascontiguousarray(bmin, np.float64)
+    bmax = np.ascontiguousarray(bmax, np.float64)
+    dims = np.zeros(3, np.int64)
+    check(lib().oppx_dh_grid_dims(_ptr(bmin), _ptr(bmax), gs, _ptr(dims)), "dh_grid_dims")
+    return dims
+
+
+def dh_overlay(m: dict, bmin, bmax, gs) -> np.ndarray:
+    """Direct-hop overlay (owner tet per structured box, flattened x-fastest)."""
+    bmin = np.ascontiguousarray(bmin, np.float64)
+    bmax = np.ascontiguousarray(bmax, np.float64)
+    dims = dh_dims(bmin, bmax, gs)
+    ov = np.zeros(int(np.prod(dims)), np.int32)
+    check(lib().oppx_dh_build_overlay_nodes(m["n_cells"], _ptr(m["c_det"]), _ptr(m["c_vol"]), None,
+                                            _ptr(m["c2n"]), _ptr(m["node_pos"]), _ptr(bmin), _ptr(bmax), gs,
+                                            _ptr(ov)), "dh_build_overlay")
+    return ov
+
+
+def neso_mesh(nx: int, ny: int, cell_width: float) -> dict:
+    """2-D periodic quad mesh of the NESO-advection app."""
+    nc = nx * ny
+    m = dict(nx=nx, ny=ny, cell_width=cell_width, n_cells=nc, c_pos_ll=np.zeros((nc, 2)),
+             c2c=np.zeros((nc, 4), np.int32))
+    check(lib().oppx_neso_mesh(nx, ny, cell_width, _ptr(m["c_pos_ll"]), _ptr(m["c2c"])), "neso_mesh")
+    return m
+
+
+def neso_particles(n: int, seed: int, m: dict, vmax: float):
+    pos, vel = np.zeros((n, 2)), np.zeros((n, 2))
+    p2c = np.zeros(n, np.int32)
+    check(lib().oppx_neso_particles(n, seed, m["nx"], m["ny"], m["cell_width"], vmax, _ptr(pos), _ptr(vel),
+                                    _ptr(p2c)), "neso_particles")
+    return pos, vel, p2c
+
+
+def sort_particles_by_cell(pos, vel, p2c):
+    """Stable reorder of an input population by containing cell (an input-layout choice)."""
+    order = np.argsort(p2c, kind="stable")
+    return pos[order], vel[order], p2c[order]
+
+
+# ---------------------------------------------------------------------------------------------
+# device state of one FEM-PIC instance
+# ---------------------------------------------------------------------------------------------
+
+class FempicDevice:
+    """Mesh + particles of one FEM-PIC instance resident on the GPU, driven through the C-ABI."""
+
+    def __init__(self, mesh: dict, c_ef: np.ndarray, pos: np.ndarray, vel: np.ndarray, p2c: np.ndarray,
+                 capacity: int | None = None, overlay: np.ndarray | None = None, bmin=None, gs: float | None = None,
+                 bmax=None, device: str = "cuda"):
+        import torch
+        self.torch = torch
+        self.dev = torch.device(device)
+        L = lib()
+        check(L.oppx_init(self.dev.index or 0), "oppx_init")
+        n = int(pos.shape[0])
+        cap = int(capacity) if capacity is not None else n
+        if cap < n:
+            raise ValueError("capacity below particle count")
+        self.n0, self.cap = n, cap
+        t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(self.dev, dtype=dt)
+        nc, nn = mesh["n_cells"], mesh["n_nodes"]
+        # host-side validation: every index a kernel dereferences must be in range
+        if not ((mesh["c2n"] >= 0).all() and (mesh["c2n"] < nn).all()):
+            raise ValueError("c2n entries out of [0, n_nodes)")
+        if not ((mesh["c2c"] >= -1).all() and (mesh["c2c"] < nc).all()):
+            raise ValueError("c2c entries out of [-1, n_cells)")
+        if n and not ((p2c >= 0).all() and (p2c < nc).all()):
+            raise ValueError("initial p2c entries out of [0, n_cells)")
+        self.nc, self.nn = nc, nn
+        self.det = t(mesh["c_det"].reshape(-1), torch.float64)
+        self.c_vol = t(mesh["c_vol"], torch.float64)
+        self.c2c = t(mesh["c2c"].reshape(-1), torch.int32)
+        self.c2n = t(mesh["c2n"].reshape(-1), torch.int32)
+        self.ef = t(c_ef.reshape(-1), torch.float64)
+        self.aux = torch.zeros(nc * 4, dtype=torch.float64, device=self.dev)  # 32 B per cell
+        check(L.oppx_tet_aux_build(self.c_vol.data_ptr(), self.c2c.data_ptr(), nc, self.aux.data_ptr(),
+                                   self.stream()), "tet_aux_build")
+        self.mesh = TetMesh(nc, nn, self.det.data_ptr(), self.aux.data_ptr(), self.c2n.data_ptr(),
+                            self.ef.data_ptr())
+        # particles: SoA with stride = capacity
+        self.pos = torch.zeros((3, cap), dtype=torch.float64, device=self.dev)
+        self.vel = torch.zeros((3, cap), dtype=torch.float64, device=self.dev)
+        self.lc = torch.zeros((4, cap), dtype=torch.float64, device=self.dev)
+        self.p2c = torch.full((cap,), MAX_CELL_INDEX, dtype=torch.int32, device=self.dev)
+        self.pos[:, :n] = t(pos.T, torch.float64)
+        self.vel[:, :n] = t(vel.T, torch.float64)
+        self.p2c[:n] = t(p2c, torch.int32)
+        self.parts = FempicParts(cap, self.pos.data_ptr(), self.vel.data_ptr(), self.lc.data_ptr(),
+                                 self.p2c.data_ptr())
+        self.counters = torch.zeros(8, dtype=torch.int32, device=self.dev)
+        self.counters[0] = n
+        self.hole_bits = torch.zeros(int(L.oppx_hole_bits_words(cap)), dtype=torch.int64, device=self.dev)
+        self.scratch_bytes = int(L.oppx_holefill_scratch_bytes(cap))
+        self.scratch = torch.zeros(self.scratch_bytes, dtype=torch.uint8, device=self.dev)
+        self.charge = torch.zeros(nn, dtype=torch.float64, device=self.dev)
+        self.hops = torch.zeros(cap, dtype=torch.int32, device=self.dev)
+        self.dh = None
+        if overlay is not None:
+            dims = dh_dims(bmin, bmax, gs)
+            if overlay.size != int(np.prod(dims)):
+                raise ValueError("overlay size does not match the grid dims")
+            if not (((overlay >= 0) & (overlay < nc)) | (overlay == MAX_CELL_INDEX)).all():
+                raise ValueError("overlay entries must be cells or MAX_CELL_INDEX")
+            self.overlay = t(overlay, torch.int32)
+            self.dh = DhGrid(self.overlay.data_ptr(), (c_int64 * 3)(*dims), (c_double * 3)(*bmin), 1.0 / gs)
+        self.dats = (DatDesc * 4)(DatDesc(self.pos.data_ptr(), 3, 8), DatDesc(self.vel.data_ptr(), 3, 8),
+                                  DatDesc(self.lc.data_ptr(), 4, 8), DatDesc(self.p2c.data_ptr(), 1, 4))
+        self.n_max = n  # upper bound of the live count (sizes only shrink without injection)
+
+    def stream(self):
+        return self.torch.cuda.current_stream(self.dev).cuda_stream
+
+    def _dh_ref(self):
+        return ctypes.byref(self.dh) if self.dh is not None else None
+
+    # -- the reference loops, one C-ABI call each --------------------------------------------
+    def push(self, charge, mass, dt):
+        k = charge / mass * dt
+        check(lib().oppx_fempic_push(self.n_max, ctypes.byref(self.parts), ctypes.byref(self.mesh), k, dt,
+                                     self.counters.data_ptr(), self.stream()), "fempic_push")
+
+    def move(self, record_hops=True):
+        check(lib().oppx_fempic_move(self.n_max, ctypes.byref(self.parts), ctypes.byref(self.mesh), self._dh_ref(),
+                                     self.counters.data_ptr(), self.hole_bits.data_ptr(),
+                                     self.hops.data_ptr() if record_hops else None, self.stream()), "fempic_move")
+
+    def holefill(self):
+        check(lib().oppx_holefill(self.n_max, self.cap, self.counters.data_ptr(), self.hole_bits.data_ptr(),
+                                  self.dats, 4, self.scratch.data_ptr(), self.scratch_bytes, self.stream()),
+              "holefill")
+
+    def deposit(self):
+        check(lib().oppx_fempic_deposit_atomic(self.n_max, ctypes.byref(self.parts), ctypes.byref(self.mesh),
+                                               self.charge.data_ptr(), self.counters.data_ptr(), self.stream()),
+              "fempic_deposit")
+
+    def step_fused(self, charge, mass, dt, record_hops=False):
+        """push + direct hop + walk + deposit in one pass, then hole-fill."""
+        self.charge.zero_()
+        k = charge / mass * dt
+        check(lib().oppx_fempic_step_fused(self.n_max, ctypes.byref(self.parts), ctypes.byref(self.mesh),
+                                           self._dh_ref(), k, dt, self.counters.data_ptr(),
+                                           self.hole_bits.data_ptr(), self.hops.data_ptr() if record_hops else None,
+                                           self.charge.data_ptr(), self.stream()), "fempic_step_fused")
+        self.holefill()
+
+    def step_unfused(self, charge, mass, dt, record_hops=True):
+        """The reference's loop order: reset, push, move (+hole-fill), deposit."""
+        self.charge.zero_()
+        self.push(charge, mass, dt)
+        self.move(record_hops)
+        self.holefill()
+        self.deposit()
+
+    # -- readback ------------------------------------------------------------------------------
+    def read_counters(self):
+        c = self.counters.cpu().numpy()
+        return dict(size=int(c[0]), removed=int(c[1]), holes_low=int(c[2]), max_hops=int(c[3]),
+                    total_hops=int(c[4:6].view(np.int64)[0]), faults=int(c[6]))
+
+    def sync_size(self):
+        """Host copy of the live count (the D2H of opp_finalize_particle_move)."""
+        self.n_max = self.read_counters()["size"]
+        return self.n_max
+
+    def download(self):
+        n = self.read_counters()["size"]
+        return dict(n=n, pos=self.pos[:, :n].T.cpu().numpy().copy(), vel=self.vel[:, :n].T.cpu().numpy().copy(),
+                    lc=self.lc[:, :n].T.cpu().numpy().copy(), p2c=self.p2c[:n].cpu().numpy().copy(),
+                    charge=self.charge.cpu().numpy().copy())
+
+
+class NesoDevice:
+    """NESO-advection particles + periodic quad mesh resident on the GPU (C-ABI driven)."""
+
+    def __init__(self, mesh: dict, pos: np.ndarray, vel: np.ndarray, p2c: np.ndarray, mdir=None, device="cuda"):
+        import torch
+        self.torch = torch
+        self.dev = torch.device(device)
+        L = lib()
+        check(L.oppx_init(self.dev.index or 0), "oppx_init")
+        nc = mesh["n_cells"]
+        if not ((mesh["c2c"] >= 0).all() and (mesh["c2c"] < nc).all()):
+            raise ValueError("c2c entries out of [0, n_cells)")
+        if pos.shape[0] and not ((p2c >= 0).all() and (p2c < nc).all()):
+            raise ValueError("p2c entries out of [0, n_cells)")
+        t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(self.dev, dtype=dt)
+        self.nc = nc
+        self.nx, self.ny, self.cw = mesh["nx"], mesh["ny"], mesh["cell_width"]
+        self.ll = t(mesh["c_pos_ll"].reshape(-1), torch.float64)
+        self.c2c = t(mesh["c2c"].reshape(-1), torch.int32)
+        self.cells = torch.zeros(nc * 4, dtype=torch.float64, device=self.dev)  # 32 B per quad
+        check(L.oppx_quad_cells_build(self.ll.data_ptr(), self.c2c.data_ptr(), nc, self.cells.data_ptr(),
+                                      self.stream()), "quad_cells_build")
+        n = pos.shape[0]
+        self.n = n
+        cap = max(n, 1)
+        self.pos = torch.zeros((2, cap), dtype=torch.float64, device=self.dev)
+        self.vel = torch.zeros((2, cap), dtype=torch.float64, device=self.dev)
+        self.mdir = torch.ones((2, cap), dtype=torch.int32, device=self.dev)
+        self.p2c = torch.zeros(cap, dtype=torch.int32, device=self.dev)
+        if n:
+            self.pos[:, :n] = t(pos.T, torch.float64)
+            self.vel[:, :n] = t(vel.T, torch.float64)
+            self.p2c[:n] = t(p2c, torch.int32)
+            if mdir is not None:
+                self.mdir[:, :n] = t(mdir.T, torch.int32)
+        self.parts = NesoParts(cap, self.pos.data_ptr(), self.vel.data_ptr(), self.mdir.data_ptr(), self.p2c.data_ptr())
+        self.counters = torch.zeros(8, dtype=torch.int32, device=self.dev)
+        self.counters[0] = n
+        self.hops = torch.zeros(cap, dtype=torch.int32, device=self.dev)
+
+    def stream(self):
+        return self.torch.cuda.current_stream(self.dev).cuda_stream
+
+    def step(self, dt, record_hops=False, do_update=True):
+        check(lib().oppx_neso_step(self.n, ctypes.byref(self.parts), self.cells.data_ptr(), self.nc, dt, self.cw,
+                                   self.nx * self.cw, self.ny * self.cw, 1 if do_update else 0,
+                                   self.counters.data_ptr(), self.hops.data_ptr() if record_hops else None,
+                                   self.stream()), "neso_step")
+
+    def read_counters(self):
+        c = self.counters.cpu().numpy()
+        return dict(size=int(c[0]), max_hops=int(c[3]), total_hops=int(c[4:6].view(np.int64)[0]), faults=int(c[6]))
+
+    def download(self):
+        n = self.n
+        return dict(pos=self.pos[:, :n].T.cpu().numpy().copy(), mdir=self.mdir[:, :n].T.cpu().numpy().copy(),
+                    p2c=self.p2c[:n].cpu().numpy().copy(), hops=self.hops[:n].cpu().numpy().copy())

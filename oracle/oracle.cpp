@@ -1,0 +1,362 @@
+// Oracle: CPU restatement of OP-PIC's particle hot path. TEST INFRASTRUCTURE ONLY: it is the
+// checker the GPU path is compared with (tests/, the smoke check, and the CPU-baseline leg of
+// bench.py); no product code links or calls it.
+//
+// Arithmetic follows the reference's GCC -O3 build, which contracts a*b+c into FMAs; the
+// contractions are written out here with std::fma and the file is built with
+// -ffp-contract=off so nothing else is fused. Each function cites the reference lines it
+// restates (paths relative to the OP-PIC source tree).
+
+#include "oracle.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace {
+
+constexpr int kMaxCell = INT_MAX;  // MAX_CELL_INDEX, opp_lib/include/opp_defs.h:150
+
+enum Status { kDone = 0, kNeedMove = 1, kNeedRemove = 2 };
+
+// Elemental move kernel, apps/app_fempic/kernels.h:106-161 (generated copy:
+// apps_codegen/app_fempic_cg/seq/move_kernel_loop.hpp:6-63). GCC contracts
+// c2*(d0 - d1*x + d2*y - d3*z) into c2*fnma(d3,z,fma(d2,y,fnma(d1,x,d0))).
+inline int fempic_move_eval(const double* pos, double* lc, const double* det, double vol, const int* c2c,
+                            int* p2c) {
+    const double c2 = (1.0 / 6.0) / vol;
+    for (int i = 0; i < 4; i++) {
+        const double* d = det + 4 * i;
+        lc[i] = c2 * std::fma(-d[3], pos[2], std::fma(d[2], pos[1], std::fma(-d[1], pos[0], d[0])));
+    }
+    if ((lc[0] > -1e-10) && ((lc[0] - 1.0) < 1e-10) && (lc[1] > -1e-10) && ((lc[1] - 1.0) < 1e-10) &&
+        (lc[2] > -1e-10) && ((lc[2] - 1.0) < 1e-10) && (lc[3] > -1e-10) && ((lc[3] - 1.0) < 1e-10))
+        return kDone;
+    int min_i = 0;
+    double min_lc = lc[0];
+    for (int i = 1; i < 4; i++)
+        if (lc[i] < min_lc) {
+            min_lc = lc[i];
+            min_i = i;
+        }
+    if (c2c[min_i] >= 0) {
+        *p2c = c2c[min_i];
+        return kNeedMove;
+    }
+    *p2c = kMaxCell;
+    return kNeedRemove;
+}
+
+// static_cast<size_t>(double) of opp_direct_hop_core.h:413-415 as x86-64 evaluates it:
+// values in (-1, 2^64) truncate toward zero; anything else (<= -1, NaN, huge) yields an
+// index no grid reaches. Returns false for the latter.
+inline bool dh_axis_index(double t, uint64_t* out) {
+    if (!(t > -1.0) || !(t < 18446744073709551616.0)) return false;
+    *out = (uint64_t)std::trunc(t < 0 ? 0.0 : t);
+    return true;
+}
+
+// Overlay lookup + omp removal rules: findStructuredCellIndex3D (opp_direct_hop_core.h:410-426),
+// opp_part_checkForGlobalMove_util (opp_lib/include/opp_omp.h:203-292, non-MPI branch).
+// Returns the new cell or kMaxCell (= remove).
+inline int dh_lookup(const double* p, const int* overlay, const int64_t dims[3], const double bmin[3],
+                     double inv_gs) {
+    uint64_t ix, iy, iz;
+    if (!dh_axis_index((p[0] - bmin[0]) * inv_gs, &ix) || !dh_axis_index((p[1] - bmin[1]) * inv_gs, &iy) ||
+        !dh_axis_index((p[2] - bmin[2]) * inv_gs, &iz))
+        return kMaxCell;
+    if (ix >= (uint64_t)dims[0] || iy >= (uint64_t)dims[1] || iz >= (uint64_t)dims[2]) return kMaxCell;
+    const uint64_t idx = ix + iy * (uint64_t)dims[0] + iz * (uint64_t)(dims[0] * dims[1]);
+    return overlay[idx];
+}
+
+int nthreads_of(int threads) { return threads > 0 ? threads : omp_get_max_threads(); }
+
+}  // namespace
+
+extern "C" {
+
+// apps_codegen/app_fempic_cg/seq/calculate_new_pos_vel_kernel_loop.hpp:5-52; elemental
+// apps/app_fempic/kernels.h:93-103. coefficient1 = charge/mass*dt; v = fma(k,E,v); x = fma(v,dt,x).
+void orc_fempic_push(int64_t n, const double* c_ef, const int* p2c, double* pos, double* vel, double charge,
+                     double mass, double dt, int threads) {
+    const double k = charge / mass * dt;
+#pragma omp parallel for schedule(static) num_threads(nthreads_of(threads))
+    for (int64_t i = 0; i < n; i++) {
+        const double* e = c_ef + 3 * (int64_t)p2c[i];
+        for (int d = 0; d < 3; d++) {
+            vel[3 * i + d] = std::fma(k, e[d], vel[3 * i + d]);
+            pos[3 * i + d] = std::fma(vel[3 * i + d], dt, pos[3 * i + d]);
+        }
+    }
+}
+
+// apps_codegen/app_fempic_cg/omp/move_kernel_loop.hpp:67-274 (global-move pass :146-170 then the
+// multi-hop pass :181-199), status rules opp_lib/include/opp_omp.h:176-199.
+int64_t orc_fempic_move(int64_t n, const int* c2c, const double* c_det, const double* c_vol, const int* overlay,
+                        const int64_t dims[3], const double bmin[3], double inv_gs, const double* pos, double* lc,
+                        int* p2c, int* hops, int threads) {
+    int64_t removed = 0;
+#pragma omp parallel for schedule(static) reduction(+ : removed) num_threads(nthreads_of(threads))
+    for (int64_t i = 0; i < n; i++) {
+        int h = 0;
+        if (overlay) {
+            p2c[i] = dh_lookup(pos + 3 * i, overlay, dims, bmin, inv_gs);
+            if (p2c[i] == kMaxCell) removed++;
+        }
+        if (p2c[i] != kMaxCell) {
+            int st;
+            do {
+                const int c = p2c[i];
+                st = fempic_move_eval(pos + 3 * i, lc + 4 * i, c_det + 16 * (int64_t)c, c_vol[c], c2c + 4 * (int64_t)c,
+                                      p2c + i);
+                h++;
+            } while (st == kNeedMove);
+            if (st == kNeedRemove) removed++;
+        }
+        if (hops) hops[i] = h;
+    }
+    return removed;
+}
+
+// seq: apps_codegen/app_fempic_cg/seq/deposit_charge_on_nodes_kernel_loop.hpp:21-60 (particle order);
+// omp: apps_codegen/app_fempic_cg/omp/deposit_charge_on_nodes_kernel_loop.hpp:21-90 with the
+// thread arrays of opp_lib/include/opp_omp.h:59-138 (thread 0 adds into the dat itself, threads
+// 1.. into zeroed copies that are then added in thread order).
+void orc_fempic_deposit(int64_t n, int64_t n_nodes, const int* c2n, const int* p2c, const double* lc,
+                        double* node_charge, int threads) {
+    const int T = nthreads_of(threads);
+    if (T <= 1) {
+        for (int64_t i = 0; i < n; i++) {
+            const int* m = c2n + 4 * (int64_t)p2c[i];
+            for (int k = 0; k < 4; k++) node_charge[m[k]] += lc[4 * i + k];
+        }
+        return;
+    }
+    std::vector<std::vector<double>> priv(T);
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        double* dst = node_charge;
+        if (t > 0) {
+            priv[t].assign(n_nodes, 0.0);
+            dst = priv[t].data();
+        }
+        const int64_t s = (n * t) / T, e = (n * (t + 1)) / T;
+        for (int64_t i = s; i < e; i++) {
+            const int* m = c2n + 4 * (int64_t)p2c[i];
+            for (int k = 0; k < 4; k++) dst[m[k]] += lc[4 * i + k];
+        }
+    }
+#pragma omp parallel for schedule(static) num_threads(T)
+    for (int64_t j = 0; j < n_nodes; j++)
+        for (int t = 1; t < T; t++) node_charge[j] += priv[t][j];
+}
+
+// particle_hole_fill_core, opp_lib/src/core/opp_particle_organize_core.cpp:124-218: walk the holes
+// upward, take the last live record from the back, stop when the two meet.
+int64_t orc_holefill(int64_t n, int64_t removed, const int* p2c_marks, void** dats, const int* dims,
+                     const int* elem_bytes, int n_dats) {
+    if (removed <= 0) return n;
+    std::vector<int> marks(p2c_marks, p2c_marks + n);  // the p2c dat is itself moved below
+    std::vector<std::pair<int64_t, int64_t>> pairs;
+    int64_t filled = 0, back = n - 1;
+    for (int64_t j = 0; j < n; j++) {
+        if (marks[j] != kMaxCell) continue;
+        while (back >= 0 && marks[back] == kMaxCell) back--;
+        if (back <= j || filled >= removed) break;
+        pairs.emplace_back(j, back);
+        back--;
+        filled++;
+    }
+    for (int d = 0; d < n_dats; d++) {
+        const size_t rec = (size_t)dims[d] * (size_t)elem_bytes[d];
+        char* base = (char*)dats[d];
+        for (const auto& pr : pairs) std::memcpy(base + pr.first * rec, base + pr.second * rec, rec);
+    }
+    return n - removed;
+}
+
+// Sort contract of this build: stable by (p2c, index) -- opp_lib/src/core/opp_particle_organize_core.cpp:98-121
+// sorts with std::sort (unstable), so only the sorted key array and per-cell multisets are
+// reference-defined.
+void orc_sort_by_cell(int64_t n, const int* p2c, void** dats, const int* dims, const int* elem_bytes, int n_dats,
+                      int64_t* perm) {
+    std::vector<int64_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::vector<int> key(p2c, p2c + n);
+    std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+    for (int d = 0; d < n_dats; d++) {
+        const size_t rec = (size_t)dims[d] * (size_t)elem_bytes[d];
+        std::vector<char> tmp((size_t)n * rec);
+        const char* src = (const char*)dats[d];
+        for (int64_t j = 0; j < n; j++) std::memcpy(tmp.data() + j * rec, src + idx[j] * rec, rec);
+        std::memcpy(dats[d], tmp.data(), (size_t)n * rec);
+    }
+    if (perm)
+        for (int64_t j = 0; j < n; j++) perm[j] = idx[j];
+}
+
+// One step of the FEM-PIC main loop restricted to the hot path
+// (apps_codegen/app_fempic_cg/fempic_opp.cpp:172-193): reset, push, move + hole-fill, deposit.
+int64_t orc_fempic_step(int64_t n, int64_t n_nodes, const int* c2n, const int* c2c, const double* c_det,
+                        const double* c_vol, const double* c_ef, const int* overlay, const int64_t dims[3],
+                        const double bmin[3], double inv_gs, double* pos, double* vel, double* lc, int* p2c,
+                        double* node_charge, double charge, double mass, double dt, int threads, int64_t* stats) {
+    std::fill(node_charge, node_charge + n_nodes, 0.0);
+    orc_fempic_push(n, c_ef, p2c, pos, vel, charge, mass, dt, threads);
+    std::vector<int> hops(stats ? n : 0);
+    const int64_t removed = orc_fempic_move(n, c2c, c_det, c_vol, overlay, dims, bmin, inv_gs, pos, lc, p2c,
+                                            stats ? hops.data() : nullptr, threads);
+    if (stats) {
+        int64_t tot = 0;
+        int mx = 0;
+        for (int64_t i = 0; i < n; i++) {
+            tot += hops[i];
+            mx = std::max(mx, hops[i]);
+        }
+        stats[0] = removed;
+        stats[1] = tot;
+        stats[2] = mx;
+        stats[3] = 0;
+    }
+    void* dats[4] = {pos, vel, lc, p2c};
+    const int dims_[4] = {3, 3, 4, 1};
+    const int eb[4] = {8, 8, 8, 4};
+    const int64_t n_new = orc_holefill(n, removed, p2c, dats, dims_, eb, 4);
+    orc_fempic_deposit(n_new, n_nodes, c2n, p2c, lc, node_charge, threads);
+    return n_new;
+}
+
+// CellMapper::generateStructuredMesh, opp_lib/src/seq/opp_direct_hop_seq.cpp:134-265 with the
+// all_cell_checker of apps_codegen/app_fempic_cg/seq/move_kernel_loop.hpp:193-217 (first cell in
+// index order whose elemental kernel reports MOVE_DONE), grid of opp_direct_hop_core.h:317-336,
+// centroid rule of opp_lib/include/opp_defs.h:121-127, corner clamp GET_VERT (opp_defs.h:115).
+int orc_dh_overlay_bruteforce(int64_t n_cells, const int* c2c, const double* c_det, const double* c_vol,
+                              const double bmin[3], const double bmax[3], double gs, int* overlay, int threads) {
+    int64_t dims[3];
+    for (int d = 0; d < 3; d++) {
+        int64_t a = 0;
+        for (double x = bmin[d]; x < bmax[d]; x += gs) a++;
+        dims[d] = a + 1;
+    }
+    const int64_t X = dims[0], XY = dims[0] * dims[1], G = dims[0] * dims[1] * dims[2];
+    auto checker = [&](const double p[3]) -> int {
+        double lc[4];
+        for (int64_t c = 0; c < n_cells; c++) {
+            int tmp = (int)c;
+            if (fempic_move_eval(p, lc, c_det + 16 * c, c_vol[c], c2c + 4 * c, &tmp) == kDone) return (int)c;
+        }
+        return kMaxCell;
+    };
+    std::vector<unsigned char> unresolved(G, 0);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads_of(threads))
+    for (int64_t idx = 0; idx < G; idx++) {
+        const int64_t dz = idx / XY, dy = (idx % XY) / X, dx = idx % X;
+        const double o[3] = {std::fma((double)dx, gs, bmin[0]), std::fma((double)dy, gs, bmin[1]),
+                             std::fma((double)dz, gs, bmin[2])};
+        double c[3];
+        for (int d = 0; d < 3; d++) c[d] = (o[d] + gs <= bmax[d]) ? o[d] + gs * 0.5 : (o[d] + bmax[d]) * 0.5;
+        int cid = checker(c);
+        if (cid == kMaxCell) { c[0] += gs / 100.0; cid = checker(c); }
+        if (cid == kMaxCell) { c[1] += gs / 100.0; cid = checker(c); }
+        if (cid == kMaxCell) { c[2] += gs / 100.0; cid = checker(c); }
+        overlay[idx] = cid;
+        unresolved[idx] = (cid == kMaxCell);
+    }
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads_of(threads))
+    for (int64_t idx = 0; idx < G; idx++) {
+        if (!unresolved[idx]) continue;
+        const int64_t dz = idx / XY, dy = (idx % XY) / X, dx = idx % X;
+        const double o[3] = {std::fma((double)dx, gs, bmin[0]), std::fma((double)dy, gs, bmin[1]),
+                             std::fma((double)dz, gs, bmin[2])};
+        auto gv = [&](int d, double v) { return (v > bmax[d]) ? bmax[d] : v; };
+        // vertex order of opp_direct_hop_seq.cpp:232-243 (the minimum is order independent)
+        int best = kMaxCell;
+        for (int kz = 0; kz < 2; kz++)
+            for (int kx = 0; kx < 2; kx++)
+                for (int ky = 0; ky < 2; ky++) {
+                    const double p[3] = {gv(0, kx ? o[0] + gs : o[0]), gv(1, ky ? o[1] + gs : o[1]),
+                                         gv(2, kz ? o[2] + gs : o[2])};
+                    const int cid = checker(p);
+                    if (cid != kMaxCell && cid < best) best = cid;
+                }
+        if (best != kMaxCell) overlay[idx] = best;
+    }
+    return 0;
+}
+
+// NESO-advection step: update_pos_kernel (apps/app_neso_advection/kernels.h:41-54; generated
+// apps_codegen/app_neso_advection_mdir_cg/seq/update_pos_kernel_loop.hpp:5-19) then the move loop
+// (kernels.h:56-113; seq/move_kernel_loop.hpp), OPP_DO_ONCE true on the first hop only.
+// GCC contracts pos += vel*dt into fma(vel,dt,pos), temp = pos + n*ext into fma(n,ext,pos) and
+// FMOD's a - b*trunc(a/b) into fma(-b,trunc(a/b),a).
+int64_t orc_neso_step(int64_t n, int nx, int ny, double cell_width, double dt, double ext_x, double ext_y,
+                      const int* c2c, const double* c_pos_ll, double* pos, const double* vel, int* mdir, int* p2c,
+                      int* hops, int threads) {
+    (void)nx;
+    (void)ny;
+    const double ext[2] = {ext_x, ext_y};
+    int64_t total = 0;
+#pragma omp parallel for schedule(static) reduction(+ : total) num_threads(nthreads_of(threads))
+    for (int64_t i = 0; i < n; i++) {
+        double* p = pos + 2 * i;
+        for (int d = 0; d < 2; d++) {
+            p[d] = std::fma(vel[2 * i + d], dt, p[d]);
+            const int off = (int)(std::abs(p[d]) + 2.0);
+            const double t = std::fma((double)off, ext[d], p[d]);
+            p[d] = std::fma(-ext[d], std::trunc(t / ext[d]), t);
+        }
+        int h = 0;
+        if (p2c[i] != kMaxCell) {
+            bool first = true;
+            int* md = mdir + 2 * i;
+            for (;;) {
+                const double* ll = c_pos_ll + 2 * (int64_t)p2c[i];
+                const int* cc = c2c + 4 * (int64_t)p2c[i];
+                const double dx = p[0] - ll[0];
+                const double dy = p[1] - ll[1];
+                h++;
+                if (first) {
+                    const bool xp = ((dx > cell_width) && (ext[0] > 2 * dx - cell_width)) ||
+                                    ((dx < 0) && (ext[0] < cell_width - 2 * dx));
+                    const bool yp = ((dy > cell_width) && (ext[1] > 2 * dy - cell_width)) ||
+                                    ((dy < 0) && (ext[1] < cell_width - 2 * dy));
+                    md[0] = xp ? 1 : -1;
+                    md[1] = yp ? 1 : -1;
+                    first = false;
+                }
+                if ((dx >= 0.0) && (dx <= cell_width)) {
+                    md[0] = 0;
+                } else if (md[0] > 0) {
+                    p2c[i] = cc[1];
+                    continue;
+                } else if (md[0] < 0) {
+                    p2c[i] = cc[0];
+                    continue;
+                }
+                if ((dy >= 0.0) && (dy <= cell_width)) {
+                    md[1] = 0;
+                } else if (md[1] > 0) {
+                    p2c[i] = cc[3];
+                    continue;
+                } else if (md[1] < 0) {
+                    p2c[i] = cc[2];
+                    continue;
+                }
+                break;
+            }
+        }
+        if (hops) hops[i] = h;
+        total += h;
+    }
+    return total;
+}
+
+}  // extern "C"

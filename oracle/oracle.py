@@ -1,0 +1,157 @@
+"""ctypes binding of the oracle (liboracle.so): the CPU checker of the GPU hot path.
+
+Test infrastructure only: imported by tests/, the smoke check and the CPU-baseline leg of
+bench.py, never by the product.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int, c_int64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+MAX_CELL_INDEX = 2147483647
+
+_SIGS = {
+    "orc_fempic_push": (None, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_int]),
+    "orc_fempic_move": (c_int64, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
+                                  c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "orc_fempic_deposit": (None, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "orc_holefill": (c_int64, [c_int64, c_int64, c_void_p, POINTER(c_void_p), c_void_p, c_void_p, c_int]),
+    "orc_sort_by_cell": (None, [c_int64, c_void_p, POINTER(c_void_p), c_void_p, c_void_p, c_int, c_void_p]),
+    "orc_fempic_step": (c_int64, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_double, c_double, c_double, c_int, c_void_p]),
+    "orc_dh_overlay_bruteforce": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
+                                          c_void_p, c_int]),
+    "orc_neso_step": (c_int64, [c_int64, c_int, c_int, c_double, c_double, c_double, c_double, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build with `make -C oracle oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        for n, (r, a) in _SIGS.items():
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+class FempicState:
+    """Host (AoS) state of one FEM-PIC instance, stepped by the oracle."""
+
+    def __init__(self, mesh, c_ef, pos, vel, p2c, overlay=None, bmin=None, gs=None, bmax=None):
+        self.c2n = np.ascontiguousarray(mesh["c2n"], np.int32)
+        self.c2c = np.ascontiguousarray(mesh["c2c"], np.int32)
+        self.c_det = np.ascontiguousarray(mesh["c_det"], np.float64)
+        self.c_vol = np.ascontiguousarray(mesh["c_vol"], np.float64)
+        self.c_ef = np.ascontiguousarray(c_ef, np.float64)
+        self.nn = int(mesh["n_nodes"])
+        self.pos = np.array(pos, np.float64, copy=True)
+        self.vel = np.array(vel, np.float64, copy=True)
+        self.lc = np.zeros((pos.shape[0], 4))
+        self.p2c = np.array(p2c, np.int32, copy=True)
+        self.n = int(pos.shape[0])
+        self.charge = np.zeros(self.nn)
+        self.overlay = None if overlay is None else np.ascontiguousarray(overlay, np.int32)
+        if overlay is not None:
+            d = []
+            for k in range(3):
+                a, x = 0, float(bmin[k])
+                while x < bmax[k]:
+                    a += 1
+                    x += gs
+                d.append(a + 1)
+            self.dims = np.array(d, np.int64)
+            self.bmin = np.ascontiguousarray(bmin, np.float64)
+            self.inv_gs = 1.0 / gs
+        else:
+            self.dims = np.zeros(3, np.int64)
+            self.bmin = np.zeros(3)
+            self.inv_gs = 0.0
+
+    def step(self, charge, mass, dt, threads=1):
+        stats = np.zeros(4, np.int64)
+        self.n = int(lib().orc_fempic_step(self.n, self.nn, _p(self.c2n), _p(self.c2c), _p(self.c_det),
+                                           _p(self.c_vol), _p(self.c_ef), _p(self.overlay), _p(self.dims),
+                                           _p(self.bmin), self.inv_gs, _p(self.pos), _p(self.vel), _p(self.lc),
+                                           _p(self.p2c), _p(self.charge), charge, mass, dt, threads, _p(stats)))
+        return dict(removed=int(stats[0]), total_hops=int(stats[1]), max_hops=int(stats[2]), size=self.n)
+
+    def push(self, charge, mass, dt, threads=1):
+        lib().orc_fempic_push(self.n, _p(self.c_ef), _p(self.p2c), _p(self.pos), _p(self.vel), charge, mass, dt,
+                              threads)
+
+    def move(self, threads=1):
+        hops = np.zeros(self.n, np.int32)
+        removed = lib().orc_fempic_move(self.n, _p(self.c2c), _p(self.c_det), _p(self.c_vol), _p(self.overlay),
+                                        _p(self.dims), _p(self.bmin), self.inv_gs, _p(self.pos), _p(self.lc),
+                                        _p(self.p2c), _p(hops), threads)
+        return int(removed), hops
+
+    def holefill(self, removed):
+        marks = self.p2c[: self.n].copy()
+        arrs = [self.pos, self.vel, self.lc, self.p2c]
+        ptrs = (c_void_p * 4)(*[a.ctypes.data for a in arrs])
+        dims = np.array([3, 3, 4, 1], np.int32)
+        eb = np.array([8, 8, 8, 4], np.int32)
+        self.n = int(lib().orc_holefill(self.n, removed, _p(marks), ptrs, _p(dims), _p(eb), 4))
+
+    def deposit(self, threads=1):
+        lib().orc_fempic_deposit(self.n, self.nn, _p(self.c2n), _p(self.p2c), _p(self.lc), _p(self.charge), threads)
+
+    def live(self):
+        n = self.n
+        return dict(n=n, pos=self.pos[:n].copy(), vel=self.vel[:n].copy(), lc=self.lc[:n].copy(),
+                    p2c=self.p2c[:n].copy(), charge=self.charge.copy())
+
+
+def dh_overlay_bruteforce(mesh, bmin, bmax, gs, threads=0):
+    bmin = np.ascontiguousarray(bmin, np.float64)
+    bmax = np.ascontiguousarray(bmax, np.float64)
+    d = []
+    for k in range(3):
+        a, x = 0, float(bmin[k])
+        while x < bmax[k]:
+            a += 1
+            x += gs
+        d.append(a + 1)
+    ov = np.zeros(int(np.prod(d)), np.int32)
+    lib().orc_dh_overlay_bruteforce(mesh["n_cells"], _p(np.ascontiguousarray(mesh["c2c"], np.int32)),
+                                    _p(np.ascontiguousarray(mesh["c_det"])), _p(np.ascontiguousarray(mesh["c_vol"])),
+                                    _p(bmin), _p(bmax), gs, _p(ov), threads)
+    return ov
+
+
+def sort_by_cell(n, arrays, p2c):
+    """Stable by (p2c, index); arrays = list of AoS numpy arrays (p2c included), permuted in place."""
+    ptrs = (c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+    dims = np.array([1 if a.ndim == 1 else a.shape[1] for a in arrays], np.int32)
+    eb = np.array([a.itemsize for a in arrays], np.int32)
+    perm = np.zeros(n, np.int64)
+    keys = np.ascontiguousarray(p2c[:n], np.int32).copy()
+    lib().orc_sort_by_cell(n, _p(keys), ptrs, _p(dims), _p(eb), len(arrays), _p(perm))
+    return perm
+
+
+def neso_step(n, nx, ny, cell_width, dt, c2c, c_pos_ll, pos, vel, mdir, p2c, threads=1):
+    hops = np.zeros(n, np.int32)
+    tot = lib().orc_neso_step(n, nx, ny, cell_width, dt, nx * cell_width, ny * cell_width, _p(c2c), _p(c_pos_ll),
+                              _p(pos), _p(vel), _p(mdir), _p(p2c), _p(hops), threads)
+    return int(tot), hops
