@@ -93,6 +93,15 @@ typedef struct oppx_move_counters {
  * reference would loop forever on a cycling walk). */
 #define OPPX_MAX_HOPS 4096
 
+/* Launch shape of the wavefront-cooperative walk: each wave owns `tile` consecutive particles
+ * (multiple of 64, <= 512) and refills its free lanes once `refill_min` of them are idle.
+ * Process-wide; chosen by the autotuner (op-pic_amd/autotune.py) or oppx_set_tuning. */
+typedef struct oppx_tuning {
+    int32_t tile;
+    int32_t refill_min;
+    int32_t reserved[6];
+} oppx_tuning;
+
 /* One particle dat for compaction/permutation: SoA base, components, element bytes (4 or 8). */
 typedef struct oppx_dat_desc {
     void* base;
@@ -180,9 +189,15 @@ int oppx_fempic_move(int64_t n_max, const oppx_fempic_parts* parts, const oppx_t
 int oppx_fempic_deposit_atomic(int64_t n_max, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh,
                                double* node_charge, oppx_move_counters* counters, void* stream);
 
+/* Start of a fused step: zeroes node_charge [n_nodes] (nullable) and the per-step counters
+ * (removed, holes_low, max_hops, total_hops) in one launch -- opp_reset_dat of the deposit
+ * target (opp_lib/src/hip/opp_hip.cpp:548-577) plus opp_init_particle_move's counter reset. */
+int oppx_fempic_step_begin(oppx_move_counters* counters, double* node_charge, int64_t n_nodes, void* stream);
+
 /* Fused step: push + (direct hop) + walk + FP64 atomic deposit in one pass over the particles.
  * Same per-particle results as the three loops above in sequence; node_charge (nullable)
- * is accumulated, not cleared. */
+ * is accumulated and the hop statistics add to counters, so a step starts with
+ * oppx_fempic_step_begin. */
 int oppx_fempic_step_fused(int64_t n_max, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh,
                            const oppx_dh_grid* dh, double k, double dt, oppx_move_counters* counters,
                            uint64_t* hole_bits, int32_t* hops, double* node_charge, void* stream);
@@ -223,7 +238,12 @@ int oppx_neso_step(int64_t n_max, const oppx_neso_parts* parts, const oppx_quad_
  * Compaction
  * ------------------------------------------------------------------------------------- */
 
-/* 64-bit words the removal bitmap of a move over n_max particles needs (whole blocks of 256). */
+/* Launch shape of the walk kernels (no reference counterpart: the reference fixes its block size
+ * by opp_threads_per_block / OPP_BLOCK_SIZE_4, apps_codegen/app_fempic_cg/hip/move_kernel_loop.hpp:221-225). */
+int oppx_set_tuning(const oppx_tuning* t);
+int oppx_get_tuning(oppx_tuning* t);
+
+/* 64-bit words the removal bitmap of a move over n_max particles needs (any tile size). */
 int64_t oppx_hole_bits_words(int64_t n_max);
 
 /* Scratch needed by oppx_holefill for n_max particles (bytes). */

@@ -168,7 +168,8 @@ using namespace oppx;
 
 extern "C" {
 
-int64_t oppx_hole_bits_words(int64_t n_max) { return n_max <= 0 ? 4 : 4 * ((n_max + kB - 1) / kB); }
+// a move covers whole tiles: at most ceil(n_max/64) words plus one partial block of the largest tile
+int64_t oppx_hole_bits_words(int64_t n_max) { return (n_max > 0 ? (n_max + 63) / 64 : 0) + 32; }
 
 size_t oppx_holefill_scratch_bytes(int64_t n_max) {
     if (n_max < 0) return 0;

@@ -50,6 +50,10 @@ class NesoParts(Structure):
     _fields_ = [("stride", c_int64), ("pos", c_void_p), ("vel", c_void_p), ("mdir", c_void_p), ("p2c", c_void_p)]
 
 
+class Tuning(Structure):
+    _fields_ = [("tile", c_int32), ("refill_min", c_int32), ("reserved", c_int32 * 6)]
+
+
 class DatDesc(Structure):
     _fields_ = [("base", c_void_p), ("dim", c_int32), ("elem_bytes", c_int32)]
 
@@ -93,7 +97,10 @@ _SIGS = {
     "oppx_quad_cells_build": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "oppx_neso_step": (c_int, [c_int64, POINTER(NesoParts), c_void_p, c_int64, c_double, c_double, c_double, c_double,
                                c_int, c_void_p, c_void_p, c_void_p]),
+    "oppx_fempic_step_begin": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
     "oppx_hole_bits_words": (c_int64, [c_int64]),
+    "oppx_set_tuning": (c_int, [POINTER(Tuning)]),
+    "oppx_get_tuning": (c_int, [POINTER(Tuning)]),
     "oppx_holefill_scratch_bytes": (c_size_t, [c_int64]),
     "oppx_holefill": (c_int, [c_int64, c_int64, c_void_p, c_void_p, POINTER(DatDesc), c_int, c_void_p,
                               c_size_t, c_void_p]),
@@ -212,6 +219,16 @@ def neso_particles(n: int, seed: int, m: dict, vmax: float):
     return pos, vel, p2c
 
 
+def set_tuning(tile: int, refill_min: int):
+    check(lib().oppx_set_tuning(ctypes.byref(Tuning(tile, refill_min))), "set_tuning")
+
+
+def get_tuning():
+    t = Tuning()
+    check(lib().oppx_get_tuning(ctypes.byref(t)), "get_tuning")
+    return int(t.tile), int(t.refill_min)
+
+
 def sort_particles_by_cell(pos, vel, p2c):
     """Stable reorder of an input population by containing cell (an input-layout choice)."""
     order = np.argsort(p2c, kind="stable")
@@ -315,14 +332,21 @@ class FempicDevice:
                                                self.charge.data_ptr(), self.counters.data_ptr(), self.stream()),
               "fempic_deposit")
 
-    def step_fused(self, charge, mass, dt, record_hops=False):
-        """push + direct hop + walk + deposit in one pass, then hole-fill."""
-        self.charge.zero_()
+    def step_begin(self):
+        check(lib().oppx_fempic_step_begin(self.counters.data_ptr(), self.charge.data_ptr(), self.nn, self.stream()),
+              "fempic_step_begin")
+
+    def pass_fused(self, charge, mass, dt, record_hops=False):
         k = charge / mass * dt
         check(lib().oppx_fempic_step_fused(self.n_max, ctypes.byref(self.parts), ctypes.byref(self.mesh),
                                            self._dh_ref(), k, dt, self.counters.data_ptr(),
                                            self.hole_bits.data_ptr(), self.hops.data_ptr() if record_hops else None,
                                            self.charge.data_ptr(), self.stream()), "fempic_step_fused")
+
+    def step_fused(self, charge, mass, dt, record_hops=False):
+        """push + direct hop + walk + deposit in one pass, then hole-fill."""
+        self.step_begin()
+        self.pass_fused(charge, mass, dt, record_hops)
         self.holefill()
 
     def step_unfused(self, charge, mass, dt, record_hops=True):

@@ -3,6 +3,7 @@ and the oracle. Bar: cell indices, hop counts, removal counts and the hole-fill 
 bit-exact; positions, velocities and lc bit-exact (the FMAs are the reference's); FP64 atomic
 deposit within 1e-12 relative to the reference's sums (atomic reordering)."""
 
+import ctypes
 import hashlib
 import math
 
@@ -75,6 +76,33 @@ def test_gpu_matches_reference_fixture(name, mode):
     assert np.array_equal(out["pos"].ravel(), g["out_final_pos"])
     assert np.array_equal(out["lc"].ravel(), g["out_final_lc"])
     assert charge_close(out["charge"], g["out_final_charge"])
+
+
+@pytest.mark.parametrize("tile,refill,split", [(64, 64, 0), (128, 8, 0), (512, 32, 0), (256, 16, 1), (512, 1, 1)])
+def test_gpu_walk_launch_shapes_bit_exact(tile, refill, split):
+    # every launch shape of the cooperative walk gives the same cells, hops, permutation and lc
+    g = load("fempic_small_dh_omp1.npz")
+    m, ef, pos, vel, p2c, kw = case(g)
+    t = oppx.Tuning(tile, refill)
+    t.reserved[0] = split
+    saved = oppx.get_tuning()
+    oppx.check(oppx.lib().oppx_set_tuning(ctypes.byref(t)), "set_tuning")
+    try:
+        gpu = oppx.FempicDevice(m, ef, pos, vel, p2c, **kw)
+        q, ms, dt = float(g["meta_charge"]), float(g["meta_mass"]), float(g["meta_dt"])
+        steps = g["out_steps_iter_before_after_hops_maxhops"]
+        for s in range(int(g["meta_steps"])):
+            gpu.step_fused(q, ms, dt, record_hops=True)
+            c = gpu.read_counters()
+            assert (c["size"], c["total_hops"], c["max_hops"], c["faults"]) == (*steps[s, 2:5], 0), f"step {s}"
+            if s == 0:
+                assert np.array_equal(gpu.hops[: steps[0, 1]].cpu().numpy(), g["out_s1_hops"])
+            gpu.sync_size()
+        out = gpu.download()
+        assert np.array_equal(out["p2c"], g["out_final_p2c"]) and np.array_equal(out["lc"].ravel(), g["out_final_lc"])
+        assert charge_close(out["charge"], g["out_final_charge"])
+    finally:
+        oppx.set_tuning(*saved)
 
 
 def test_gpu_c1_scale_matches_reference():
