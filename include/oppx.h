@@ -257,6 +257,23 @@ size_t oppx_holefill_scratch_bytes(int64_t n_max);
 int oppx_holefill(int64_t n_max, int64_t stride, oppx_move_counters* counters, const uint64_t* hole_bits,
                   const oppx_dat_desc* dats, int n_dats, void* scratch, size_t scratch_bytes, void* stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Sort
+ * ------------------------------------------------------------------------------------- */
+
+/* Scratch needed by oppx_sort_by_cell for n_max particles (bytes). */
+size_t oppx_sort_scratch_bytes(int64_t n_max);
+
+/* opp_particle_sort and the Sort_All / Sort_Periodic finalize (opp_lib/src/core/opp_particle_organize_core.cpp:98-121;
+ * opp_lib/src/omp/opp_particle_mover.cpp:223-265; hip opp_lib/src/hip/opp_particle_sorter.cpp:188-267): orders the
+ * counters->size particles by p2c, removed particles (MAX_CELL_INDEX) last, stable by (p2c, index). Each dat in
+ * `dats` is gathered into the matching separate buffer of `out_dats` (same dims; the caller swaps them in).
+ * perm_out (nullable) receives the source index of every output slot. drop_removed != 0 also applies the
+ * finalize: counters->removed = number removed, counters->size -= removed. */
+int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counters, int64_t n_cells,
+                      const int32_t* p2c, const oppx_dat_desc* dats, const oppx_dat_desc* out_dats, int n_dats,
+                      int32_t* perm_out, int drop_removed, void* scratch, size_t scratch_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

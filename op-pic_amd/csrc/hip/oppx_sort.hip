@@ -1,0 +1,312 @@
+// Particle sort by cell on gfx950: stable LSD radix sort of (cell, index) pairs, then one
+// permutation gather per particle dat.
+//
+// Reference: opp_particle_sort / the Sort_* fill policies (opp_lib/src/core/opp_particle_organize_core.cpp:98-121,
+// opp_lib/src/omp/opp_particle_mover.cpp:223-265, hip opp_lib/src/hip/opp_particle_sorter.cpp:188-267) order the
+// particles by p2c with removed particles (MAX_CELL_INDEX) last. Their tie order differs between backends
+// (std::sort / thrust radix); this build's contract is stable by (p2c, index).
+//
+// Each pass sorts by one 8-bit digit. A block owns kElems consecutive elements; ranks inside the block
+// come from wave-level digit matching (8 ballots give the lanes holding the same digit), so equal digits
+// keep their index order and no LDS atomic is contended by nearly sorted keys. Per-block digit counts
+// are scanned digit-major across blocks to give every (digit, block) its output offset.
+
+#include "oppx_internal.h"
+
+namespace oppx {
+namespace {
+
+constexpr int kB = 256;
+constexpr int kItems = 16;
+constexpr int kElems = kB * kItems;  // elements per block
+constexpr int kBins = 256;
+constexpr int kMaxDats = 16;
+
+struct GatherTable {
+    int n;
+    int64_t stride;
+    const void* in[kMaxDats];
+    void* out[kMaxDats];
+    int dim[kMaxDats];
+    int eb[kMaxDats];
+};
+
+__device__ __forceinline__ int64_t live_n(const oppx_move_counters* ctr, int64_t n_max) {
+    const int64_t n = ctr->size;
+    return n < 0 ? 0 : (n > n_max ? n_max : n);
+}
+
+// lanes of the wave (among `active`) holding the same 8-bit digit as this lane
+__device__ __forceinline__ unsigned long long digit_peers(uint32_t d, unsigned long long active) {
+    unsigned long long peers = active;
+#pragma unroll
+    for (int bit = 0; bit < 8; bit++) {
+        const unsigned long long m = __ballot((d >> bit) & 1u);
+        peers &= ((d >> bit) & 1u) ? m : ~m;
+    }
+    return peers;
+}
+
+// keys0 = p2c with MAX_CELL_INDEX mapped to n_cells (the last bucket), vals0 = index; count removed
+__global__ void __launch_bounds__(kB) sort_prepare(const int32_t* __restrict__ p2c, const oppx_move_counters* ctr,
+                                                   int64_t n_max, int64_t n_cells, uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals, unsigned long long* __restrict__ n_removed) {
+    const int64_t n = live_n(ctr, n_max);
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    bool rem = false;
+    if (i < n) {
+        const int32_t c = p2c[i];
+        rem = (c == kMaxCell);
+        keys[i] = rem ? (uint32_t)n_cells : (uint32_t)c;
+        vals[i] = (uint32_t)i;
+    }
+    const unsigned long long m = __ballot(rem);
+    if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(n_removed, (unsigned long long)__popcll(m));
+}
+
+__global__ void __launch_bounds__(kB) sort_hist(const uint32_t* __restrict__ keys, const oppx_move_counters* ctr,
+                                                int64_t n_max, int shift, uint32_t* __restrict__ hist, int64_t nblocks) {
+    __shared__ uint32_t s_h[kBins];
+    s_h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t n = live_n(ctr, n_max);
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int r = 0; r < kItems; r++) {
+        const int64_t i = (int64_t)blockIdx.x * kElems + r * kB + threadIdx.x;
+        const bool ok = i < n;
+        const uint32_t d = ok ? (keys[i] >> shift) & 0xFFu : 0u;
+        const unsigned long long act = __ballot(ok);
+        const unsigned long long peers = digit_peers(d, act);
+        if (ok && lane == __ffsll((long long)peers) - 1) atomicAdd(&s_h[d], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = s_h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kB) sort_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                   const oppx_move_counters* ctr, int64_t n_max, int shift,
+                                                   const uint32_t* __restrict__ offs, int64_t nblocks,
+                                                   uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    __shared__ uint32_t s_run[kBins];                     // elements of each digit already placed by this block
+    __shared__ uint32_t s_wave[kB / kWave][kBins];        // per-wave digit counts of the current round
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    s_run[threadIdx.x] = 0;
+    const int64_t n = live_n(ctr, n_max);
+    for (int r = 0; r < kItems; r++) {
+        for (int q = 0; q < kB / kWave; q++) s_wave[q][threadIdx.x] = 0;
+        __syncthreads();
+        const int64_t i = (int64_t)blockIdx.x * kElems + r * kB + threadIdx.x;
+        const bool ok = i < n;
+        const uint32_t key = ok ? kin[i] : 0u;
+        const uint32_t val = ok ? vin[i] : 0u;
+        const uint32_t d = (key >> shift) & 0xFFu;
+        const unsigned long long act = __ballot(ok);
+        const unsigned long long peers = digit_peers(d, act);
+        const uint32_t rank_w = (uint32_t)__popcll(peers & below);
+        if (ok && lane == __ffsll((long long)peers) - 1) s_wave[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (ok) {
+            uint32_t before = s_run[d];
+            for (int q = 0; q < w; q++) before += s_wave[q][d];
+            const uint64_t dst = (uint64_t)offs[(int64_t)d * nblocks + blockIdx.x] + before + rank_w;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int q = 0; q < kB / kWave; q++) add += s_wave[q][threadIdx.x];
+        s_run[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+// ---- exclusive scan of a uint32 array (chunks of kElems) ----
+__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t s_w[kB / kWave];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t t = __shfl_up(inc, d);
+        if (lane >= d) inc += t;
+    }
+    if (lane == kWave - 1) s_w[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, all = 0;
+    for (int q = 0; q < kB / kWave; q++) {
+        if (q < w) off += s_w[q];
+        all += s_w[q];
+    }
+    __syncthreads();
+    *total = all;
+    return off + inc - v;
+}
+
+__global__ void __launch_bounds__(kB) scan_reduce(const uint32_t* __restrict__ a, int64_t n, uint32_t* __restrict__ sums) {
+    uint32_t s = 0;
+    for (int r = 0; r < kItems; r++) {
+        const int64_t i = (int64_t)blockIdx.x * kElems + (int64_t)threadIdx.x * kItems + r;
+        if (i < n) s += a[i];
+    }
+    uint32_t tot;
+    block_scan_excl(s, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kB) scan_sums(uint32_t* __restrict__ sums, int64_t nb) {
+    uint32_t carry = 0;
+    for (int64_t base = 0; base < nb; base += kB) {
+        const int64_t j = base + threadIdx.x;
+        const uint32_t v = (j < nb) ? sums[j] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_scan_excl(v, &tot);
+        if (j < nb) sums[j] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ void __launch_bounds__(kB) scan_apply(uint32_t* __restrict__ a, int64_t n, const uint32_t* __restrict__ sums) {
+    uint32_t loc[kItems];
+    uint32_t s = 0;
+    for (int r = 0; r < kItems; r++) {
+        const int64_t i = (int64_t)blockIdx.x * kElems + (int64_t)threadIdx.x * kItems + r;
+        loc[r] = (i < n) ? a[i] : 0u;
+        s += loc[r];
+    }
+    uint32_t tot;
+    uint32_t run = sums[blockIdx.x] + block_scan_excl(s, &tot);
+    for (int r = 0; r < kItems; r++) {
+        const int64_t i = (int64_t)blockIdx.x * kElems + (int64_t)threadIdx.x * kItems + r;
+        if (i < n) a[i] = run;
+        run += loc[r];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void gather_rec(const void* in, void* out, int dim, int64_t stride, int64_t from, int64_t to) {
+    const T* s = reinterpret_cast<const T*>(in);
+    T* d = reinterpret_cast<T*>(out);
+    for (int q = 0; q < dim; q++) d[q * stride + to] = s[q * stride + from];
+}
+
+__global__ void __launch_bounds__(kB) sort_gather(const uint32_t* __restrict__ perm, const oppx_move_counters* ctr,
+                                                  int64_t n_max, GatherTable T, int32_t* __restrict__ perm_out) {
+    const int64_t n = live_n(ctr, n_max);
+    const int64_t j = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (j >= n) return;
+    const int64_t from = perm[j];
+    for (int q = 0; q < T.n; q++) {
+        if (T.eb[q] == 8) gather_rec<double>(T.in[q], T.out[q], T.dim[q], T.stride, from, j);
+        else gather_rec<int32_t>(T.in[q], T.out[q], T.dim[q], T.stride, from, j);
+    }
+    if (perm_out) perm_out[j] = (int32_t)from;
+}
+
+__global__ void zero_u64(unsigned long long* p) { *p = 0ull; }
+
+__global__ void sort_finalize(oppx_move_counters* ctr, int64_t n_max, const unsigned long long* n_removed, int drop) {
+    const int64_t n = live_n(ctr, n_max);
+    const int64_t r = (int64_t)*n_removed;
+    ctr->removed = (int32_t)r;
+    ctr->holes_low = 0;
+    if (drop) ctr->size = (int32_t)(n - r);
+}
+
+struct SortScratch {
+    uint32_t *ka, *va, *kb, *vb, *hist, *sums;
+    unsigned long long* n_removed;
+    int64_t nblocks, hist_n, scan_blocks;
+};
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+SortScratch carve(void* base, int64_t n_max) {
+    SortScratch s;
+    s.nblocks = (n_max + kElems - 1) / kElems;
+    if (s.nblocks < 1) s.nblocks = 1;
+    s.hist_n = s.nblocks * kBins;
+    s.scan_blocks = (s.hist_n + kElems - 1) / kElems;
+    char* p = reinterpret_cast<char*>(base);
+    const size_t arr = align256((size_t)(n_max > 0 ? n_max : 1) * sizeof(uint32_t));
+    s.ka = reinterpret_cast<uint32_t*>(p); p += arr;
+    s.va = reinterpret_cast<uint32_t*>(p); p += arr;
+    s.kb = reinterpret_cast<uint32_t*>(p); p += arr;
+    s.vb = reinterpret_cast<uint32_t*>(p); p += arr;
+    s.hist = reinterpret_cast<uint32_t*>(p); p += align256((size_t)s.hist_n * sizeof(uint32_t));
+    s.sums = reinterpret_cast<uint32_t*>(p); p += align256((size_t)s.scan_blocks * sizeof(uint32_t));
+    s.n_removed = reinterpret_cast<unsigned long long*>(p);
+    return s;
+}
+
+size_t scratch_need(int64_t n_max) {
+    const int64_t nb = n_max > 0 ? (n_max + kElems - 1) / kElems : 1;
+    const int64_t hist_n = nb * kBins;
+    const int64_t scan_blocks = (hist_n + kElems - 1) / kElems;
+    return 4 * align256((size_t)(n_max > 0 ? n_max : 1) * 4) + align256((size_t)hist_n * 4) +
+           align256((size_t)scan_blocks * 4) + 256;
+}
+
+}  // namespace
+}  // namespace oppx
+
+using namespace oppx;
+
+extern "C" {
+
+size_t oppx_sort_scratch_bytes(int64_t n_max) { return n_max < 0 ? 0 : scratch_need(n_max); }
+
+int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counters, int64_t n_cells,
+                      const int32_t* p2c, const oppx_dat_desc* dats, const oppx_dat_desc* out_dats, int n_dats,
+                      int32_t* perm_out, int drop_removed, void* scratch, size_t scratch_bytes, void* stream) {
+    OPPX_ARG_CHECK(counters && p2c && scratch, "oppx_sort_by_cell: null pointer");
+    OPPX_ARG_CHECK(n_max >= 0 && stride >= n_max, "oppx_sort_by_cell: need 0 <= n_max <= stride");
+    OPPX_ARG_CHECK(n_cells > 0 && n_cells < kMaxCell, "oppx_sort_by_cell: n_cells out of range");
+    OPPX_ARG_CHECK(n_max < (int64_t)0xFFFFFFFFll, "oppx_sort_by_cell: n_max beyond 32-bit indices");
+    OPPX_ARG_CHECK(n_dats >= 0 && n_dats <= kMaxDats && (n_dats == 0 || (dats && out_dats)),
+                   "oppx_sort_by_cell: 0..16 dats with output buffers");
+    OPPX_ARG_CHECK(scratch_bytes >= scratch_need(n_max), "oppx_sort_by_cell: scratch too small");
+    GatherTable T{};
+    T.n = n_dats;
+    T.stride = stride;
+    for (int q = 0; q < n_dats; q++) {
+        OPPX_ARG_CHECK(dats[q].base && out_dats[q].base && dats[q].dim > 0 && dats[q].dim == out_dats[q].dim &&
+                           dats[q].elem_bytes == out_dats[q].elem_bytes &&
+                           (dats[q].elem_bytes == 4 || dats[q].elem_bytes == 8) && dats[q].base != out_dats[q].base,
+                       "oppx_sort_by_cell: bad dat descriptor (dims/sizes must match, output must be a separate buffer)");
+        T.in[q] = dats[q].base;
+        T.out[q] = out_dats[q].base;
+        T.dim[q] = dats[q].dim;
+        T.eb[q] = dats[q].elem_bytes;
+    }
+    hipStream_t s = as_stream(stream);
+    SortScratch sc = carve(scratch, n_max);
+    zero_u64<<<1, 1, 0, s>>>(sc.n_removed);  // a kernel, not a memset node, so the sort can be captured
+    if (n_max > 0) {
+        sort_prepare<<<grid_for(n_max, kB), kB, 0, s>>>(p2c, counters, n_max, n_cells, sc.ka, sc.va, sc.n_removed);
+        OPPX_LAUNCH_CHECK("sort_prepare");
+        int bits = 1;
+        while (bits < 32 && ((int64_t)1 << bits) <= n_cells) bits++;  // keys are in [0, n_cells]
+        uint32_t *kin = sc.ka, *vin = sc.va, *kout = sc.kb, *vout = sc.vb;
+        for (int shift = 0; shift < bits; shift += 8) {
+            sort_hist<<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, counters, n_max, shift, sc.hist, sc.nblocks);
+            OPPX_LAUNCH_CHECK("sort_hist");
+            scan_reduce<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
+            scan_sums<<<1, kB, 0, s>>>(sc.sums, sc.scan_blocks);
+            scan_apply<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
+            OPPX_LAUNCH_CHECK("sort_scan");
+            sort_scatter<<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, vin, counters, n_max, shift, sc.hist, sc.nblocks,
+                                                              kout, vout);
+            OPPX_LAUNCH_CHECK("sort_scatter");
+            uint32_t* t = kin; kin = kout; kout = t;
+            t = vin; vin = vout; vout = t;
+        }
+        sort_gather<<<grid_for(n_max, kB), kB, 0, s>>>(vin, counters, n_max, T, perm_out);
+        OPPX_LAUNCH_CHECK("sort_gather");
+    }
+    sort_finalize<<<1, 1, 0, s>>>(counters, n_max, sc.n_removed, drop_removed);
+    OPPX_LAUNCH_CHECK("sort_finalize");
+    return OPPX_OK;
+}
+
+}  // extern "C"

@@ -99,6 +99,9 @@ _SIGS = {
                                c_int, c_void_p, c_void_p, c_void_p]),
     "oppx_fempic_step_begin": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
     "oppx_hole_bits_words": (c_int64, [c_int64]),
+    "oppx_sort_scratch_bytes": (c_size_t, [c_int64]),
+    "oppx_sort_by_cell": (c_int, [c_int64, c_int64, c_void_p, c_int64, c_void_p, POINTER(DatDesc), POINTER(DatDesc),
+                                  c_int, c_void_p, c_int, c_void_p, c_size_t, c_void_p]),
     "oppx_set_tuning": (c_int, [POINTER(Tuning)]),
     "oppx_get_tuning": (c_int, [POINTER(Tuning)]),
     "oppx_holefill_scratch_bytes": (c_size_t, [c_int64]),
@@ -327,6 +330,31 @@ class FempicDevice:
                                   self.dats, 4, self.scratch.data_ptr(), self.scratch_bytes, self.stream()),
               "holefill")
 
+    def sort(self, drop_removed=False, perm=None):
+        """Stable sort of the particle dats by cell (removed last); drop_removed applies the finalize."""
+        torch = self.torch
+        if not hasattr(self, "_swap"):
+            self._swap = [torch.empty_like(t) for t in (self.pos, self.vel, self.lc, self.p2c)]
+            self.sort_scratch_bytes = int(lib().oppx_sort_scratch_bytes(self.cap))
+            self.sort_scratch = torch.zeros(self.sort_scratch_bytes, dtype=torch.uint8, device=self.dev)
+        cur = [self.pos, self.vel, self.lc, self.p2c]
+        dims = [(3, 8), (3, 8), (4, 8), (1, 4)]
+        ins = (DatDesc * 4)(*[DatDesc(t.data_ptr(), d, e) for t, (d, e) in zip(cur, dims)])
+        outs = (DatDesc * 4)(*[DatDesc(t.data_ptr(), d, e) for t, (d, e) in zip(self._swap, dims)])
+        check(lib().oppx_sort_by_cell(self.n_max, self.cap, self.counters.data_ptr(), self.nc, self.p2c.data_ptr(),
+                                      ins, outs, 4, perm.data_ptr() if perm is not None else None,
+                                      1 if drop_removed else 0, self.sort_scratch.data_ptr(), self.sort_scratch_bytes,
+                                      self.stream()), "sort_by_cell")
+        self.pos, self.vel, self.lc, self.p2c = self._swap
+        self._swap = cur
+        self._rebind()
+
+    def _rebind(self):
+        self.parts = FempicParts(self.cap, self.pos.data_ptr(), self.vel.data_ptr(), self.lc.data_ptr(),
+                                 self.p2c.data_ptr())
+        self.dats = (DatDesc * 4)(DatDesc(self.pos.data_ptr(), 3, 8), DatDesc(self.vel.data_ptr(), 3, 8),
+                                  DatDesc(self.lc.data_ptr(), 4, 8), DatDesc(self.p2c.data_ptr(), 1, 4))
+
     def deposit(self):
         check(lib().oppx_fempic_deposit_atomic(self.n_max, ctypes.byref(self.parts), ctypes.byref(self.mesh),
                                                self.charge.data_ptr(), self.counters.data_ptr(), self.stream()),
@@ -343,18 +371,24 @@ class FempicDevice:
                                            self.hole_bits.data_ptr(), self.hops.data_ptr() if record_hops else None,
                                            self.charge.data_ptr(), self.stream()), "fempic_step_fused")
 
-    def step_fused(self, charge, mass, dt, record_hops=False):
-        """push + direct hop + walk + deposit in one pass, then hole-fill."""
+    def step_fused(self, charge, mass, dt, record_hops=False, fill="holefill"):
+        """push + direct hop + walk + deposit in one pass, then hole-fill (or a sort finalize)."""
         self.step_begin()
         self.pass_fused(charge, mass, dt, record_hops)
-        self.holefill()
+        if fill == "sort":
+            self.sort(drop_removed=True)
+        else:
+            self.holefill()
 
-    def step_unfused(self, charge, mass, dt, record_hops=True):
-        """The reference's loop order: reset, push, move (+hole-fill), deposit."""
+    def step_unfused(self, charge, mass, dt, record_hops=True, fill="holefill"):
+        """The reference's loop order: reset, push, move (+ hole-fill or sort finalize), deposit."""
         self.charge.zero_()
         self.push(charge, mass, dt)
         self.move(record_hops)
-        self.holefill()
+        if fill == "sort":
+            self.sort(drop_removed=True)
+        else:
+            self.holefill()
         self.deposit()
 
     # -- readback ------------------------------------------------------------------------------

@@ -116,6 +116,27 @@ class FempicState:
     def deposit(self, threads=1):
         lib().orc_fempic_deposit(self.n, self.nn, _p(self.c2n), _p(self.p2c), _p(self.lc), _p(self.charge), threads)
 
+    def sort(self, drop_removed=True):
+        """Stable sort by (p2c, index) with removed last; drop_removed shrinks n by the removed count."""
+        n = self.n
+        removed = int((self.p2c[:n] == MAX_CELL_INDEX).sum())
+        arrs = [self.pos[:n].copy(), self.vel[:n].copy(), self.lc[:n].copy(), self.p2c[:n].copy()]
+        perm = sort_by_cell(n, arrs, self.p2c)
+        self.pos[:n], self.vel[:n], self.lc[:n], self.p2c[:n] = arrs
+        if drop_removed:
+            self.n = n - removed
+        return perm
+
+    def step_sorted(self, charge, mass, dt, threads=1):
+        """The step with the Sort_All finalize: push, move, sort (removed last, dropped), deposit."""
+        self.charge[:] = 0.0
+        self.push(charge, mass, dt, threads)
+        removed, hops = self.move(threads)
+        self.sort(drop_removed=True)
+        self.deposit(threads)
+        return dict(removed=removed, total_hops=int(hops.sum()), max_hops=int(hops.max()) if hops.size else 0,
+                    size=self.n)
+
     def live(self):
         n = self.n
         return dict(n=n, pos=self.pos[:n].copy(), vel=self.vel[:n].copy(), lc=self.lc[:n].copy(),

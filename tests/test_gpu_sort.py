@@ -1,0 +1,81 @@
+"""GPU particle sort against the oracle: stable by (p2c, index), removed particles last and
+dropped by the finalize; the Sort_All step policy bit-exact against the oracle's."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oppx
+import oracle
+from conftest import golden_path
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,n_cells,frac_removed,seed", [(1, 5, 0.0, 0), (1000, 48, 0.1, 1), (70001, 300, 0.3, 2),
+                                                         (300000, 6_000_000, 0.05, 3), (5000, 7, 0.0, 4),
+                                                         (4096, 100000, 1.0, 5)])
+def test_gpu_sort_matches_oracle(n, n_cells, frac_removed, seed):
+    rng = np.random.default_rng(seed)
+    p2c = rng.integers(0, n_cells, n).astype(np.int32)
+    p2c[rng.random(n) < frac_removed] = oppx.MAX_CELL_INDEX
+    if seed == 4:
+        p2c[:] = 3  # one long run of equal keys: order must be the index order
+    vals = rng.random((n, 3))
+    cap = n + 5
+    L = oppx.lib()
+    dev = "cuda"
+    v_in = torch.zeros((3, cap), dtype=torch.float64, device=dev)
+    v_in[:, :n] = torch.from_numpy(vals.T.copy())
+    k_in = torch.zeros(cap, dtype=torch.int32, device=dev)
+    k_in[:n] = torch.from_numpy(p2c)
+    v_out, k_out = torch.zeros_like(v_in), torch.zeros_like(k_in)
+    perm = torch.zeros(cap, dtype=torch.int32, device=dev)
+    ctr = torch.zeros(8, dtype=torch.int32, device=dev)
+    ctr[0] = n
+    sb = int(L.oppx_sort_scratch_bytes(n))
+    scratch = torch.zeros(sb, dtype=torch.uint8, device=dev)
+    ins = (oppx.DatDesc * 2)(oppx.DatDesc(v_in.data_ptr(), 3, 8), oppx.DatDesc(k_in.data_ptr(), 1, 4))
+    outs = (oppx.DatDesc * 2)(oppx.DatDesc(v_out.data_ptr(), 3, 8), oppx.DatDesc(k_out.data_ptr(), 1, 4))
+    oppx.check(L.oppx_sort_by_cell(n, cap, ctr.data_ptr(), n_cells, k_in.data_ptr(), ins, outs, 2, perm.data_ptr(), 1,
+                                   scratch.data_ptr(), sb, torch.cuda.current_stream().cuda_stream), "sort")
+    removed = int((p2c == oppx.MAX_CELL_INDEX).sum())
+    c = ctr.cpu().numpy()
+    assert c[0] == n - removed and c[1] == removed
+    expect = np.argsort(p2c, kind="stable")  # MAX sorts last
+    got = perm[:n].cpu().numpy()
+    live = n - removed
+    assert np.array_equal(got[:live], expect[:live])
+    assert np.array_equal(np.sort(got), np.arange(n))  # a permutation, removed ones in the tail
+    assert np.array_equal(k_out[:n].cpu().numpy(), p2c[got])
+    assert np.array_equal(v_out[:, :n].cpu().numpy().T, vals[got])
+    # the oracle's sort agrees on the live range
+    arrs = [vals.copy(), p2c.copy()]
+    operm = oracle.sort_by_cell(n, arrs, p2c)
+    assert np.array_equal(operm[:live], got[:live])
+
+
+def test_gpu_sort_fill_policy_steps_match_oracle():
+    g = np.load(golden_path("fempic_small_dh_omp1.npz"), allow_pickle=False)
+    n = int(g["meta_nx"])
+    m = oppx.box_mesh(n, n, n)
+    kw = dict(overlay=g["out_overlay"], bmin=tuple(g["bmin"]), bmax=tuple(g["bmax"]), gs=float(g["meta_gs"]))
+    gpu = oppx.FempicDevice(m, g["in_c_ef"], g["in_pos"], g["in_vel"], g["in_p2c"], **kw)
+    ref = oracle.FempicState(m, g["in_c_ef"], g["in_pos"], g["in_vel"], g["in_p2c"], **kw)
+    q, ms, dt = float(g["meta_charge"]), float(g["meta_mass"]), float(g["meta_dt"])
+    steps = g["out_steps_iter_before_after_hops_maxhops"]
+    for s in range(int(g["meta_steps"])):
+        gpu.step_fused(q, ms, dt, fill="sort")
+        r = ref.step_sorted(q, ms, dt)
+        c = gpu.read_counters()
+        # counts and hops do not depend on the fill policy
+        assert (c["size"], c["total_hops"], c["max_hops"]) == tuple(steps[s, 2:5])
+        assert (c["size"], c["total_hops"]) == (r["size"], r["total_hops"])
+        out, exp = gpu.download(), ref.live()
+        assert np.array_equal(out["p2c"], exp["p2c"]) and np.all(np.diff(out["p2c"]) >= 0)
+        assert np.array_equal(out["pos"], exp["pos"]) and np.array_equal(out["lc"], exp["lc"])
+        scale = max(np.abs(exp["charge"]).max(), 1e-300)
+        assert np.abs(out["charge"] - exp["charge"]).max() <= 1e-12 * scale
+        gpu.sync_size()
