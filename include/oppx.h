@@ -124,6 +124,11 @@ int oppx_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
 int oppx_memset_zero(void* dst, size_t bytes, void* stream);
 int oppx_stream_sync(void* stream);
 
+/* Fill n elements of dim components (elem_bytes 4 or 8, dim*elem_bytes <= 256) with the per-component
+ * values in `pattern` (host memory): AoS when stride == 0, else SoA with `stride`. A kernel, so it can be
+ * graph-captured. Serves opp_reset_dat (opp_lib/include/opp_lib.h:98-102). */
+int oppx_fill_pattern(void* dst, int64_t n, int dim, int elem_bytes, const void* pattern, int64_t stride, void* stream);
+
 /* AoS (host layout, doc/api.rst:115-118) <-> SoA (device layout) transposes on the device,
  * replacing the host transposes of opp_lib/src/hip/opp_hip_utils.cpp:35-153. */
 int oppx_aos_to_soa(const void* aos, void* soa, int64_t n, int dim, int64_t stride, int elem_bytes, void* stream);
@@ -143,6 +148,8 @@ int oppx_box_field_sines(int64_t n_nodes, int64_t n_cells, const double* node_po
 /* Seeded uniform positions, N(0,sigma^2) velocities, containing tet (AoS host arrays). */
 int oppx_box_particles_uniform(int64_t n_part, uint64_t seed, double sigma, int nx, int ny, int nz,
                                double lx, double ly, double lz, double* pos, double* vel, int* p2c);
+/* node -> (cell, vertex) incidence: node_start [n_nodes+1], node_ref [4*n_cells] = c*4+k ascending per node. */
+int oppx_node_cell_csr(int64_t n_cells, int64_t n_nodes, const int* c2n, int* node_start, int* node_ref);
 /* Overlay dims of CellMapper::init_host (opp_lib/include/opp_direct_hop_core.h:317-336). */
 int oppx_dh_grid_dims(const double bmin[3], const double bmax[3], double gs, int64_t dims[3]);
 /* Overlay contents of CellMapper::generateStructuredMesh (opp_lib/src/seq/opp_direct_hop_seq.cpp:134-265),
@@ -150,6 +157,9 @@ int oppx_dh_grid_dims(const double bmin[3], const double bmax[3], double gs, int
 int oppx_dh_build_overlay_nodes(int64_t n_cells, const double* c_det, const double* c_vol, const int* c_gbl_id,
                                 const int* c2n, const double* node_pos, const double bmin[3],
                                 const double bmax[3], double gs, int* overlay);
+/* The same overlay from c_det / c_volume alone (tet vertices recovered from the barycentric planes). */
+int oppx_dh_build_overlay_geom(int64_t n_cells, const double* c_det, const double* c_vol, const int* c_gbl_id,
+                               const double bmin[3], const double bmax[3], double gs, int* overlay);
 
 /* NESO-advection 2-D periodic quad mesh (apps/app_neso_advection/advec_misc.h:115-143):
  * c_pos_ll [2*nx*ny] lower-left corners, c2c [4*nx*ny] = {x-1, x+1, y-1, y+1} wrapped. */
@@ -226,12 +236,15 @@ typedef struct oppx_neso_parts {
 int oppx_quad_cells_build(const double* c_pos_ll, const int32_t* c2c, int64_t n_cells, oppx_quad_cell* cells,
                           void* stream);
 
-/* opp_par_loop_all__update_pos_kernel (do_update != 0; apps/app_neso_advection/kernels.h:41-54) then
- * opp_particle_move__move_kernel (apps/app_neso_advection/kernels.h:56-113;
- * apps_codegen/app_neso_advection_mdir_cg/hip/move_kernel_loop.hpp:166-420) over counters->size particles.
- * Hop statistics go to counters; `hops` (nullable) receives per-particle hop counts. */
+#define OPPX_NESO_UPDATE 1
+#define OPPX_NESO_MOVE 2
+/* flags & OPPX_NESO_UPDATE: opp_par_loop_all__update_pos_kernel (apps/app_neso_advection/kernels.h:41-54);
+ * flags & OPPX_NESO_MOVE: opp_particle_move__move_kernel (apps/app_neso_advection/kernels.h:56-113;
+ * apps_codegen/app_neso_advection_mdir_cg/hip/move_kernel_loop.hpp:166-420); both in one pass when both are set,
+ * over counters->size particles. The counters' hop statistics are reset first; `hops` (nullable) receives
+ * per-particle hop counts. */
 int oppx_neso_step(int64_t n_max, const oppx_neso_parts* parts, const oppx_quad_cell* cells, int64_t n_cells,
-                   double dt, double cell_width, double ext_x, double ext_y, int do_update,
+                   double dt, double cell_width, double ext_x, double ext_y, int flags,
                    oppx_move_counters* counters, int32_t* hops, void* stream);
 
 /* ---------------------------------------------------------------------------------------
@@ -256,6 +269,17 @@ size_t oppx_holefill_scratch_bytes(int64_t n_max);
  * counters->size -= removed and counters->holes_low. */
 int oppx_holefill(int64_t n_max, int64_t stride, oppx_move_counters* counters, const uint64_t* hole_bits,
                   const oppx_dat_desc* dats, int n_dats, void* scratch, size_t scratch_bytes, void* stream);
+
+/* FP32 compensated deposit over cell-sorted particles (p2c ascending, e.g. after oppx_sort_by_cell):
+ * per-cell Kahan sums of lc in index order into cell_partials [4*n_cells] (scratch, zeroed here), then per
+ * node a Kahan sum of its (cell, vertex) partials in node_ref order. Writes node_charge (double, nullable)
+ * and/or node_charge_f32 (nullable). The OPP_INC deposit of apps/app_fempic/kernels.h:164-175 in the
+ * segmented-reduction form of apps_codegen/app_fempic_cg/hip/deposit_charge_on_nodes_kernel_loop.hpp:98-158;
+ * every addition order is fixed by the data (deterministic). */
+int oppx_fempic_deposit_sorted_f32(int64_t n_max, int64_t stride, oppx_move_counters* counters, const int32_t* p2c,
+                                   const double* lc, int64_t n_cells, const int32_t* node_start, const int32_t* node_ref,
+                                   int64_t n_nodes, float* cell_partials, double* node_charge, float* node_charge_f32,
+                                   void* stream);
 
 /* ---------------------------------------------------------------------------------------
  * Sort

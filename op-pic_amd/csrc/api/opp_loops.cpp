@@ -1,0 +1,315 @@
+// Generated-loop entry points of the FEM-PIC and NESO-advection hot paths, served by gfx950
+// kernels through the oppx C-ABI. Applications call them exactly as the translator-generated
+// code does (apps_codegen/app_fempic_cg/fempic_opp.cpp:41-50, apps_codegen/app_neso_advection_mdir_cg/advec_opp.cpp:40-43).
+//
+// Each loop: uploads host-dirty argument dats, launches on the default stream, marks written dats
+// device-newer, and (for moves) runs the fill policy and reads back the new particle count -- the
+// one device->host transfer of opp_finalize_particle_move (opp_lib/src/hip/opp_particle_comm.cpp:115-226).
+
+#include <cfloat>
+#include <map>
+#include <utility>
+
+#include "opp_api_internal.h"
+
+namespace oppx_api {
+
+// derived per-cell records, rebuilt when the source dat/map pointers change
+struct TetAuxCache {
+    const void* vol_d = nullptr;
+    const void* c2c_d = nullptr;
+    int64_t n = 0;
+    oppx_tet_aux* aux = nullptr;
+};
+struct QuadCache {
+    const void* ll_d = nullptr;
+    const void* c2c_d = nullptr;
+    int64_t n = 0;
+    oppx_quad_cell* cells = nullptr;
+};
+static TetAuxCache g_tet;
+static QuadCache g_quad;
+
+void free_loop_caches() {
+    dev_free(g_tet.aux);
+    dev_free(g_quad.cells);
+    g_tet = TetAuxCache();
+    g_quad = QuadCache();
+}
+
+static const oppx_tet_aux* tet_aux(opp_dat c_volume, opp_map c2c) {
+    const int64_t n = c2c->from->size;
+    if (g_tet.vol_d != c_volume->data_d || g_tet.c2c_d != c2c->map_d || g_tet.n != n || !g_tet.aux) {
+        dev_free(g_tet.aux);
+        g_tet.aux = (oppx_tet_aux*)dev_alloc((size_t)n * sizeof(oppx_tet_aux));
+        check(oppx_tet_aux_build((const double*)c_volume->data_d, c2c->map_d, n, g_tet.aux, nullptr), "tet aux");
+        g_tet.vol_d = c_volume->data_d;
+        g_tet.c2c_d = c2c->map_d;
+        g_tet.n = n;
+    }
+    return g_tet.aux;
+}
+
+static double const_real(const char* name) {
+    int bytes = 0;
+    const char* p = opp_const_data(name, &bytes);
+    if (!p || bytes < (int)sizeof(double)) opp_abort(std::string("constant not declared: ") + name);
+    double v;
+    memcpy(&v, p, sizeof(v));
+    return v;
+}
+
+static void particle_dat_descs(opp_set set, std::vector<oppx_dat_desc>& in, std::vector<oppx_dat_desc>& swap) {
+    in.clear();
+    swap.clear();
+    for (opp_dat d : *set->particle_dats) {
+        in.push_back(oppx_dat_desc{d->data_d, d->dim, elem_bytes(d)});
+        swap.push_back(oppx_dat_desc{d->data_swap_d, d->dim, elem_bytes(d)});
+    }
+    if (in.size() > 16) opp_abort("particle sets with more than 16 dats are not supported");
+}
+
+static void all_particle_dats_device_newer(opp_set set) {
+    for (opp_dat d : *set->particle_dats) d->dirty_hd = Dirty::Host;
+}
+
+void sort_particles_device(opp_set set, bool drop_removed) {
+    ensure_capacity_scratch(set);
+    ParticleDev& pd = pdev(set);
+    sync_size_to_device(set);
+    for (opp_dat d : *set->particle_dats)
+        if (d->dirty_hd == Dirty::Device) opp_upload_dat(d);
+    std::vector<oppx_dat_desc> in, out;
+    particle_dat_descs(set, in, out);
+    check(oppx_sort_by_cell(set->size, set->set_capacity, pd.counters, set->cells_set->size,
+                            (const int32_t*)set->mesh_relation_dat->data_d, in.data(), out.data(), (int)in.size(),
+                            nullptr, drop_removed ? 1 : 0, pd.scratch, pd.scratch_bytes, nullptr),
+          "particle sort");
+    for (opp_dat d : *set->particle_dats) std::swap(d->data_d, d->data_swap_d);
+    all_particle_dats_device_newer(set);
+    oppx_move_counters c;
+    read_counters(set, &c);
+    set->size = c.size;
+}
+
+// opp_finalize_particle_move with the fill policy of opp_lib/src/core/opp_particle_organize_core.cpp:49-95
+void finalize_move_device(opp_set set) {
+    ParticleDev& pd = pdev(set);
+    oppx_move_counters c;
+    read_counters(set, &c);
+    set->particle_remove_count = c.removed;
+    OPP_move_max_hops = c.max_hops;
+    if (c.faults) opp_abort("particle move: " + std::to_string(c.faults) + " walks hit an invalid cell or the hop cap");
+    if (c.removed <= 0) return;
+    const bool periodic = (OPP_fill_type == OPP_Sort_Periodic || OPP_fill_type == OPP_Shuffle_Periodic);
+    const bool holefill = OPP_fill_type == OPP_HoleFill_All ||
+                          (periodic && (OPP_main_loop_iter % OPP_fill_period != 0 || OPP_comm_iteration != 0));
+    if (holefill) {
+        std::vector<oppx_dat_desc> in, unused;
+        particle_dat_descs(set, in, unused);
+        check(oppx_holefill(set->size, set->set_capacity, pd.counters, pd.hole_bits, in.data(), (int)in.size(),
+                            pd.scratch, pd.scratch_bytes, nullptr),
+              "hole fill");
+        all_particle_dats_device_newer(set);
+        read_counters(set, &c);
+        set->size = c.size;
+    } else {
+        // Sort_* (and Shuffle_*, whose seq reference is a sort as well): order by cell, removed last
+        sort_particles_device(set, true);
+    }
+}
+
+}  // namespace oppx_api
+
+using namespace oppx_api;
+
+// =============================================================================================
+// FEM-PIC
+// =============================================================================================
+
+// apps_codegen/app_fempic_cg/hip/calculate_new_pos_vel_kernel_loop.hpp; elemental apps/app_fempic/kernels.h:93-103
+void opp_par_loop_all__calculate_new_pos_vel_kernel(opp_set set, opp_arg arg0, opp_arg arg1, opp_arg arg2) {
+    opp_profiler->start("calculate_new_pos_vel_kernel");
+    opp_arg args[3] = {arg0, arg1, arg2};
+    args_to_device(3, args);
+    if (set->mesh_relation_dat->dirty_hd == Dirty::Device) opp_upload_dat(set->mesh_relation_dat);
+    sync_size_to_device(set);
+    const double k = const_real("CONST_charge") / const_real("CONST_mass") * const_real("CONST_dt");
+    oppx_tet_mesh mesh{};
+    mesh.n_cells = arg0.dat->set->size;
+    mesh.ef = (const double*)arg0.dat->data_d;
+    oppx_fempic_parts parts{set->set_capacity, (double*)arg1.dat->data_d, (double*)arg2.dat->data_d, nullptr,
+                            (int32_t*)set->mesh_relation_dat->data_d};
+    check(oppx_fempic_push(set->size, &parts, &mesh, k, const_real("CONST_dt"), pdev(set).counters, nullptr),
+          "calculate_new_pos_vel");
+    args_written_on_device(3, args);
+    opp_profiler->end("calculate_new_pos_vel_kernel");
+}
+
+// apps_codegen/app_fempic_cg/hip/move_kernel_loop.hpp:191-407; elemental apps/app_fempic/kernels.h:106-161
+void opp_particle_move__move_kernel(opp_set set, opp_map c2c_map, opp_map p2c_map, opp_arg arg0, opp_arg arg1,
+                                    opp_arg arg2, opp_arg arg3) {
+    opp_profiler->start("move_kernel");
+    opp_arg args[4] = {arg0, arg1, arg2, arg3};
+    args_to_device(4, args);
+    if (p2c_map->p2c_dat->dirty_hd == Dirty::Device) opp_upload_dat(p2c_map->p2c_dat);
+    opp_init_particle_move(set, 4, args);
+    sync_size_to_device(set);
+    ensure_capacity_scratch(set);
+    oppx_tet_mesh mesh{};
+    mesh.n_cells = c2c_map->from->size;
+    mesh.det = (const double*)arg3.dat->data_d;
+    mesh.aux = tet_aux(arg2.dat, c2c_map);
+    oppx_fempic_parts parts{set->set_capacity, (double*)arg0.dat->data_d, nullptr, (double*)arg1.dat->data_d,
+                            (int32_t*)p2c_map->p2c_dat->data_d};
+    oppx_dh_grid grid{};
+    const oppx_dh_grid* dh = nullptr;
+    if (useGlobalMove && cellMapper) {
+        const opp_point& mn = cellMapper->boundingBox->getGlobalMin();
+        grid.cell = cellMapper->structMeshToCellMapping_d;
+        grid.dims[0] = (int64_t)cellMapper->globalGridDimsX;
+        grid.dims[1] = (int64_t)cellMapper->globalGridDimsY;
+        grid.dims[2] = (int64_t)cellMapper->globalGridDimsZ;
+        grid.min[0] = mn.x;
+        grid.min[1] = mn.y;
+        grid.min[2] = mn.z;
+        grid.inv_gs = cellMapper->oneOverGridSpacing;
+        dh = &grid;
+    }
+    ParticleDev& pd = pdev(set);
+    check(oppx_fempic_move(set->size, &parts, &mesh, dh, pd.counters, pd.hole_bits, nullptr, nullptr), "move_kernel");
+    arg1.dat->dirty_hd = Dirty::Host;
+    p2c_map->p2c_dat->dirty_hd = Dirty::Host;
+    opp_finalize_particle_move(set);
+    opp_profiler->end("move_kernel");
+}
+
+// apps_codegen/app_fempic_cg/hip/deposit_charge_on_nodes_kernel_loop.hpp:39-95 (FP64 atomics)
+void opp_par_loop_all__deposit_charge_on_nodes_kernel(opp_set set, opp_arg arg0, opp_arg arg1, opp_arg arg2,
+                                                      opp_arg arg3, opp_arg arg4) {
+    opp_profiler->start("deposit_charge_on_nodes_kernel");
+    opp_arg args[5] = {arg0, arg1, arg2, arg3, arg4};
+    args_to_device(5, args);
+    if (set->mesh_relation_dat->dirty_hd == Dirty::Device) opp_upload_dat(set->mesh_relation_dat);
+    sync_size_to_device(set);
+    if (!arg1.map || arg1.map->dim != 4 || arg1.idx != 0 || arg4.idx != 3)
+        opp_abort("deposit_charge_on_nodes: expects n_charge_den through c2n indices 0..3");
+    oppx_tet_mesh mesh{};
+    mesh.n_cells = arg1.map->from->size;
+    mesh.c2n = arg1.map->map_d;
+    oppx_fempic_parts parts{set->set_capacity, nullptr, nullptr, (double*)arg0.dat->data_d,
+                            (int32_t*)set->mesh_relation_dat->data_d};
+    check(oppx_fempic_deposit_atomic(set->size, &parts, &mesh, (double*)arg1.dat->data_d, pdev(set).counters, nullptr),
+          "deposit_charge_on_nodes");
+    arg1.dat->dirty_hd = Dirty::Host;
+    opp_profiler->end("deposit_charge_on_nodes_kernel");
+}
+
+// apps_codegen/app_fempic_cg/hip/move_kernel_loop.hpp:409-484 (opp_init_direct_hop_cg); overlay per
+// opp_lib/src/seq/opp_direct_hop_seq.cpp:134-265, built with the candidate-restricted search
+void opp_init_direct_hop_cg(double grid_spacing, const opp_dat c_gbl_id, const opp::BoundingBox& b_box,
+                            opp_map c2c_map, opp_map p2c_map, opp_arg arg0, opp_arg arg1, opp_arg arg2, opp_arg arg3) {
+    (void)c2c_map;
+    (void)p2c_map;
+    (void)arg0;
+    (void)arg1;
+    opp_profiler->start("Setup_Mover");
+    useGlobalMove = opp_params->get<OPP_BOOL>("opp_global_move", false);
+    if (useGlobalMove) {
+        boundingBox = std::make_shared<opp::BoundingBox>(b_box);
+        cellMapper = std::make_shared<opp::CellMapper>(boundingBox, grid_spacing);
+        if (!opp_params->get<OPP_BOOL>("opp_dh_data_generate", false))
+            opp_abort("opp_init_direct_hop: reading a stored overlay is not supported; set opp_dh_data_generate = true");
+        const opp_point& mn = boundingBox->getGlobalMin();
+        const opp_point& mx = boundingBox->getGlobalMax();
+        const double lo[3] = {mn.x, mn.y, mn.z}, hi[3] = {mx.x, mx.y, mx.z};
+        opp_dat c_vol = arg2.dat, c_det = arg3.dat;
+        if (c_vol->dirty_hd == Dirty::Host) opp_download_dat(c_vol);
+        if (c_det->dirty_hd == Dirty::Host) opp_download_dat(c_det);
+        if (c_gbl_id->dirty_hd == Dirty::Host) opp_download_dat(c_gbl_id);
+        check(oppx_dh_build_overlay_geom(c_det->set->size, (const double*)c_det->data, (const double*)c_vol->data,
+                                         (const int*)c_gbl_id->data, lo, hi, grid_spacing,
+                                         cellMapper->structMeshToCellMapping),
+              "direct-hop overlay");
+        cellMapper->hostToDeviceTransferMappings();
+    }
+    opp_profiler->end("Setup_Mover");
+}
+
+// =============================================================================================
+// NESO-advection (mdir)
+// =============================================================================================
+
+static const oppx_quad_cell* quad_cells(opp_dat c_pos_ll, opp_map c2c) {
+    const int64_t n = c2c->from->size;
+    if (g_quad.ll_d != c_pos_ll->data_d || g_quad.c2c_d != c2c->map_d || g_quad.n != n || !g_quad.cells) {
+        dev_free(g_quad.cells);
+        g_quad.cells = (oppx_quad_cell*)dev_alloc((size_t)n * sizeof(oppx_quad_cell));
+        check(oppx_quad_cells_build((const double*)c_pos_ll->data_d, c2c->map_d, n, g_quad.cells, nullptr), "quad cells");
+        g_quad.ll_d = c_pos_ll->data_d;
+        g_quad.c2c_d = c2c->map_d;
+        g_quad.n = n;
+    }
+    return g_quad.cells;
+}
+
+static void neso_consts(double& dt, double& cw, double& ex, double& ey) {
+    dt = const_real("CONST_dt");
+    cw = const_real("CONST_cell_width");
+    int bytes = 0;
+    const char* p = opp_const_data("CONST_extents", &bytes);
+    if (!p || bytes < 2 * (int)sizeof(double)) opp_abort("constant not declared: CONST_extents");
+    memcpy(&ex, p, sizeof(double));
+    memcpy(&ey, p + sizeof(double), sizeof(double));
+}
+
+// apps_codegen/app_neso_advection_mdir_cg/hip/update_pos_kernel_loop.hpp; elemental apps/app_neso_advection/kernels.h:41-54
+void opp_par_loop_all__update_pos_kernel(opp_set set, opp_arg arg0, opp_arg arg1) {
+    opp_profiler->start("update_pos_kernel");
+    opp_arg args[2] = {arg0, arg1};
+    args_to_device(2, args);
+    sync_size_to_device(set);
+    double dt, cw, ex, ey;
+    neso_consts(dt, cw, ex, ey);
+    oppx_neso_parts parts{set->set_capacity, (double*)arg1.dat->data_d, (const double*)arg0.dat->data_d, nullptr,
+                          nullptr};
+    check(oppx_neso_step(set->size, &parts, nullptr, set->cells_set->size, dt, cw, ex, ey, OPPX_NESO_UPDATE,
+                         pdev(set).counters, nullptr, nullptr),
+          "update_pos");
+    args_written_on_device(2, args);
+    opp_profiler->end("update_pos_kernel");
+}
+
+// apps_codegen/app_neso_advection_mdir_cg/hip/move_kernel_loop.hpp:213-420; elemental kernels.h:56-113
+void opp_particle_move__move_kernel(opp_set set, opp_map c2c_map, opp_map p2c_map, opp_arg arg0, opp_arg arg1,
+                                    opp_arg arg2) {
+    opp_profiler->start("move_kernel");
+    opp_arg args[3] = {arg0, arg1, arg2};
+    args_to_device(3, args);
+    if (p2c_map->p2c_dat->dirty_hd == Dirty::Device) opp_upload_dat(p2c_map->p2c_dat);
+    opp_init_particle_move(set, 3, args);
+    sync_size_to_device(set);
+    double dt, cw, ex, ey;
+    neso_consts(dt, cw, ex, ey);
+    oppx_neso_parts parts{set->set_capacity, (double*)arg0.dat->data_d, nullptr, (int32_t*)arg1.dat->data_d,
+                          (int32_t*)p2c_map->p2c_dat->data_d};
+    check(oppx_neso_step(set->size, &parts, quad_cells(arg2.dat, c2c_map), c2c_map->from->size, dt, cw, ex, ey,
+                         OPPX_NESO_MOVE, pdev(set).counters, nullptr, nullptr),
+          "move_kernel");
+    arg1.dat->dirty_hd = Dirty::Host;
+    p2c_map->p2c_dat->dirty_hd = Dirty::Host;
+    oppx_move_counters c;
+    read_counters(set, &c);
+    if (c.faults) opp_abort("particle move: " + std::to_string(c.faults) + " walks hit an invalid cell or the hop cap");
+    OPP_move_max_hops = c.max_hops;
+    set->particle_remove_count = 0;
+    opp_profiler->end("move_kernel");
+}
+
+// NESO's direct-hop setup (advec_opp.cpp:43): the periodic NESO walk does not use an overlay here
+void opp_init_direct_hop_cg(double grid_spacing, const opp_dat c_gbl_id, const opp::BoundingBox& b_box,
+                            opp_map c2c_map, opp_map p2c_map, opp_arg arg0, opp_arg arg1, opp_arg arg2) {
+    (void)grid_spacing; (void)c_gbl_id; (void)b_box; (void)c2c_map; (void)p2c_map; (void)arg0; (void)arg1; (void)arg2;
+    useGlobalMove = opp_params->get<OPP_BOOL>("opp_global_move", false);
+    if (useGlobalMove) opp_abort("NESO-advection direct hop (2-D overlay) is not supported; set opp_global_move = false");
+}

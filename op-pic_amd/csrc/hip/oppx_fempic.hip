@@ -395,9 +395,16 @@ __global__ void __launch_bounds__(kBlock) fempic_tile_kernel(StepArgs P, uint32_
     }
 
     __syncthreads();
+    int nrem = 0;
     if (threadIdx.x < kWavesPerBlock * tile_words) {
         const int w = threadIdx.x / tile_words, k = threadIdx.x % tile_words;
-        P.hole_bits[((int64_t)b * kWavesPerBlock + w) * tile_words + k] = s_holes[w][k];
+        const unsigned long long bits = s_holes[w][k];
+        P.hole_bits[((int64_t)b * kWavesPerBlock + w) * tile_words + k] = bits;
+        nrem = __popcll(bits);
+    }
+    if (threadIdx.x < kWave) {  // the hole words sit in the first wave's lanes
+        const int tot = wave_sum_i32(nrem);
+        if (threadIdx.x == 0 && tot) atomicAdd(&P.ctr->removed, tot);
     }
     if (DEPOSIT) table_flush(s_table, P.charge);
     const unsigned long long fm = __ballot(nfault > 0);

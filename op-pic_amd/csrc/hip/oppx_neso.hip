@@ -36,7 +36,7 @@ __device__ __forceinline__ double periodic_update(double p, double v, double dt,
 }
 
 template <bool STATS>
-__global__ void __launch_bounds__(kBlock) neso_step_kernel(NesoArgs P, uint32_t nblocks, int do_update) {
+__global__ void __launch_bounds__(kBlock) neso_step_kernel(NesoArgs P, uint32_t nblocks, int flags) {
     const uint32_t b = xcd_block(blockIdx.x, nblocks);
     const int64_t i = (int64_t)b * kBlock + threadIdx.x;
     int64_t n = P.ctr->size;
@@ -47,13 +47,13 @@ __global__ void __launch_bounds__(kBlock) neso_step_kernel(NesoArgs P, uint32_t 
     bool fault = false;
     if (live) {
         double x = P.pos[i], y = P.pos[S + i];
-        if (do_update) {
+        if (flags & OPPX_NESO_UPDATE) {
             x = periodic_update(x, P.vel[i], P.dt, P.ext_x);
             y = periodic_update(y, P.vel[S + i], P.dt, P.ext_y);
             P.pos[i] = x;
             P.pos[S + i] = y;
         }
-        int cell = P.p2c[i];
+        int cell = (flags & OPPX_NESO_MOVE) ? P.p2c[i] : kMaxCell;
         if (cell != kMaxCell) {
             if (!((unsigned)cell < (unsigned)P.n_cells)) {
                 fault = true;
@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(kBlock) neso_step_kernel(NesoArgs P, uint32_t 
                 if (!fault) P.p2c[i] = cell;
             }
         }
-        if (P.hops) P.hops[i] = h;
+        if (P.hops && (flags & OPPX_NESO_MOVE)) P.hops[i] = h;
     }
     const int lane = threadIdx.x & (kWave - 1);
     const unsigned long long fmask = __ballot(fault);
@@ -152,9 +152,12 @@ int oppx_quad_cells_build(const double* c_pos_ll, const int32_t* c2c, int64_t n_
 }
 
 int oppx_neso_step(int64_t n_max, const oppx_neso_parts* parts, const oppx_quad_cell* cells, int64_t n_cells,
-                   double dt, double cell_width, double ext_x, double ext_y, int do_update,
+                   double dt, double cell_width, double ext_x, double ext_y, int flags,
                    oppx_move_counters* counters, int32_t* hops, void* stream) {
-    OPPX_ARG_CHECK(parts && cells && counters, "oppx_neso_step: null pointer");
+    OPPX_ARG_CHECK(parts && counters, "oppx_neso_step: null pointer");
+    OPPX_ARG_CHECK(flags >= 1 && flags <= 3, "oppx_neso_step: flags must be OPPX_NESO_UPDATE and/or OPPX_NESO_MOVE");
+    OPPX_ARG_CHECK(!(flags & OPPX_NESO_MOVE) || (cells && parts->mdir && parts->p2c), "oppx_neso_step: move needs cells, mdir, p2c");
+    OPPX_ARG_CHECK(!(flags & OPPX_NESO_UPDATE) || (parts->pos && parts->vel), "oppx_neso_step: update needs pos, vel");
     OPPX_ARG_CHECK(n_cells > 0 && n_cells < kMaxCell, "oppx_neso_step: n_cells out of range");
     OPPX_ARG_CHECK(n_max >= 0 && n_max <= parts->stride, "oppx_neso_step: n_max beyond capacity");
     OPPX_ARG_CHECK(cell_width > 0 && ext_x > 0 && ext_y > 0, "oppx_neso_step: widths must be positive");
@@ -178,7 +181,7 @@ int oppx_neso_step(int64_t n_max, const oppx_neso_parts* parts, const oppx_quad_
     P.ctr = counters;
     P.hops = hops;
     const unsigned g = grid_for(n_max, kBlock);
-    neso_step_kernel<true><<<g, kBlock, 0, s>>>(P, g, do_update);
+    neso_step_kernel<true><<<g, kBlock, 0, s>>>(P, g, flags);
     OPPX_LAUNCH_CHECK("neso_step");
     return OPPX_OK;
 }

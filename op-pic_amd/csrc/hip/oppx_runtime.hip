@@ -2,6 +2,8 @@
 // Replaces opp_lib/src/hip/opp_hip.cpp:221-271 (device init) and the host-side layout
 // conversions of opp_lib/src/hip/opp_hip_utils.cpp:35-153, which here run on the device.
 
+#include <cstring>
+
 #include "oppx_internal.h"
 
 namespace oppx {
@@ -30,6 +32,24 @@ __global__ void soa_to_aos_kernel(const T* __restrict__ soa, T* __restrict__ aos
     if (t >= n * dim) return;
     const int64_t i = t / dim, d = t % dim;
     aos[t] = soa[d * stride + i];
+}
+
+struct Pattern {
+    unsigned char b[256];
+};
+
+// element i, component d = pattern[d]; AoS when stride == 0, else SoA with the given stride
+template <typename T>
+__global__ void fill_pattern_kernel(T* __restrict__ dst, int64_t n, int dim, int64_t stride, Pattern pat) {
+    const T* pv = reinterpret_cast<const T*>(pat.b);
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * dim; t += (int64_t)gridDim.x * blockDim.x) {
+        if (stride == 0) {
+            dst[t] = pv[t % dim];
+        } else {
+            const int64_t d = t / n, i = t % n;
+            dst[d * stride + i] = pv[d];
+        }
+    }
 }
 
 }  // namespace
@@ -128,6 +148,24 @@ int oppx_soa_to_aos(const void* soa, void* aos, int64_t n, int dim, int64_t stri
     else
         soa_to_aos_kernel<int32_t><<<g, 256, 0, as_stream(stream)>>>((const int32_t*)soa, (int32_t*)aos, n, dim, stride);
     OPPX_LAUNCH_CHECK("soa_to_aos");
+    return OPPX_OK;
+}
+
+int oppx_fill_pattern(void* dst, int64_t n, int dim, int elem_bytes, const void* pattern, int64_t stride, void* stream) {
+    OPPX_ARG_CHECK(n >= 0 && dim > 0 && (elem_bytes == 4 || elem_bytes == 8), "oppx_fill_pattern: bad sizes");
+    OPPX_ARG_CHECK((int64_t)dim * elem_bytes <= 256, "oppx_fill_pattern: pattern above 256 bytes");
+    OPPX_ARG_CHECK(stride == 0 || stride >= n, "oppx_fill_pattern: stride below n");
+    if (n == 0) return OPPX_OK;
+    OPPX_ARG_CHECK(dst && pattern, "oppx_fill_pattern: null pointer");
+    Pattern p{};
+    memcpy(p.b, pattern, (size_t)dim * elem_bytes);
+    int64_t g = grid_for(n * dim, 256);
+    if (g > 8192) g = 8192;
+    if (elem_bytes == 8)
+        fill_pattern_kernel<double><<<(unsigned)g, 256, 0, as_stream(stream)>>>((double*)dst, n, dim, stride, p);
+    else
+        fill_pattern_kernel<int32_t><<<(unsigned)g, 256, 0, as_stream(stream)>>>((int32_t*)dst, n, dim, stride, p);
+    OPPX_LAUNCH_CHECK("fill_pattern");
     return OPPX_OK;
 }
 

@@ -220,6 +220,23 @@ int oppx_box_mesh_build(int nx, int ny, int nz, double lx, double ly, double lz,
     return OPPX_OK;
 }
 
+int oppx_node_cell_csr(int64_t n_cells, int64_t n_nodes, const int* c2n, int* node_start, int* node_ref) {
+    // For every node, the entries c*4+k with c2n[4c+k] == node, in ascending (c, k) order: the
+    // fixed summation order of the sorted FP32 deposit.
+    if (n_cells <= 0 || n_nodes <= 0 || !c2n || !node_start || !node_ref) return OPPX_ERR_ARG;
+    if (4 * n_cells >= (int64_t)INT_MAX) return OPPX_ERR_ARG;
+    std::vector<int64_t> cnt(n_nodes + 1, 0);
+    for (int64_t e = 0; e < 4 * n_cells; e++) {
+        const int v = c2n[e];
+        if (v < 0 || v >= n_nodes) return OPPX_ERR_ARG;
+        cnt[v + 1]++;
+    }
+    for (int64_t v = 0; v < n_nodes; v++) cnt[v + 1] += cnt[v];
+    for (int64_t v = 0; v <= n_nodes; v++) node_start[v] = (int)cnt[v];
+    for (int64_t e = 0; e < 4 * n_cells; e++) node_ref[cnt[c2n[e]]++] = (int)e;
+    return OPPX_OK;
+}
+
 int oppx_box_field_sines(int64_t n_nodes, int64_t n_cells, const double* node_pos, const int* c2n,
                          const double* c_det, const double* c_vol, double* c_ef) {
     // E = -grad(phi_h), phi_h the P1 interpolant of phi = sin(pi x) sin(pi y) sin(pi z).
@@ -293,32 +310,24 @@ int oppx_dh_grid_dims(const double bmin[3], const double bmax[3], double gs, int
     return OPPX_OK;
 }
 
-int oppx_dh_build_overlay_nodes(int64_t n_cells, const double* c_det, const double* c_vol, const int* c_gbl_id,
-                                const int* c2n, const double* node_pos, const double bmin[3],
-                                const double bmax[3], double gs, int* overlay) {
-    if (n_cells <= 0 || !c_det || !c_vol || !c2n || !node_pos || !bmin || !bmax || !overlay)
-        return OPPX_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// Overlay builder over per-cell bounding boxes box[6c..6c+5] = {min xyz, max xyz}.
+int build_overlay(int64_t n_cells, const double* c_det, const double* c_vol, const int* c_gbl_id,
+                  std::vector<double>& box, const double bmin[3], const double bmax[3], double gs, int* overlay) {
     int64_t dims[3];
     int rc = oppx_dh_grid_dims(bmin, bmax, gs, dims);
     if (rc != OPPX_OK) return rc;
     const int64_t X = dims[0], XY = dims[0] * dims[1];
     const int64_t gsize = dims[0] * dims[1] * dims[2];
-
-    // ---- bin the tets by their widened bounding boxes (bins listed in ascending tet id)
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    std::vector<double> box(6 * (size_t)n_cells);
-    for (int64_t c = 0; c < n_cells; c++) {
-        double* b = &box[6 * c];
-        for (int d = 0; d < 3; d++) { b[d] = 1e300; b[3 + d] = -1e300; }
-        for (int v = 0; v < 4; v++) {
-            const double* p = node_pos + 3 * (int64_t)c2n[4 * c + v];
-            for (int d = 0; d < 3; d++) {
-                b[d] = std::min(b[d], p[d]);
-                b[3 + d] = std::max(b[3 + d], p[d]);
-            }
+    for (int64_t c = 0; c < n_cells; c++)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], box[6 * c + d]);
+            hi[d] = std::max(hi[d], box[6 * c + 3 + d]);
         }
-        for (int d = 0; d < 3; d++) { lo[d] = std::min(lo[d], b[d]); hi[d] = std::max(hi[d], b[3 + d]); }
-    }
     double ext = 0.0;
     for (int d = 0; d < 3; d++) ext = std::max(ext, hi[d] - lo[d]);
     const double margin = 1e-7 * ext;
@@ -413,5 +422,73 @@ int oppx_dh_build_overlay_nodes(int64_t n_cells, const double* c_det, const doub
     }
     return OPPX_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+int oppx_dh_build_overlay_nodes(int64_t n_cells, const double* c_det, const double* c_vol, const int* c_gbl_id,
+                                const int* c2n, const double* node_pos, const double bmin[3],
+                                const double bmax[3], double gs, int* overlay) {
+    if (n_cells <= 0 || !c_det || !c_vol || !c2n || !node_pos || !bmin || !bmax || !overlay)
+        return OPPX_ERR_ARG;
+    std::vector<double> box(6 * (size_t)n_cells);
+    for (int64_t c = 0; c < n_cells; c++) {
+        double* b = &box[6 * c];
+        for (int d = 0; d < 3; d++) { b[d] = 1e300; b[3 + d] = -1e300; }
+        for (int v = 0; v < 4; v++) {
+            const double* p = node_pos + 3 * (int64_t)c2n[4 * c + v];
+            for (int d = 0; d < 3; d++) {
+                b[d] = std::min(b[d], p[d]);
+                b[3 + d] = std::max(b[3 + d], p[d]);
+            }
+        }
+    }
+    return build_overlay(n_cells, c_det, c_vol, c_gbl_id, box, bmin, bmax, gs, overlay);
+}
+
+int oppx_dh_build_overlay_geom(int64_t n_cells, const double* c_det, const double* c_vol, const int* c_gbl_id,
+                               const double bmin[3], const double bmax[3], double gs, int* overlay) {
+    // Vertex j of a tet is where the three barycentric functions of the other vertices vanish:
+    // -b_i x + g_i y - d_i z = -a_i for i != j (Cramer's rule); the boxes only need to be conservative.
+    if (n_cells <= 0 || !c_det || !c_vol || !bmin || !bmax || !overlay) return OPPX_ERR_ARG;
+    std::vector<double> box(6 * (size_t)n_cells);
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < n_cells; c++) {
+        const double* t = c_det + 16 * c;
+        double* b = &box[6 * c];
+        for (int d = 0; d < 3; d++) { b[d] = 1e300; b[3 + d] = -1e300; }
+        for (int j = 0; j < 4; j++) {
+            double A[3][3], r[3];
+            int row = 0;
+            for (int i = 0; i < 4; i++) {
+                if (i == j) continue;
+                A[row][0] = -t[4 * i + 1];
+                A[row][1] = t[4 * i + 2];
+                A[row][2] = -t[4 * i + 3];
+                r[row] = -t[4 * i + 0];
+                row++;
+            }
+            const double D = det3(A);
+            if (D == 0.0 || !std::isfinite(D)) {  // degenerate tet: contains no point, never a candidate
+                for (int d = 0; d < 3; d++) { b[d] = 1e300; b[3 + d] = -1e300; }
+                break;
+            }
+            double v[3];
+            for (int k = 0; k < 3; k++) {
+                double M[3][3];
+                for (int q = 0; q < 3; q++)
+                    for (int w = 0; w < 3; w++) M[q][w] = (w == k) ? r[q] : A[q][w];
+                v[k] = det3(M) / D;
+            }
+            for (int d = 0; d < 3; d++) {
+                b[d] = std::min(b[d], v[d]);
+                b[3 + d] = std::max(b[3 + d], v[d]);
+            }
+        }
+    }
+    return build_overlay(n_cells, c_det, c_vol, c_gbl_id, box, bmin, bmax, gs, overlay);
+}
+
 
 }  // extern "C"

@@ -71,6 +71,7 @@ _SIGS = {
     "oppx_memcpy_dtod": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "oppx_memset_zero": (c_int, [c_void_p, c_size_t, c_void_p]),
     "oppx_stream_sync": (c_int, [c_void_p]),
+    "oppx_fill_pattern": (c_int, [c_void_p, c_int64, c_int, c_int, c_void_p, c_int64, c_void_p]),
     "oppx_aos_to_soa": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int64, c_int, c_void_p]),
     "oppx_soa_to_aos": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int64, c_int, c_void_p]),
     "oppx_box_mesh_sizes": (c_int, [c_int, c_int, c_int, POINTER(c_int64), POINTER(c_int64)]),
@@ -85,6 +86,8 @@ _SIGS = {
     "oppx_neso_mesh": (c_int, [c_int, c_int, c_double, c_void_p, c_void_p]),
     "oppx_neso_particles": (c_int, [c_int64, c_uint64, c_int, c_int, c_double, c_double, c_void_p, c_void_p,
                                     c_void_p]),
+    "oppx_dh_build_overlay_geom": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
+                                           c_void_p]),
     "oppx_tet_aux_build": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "oppx_fempic_push": (c_int, [c_int64, POINTER(FempicParts), POINTER(TetMesh), c_double, c_double,
                                  c_void_p, c_void_p]),
@@ -99,6 +102,9 @@ _SIGS = {
                                c_int, c_void_p, c_void_p, c_void_p]),
     "oppx_fempic_step_begin": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
     "oppx_hole_bits_words": (c_int64, [c_int64]),
+    "oppx_node_cell_csr": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
+    "oppx_fempic_deposit_sorted_f32": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+                                               c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "oppx_sort_scratch_bytes": (c_size_t, [c_int64]),
     "oppx_sort_by_cell": (c_int, [c_int64, c_int64, c_void_p, c_int64, c_void_p, POINTER(DatDesc), POINTER(DatDesc),
                                   c_int, c_void_p, c_int, c_void_p, c_size_t, c_void_p]),
@@ -232,6 +238,24 @@ def get_tuning():
     return int(t.tile), int(t.refill_min)
 
 
+def node_cell_csr(m: dict):
+    start = np.zeros(m["n_nodes"] + 1, np.int32)
+    ref = np.zeros(4 * m["n_cells"], np.int32)
+    check(lib().oppx_node_cell_csr(m["n_cells"], m["n_nodes"], _ptr(np.ascontiguousarray(m["c2n"], np.int32)),
+                                   _ptr(start), _ptr(ref)), "node_cell_csr")
+    return start, ref
+
+
+def dh_overlay_geom(m: dict, bmin, bmax, gs) -> np.ndarray:
+    """Direct-hop overlay built from c_det / c_volume only."""
+    bmin = np.ascontiguousarray(bmin, np.float64)
+    bmax = np.ascontiguousarray(bmax, np.float64)
+    ov = np.zeros(int(np.prod(dh_dims(bmin, bmax, gs))), np.int32)
+    check(lib().oppx_dh_build_overlay_geom(m["n_cells"], _ptr(m["c_det"]), _ptr(m["c_vol"]), None, _ptr(bmin),
+                                           _ptr(bmax), gs, _ptr(ov)), "dh_build_overlay_geom")
+    return ov
+
+
 def sort_particles_by_cell(pos, vel, p2c):
     """Stable reorder of an input population by containing cell (an input-layout choice)."""
     order = np.argsort(p2c, kind="stable")
@@ -268,6 +292,7 @@ class FempicDevice:
         if n and not ((p2c >= 0).all() and (p2c < nc).all()):
             raise ValueError("initial p2c entries out of [0, n_cells)")
         self.nc, self.nn = nc, nn
+        self._mesh_host = dict(n_cells=nc, n_nodes=nn, c2n=mesh["c2n"])
         self.det = t(mesh["c_det"].reshape(-1), torch.float64)
         self.c_vol = t(mesh["c_vol"], torch.float64)
         self.c2c = t(mesh["c2c"].reshape(-1), torch.int32)
@@ -348,6 +373,22 @@ class FempicDevice:
         self.pos, self.vel, self.lc, self.p2c = self._swap
         self._swap = cur
         self._rebind()
+
+    def deposit_sorted_f32(self):
+        """FP32 Kahan deposit over the (cell-sorted) particles; returns the FP32 node charge tensor."""
+        torch = self.torch
+        if not hasattr(self, "node_start"):
+            st, ref = node_cell_csr(self._mesh_host)
+            self.node_start = torch.from_numpy(st).to(self.dev)
+            self.node_ref = torch.from_numpy(ref).to(self.dev)
+            self.cell_partials = torch.zeros(4 * self.nc, dtype=torch.float32, device=self.dev)
+            self.charge_f32 = torch.zeros(self.nn, dtype=torch.float32, device=self.dev)
+        check(lib().oppx_fempic_deposit_sorted_f32(self.n_max, self.cap, self.counters.data_ptr(), self.p2c.data_ptr(),
+                                                   self.lc.data_ptr(), self.nc, self.node_start.data_ptr(),
+                                                   self.node_ref.data_ptr(), self.nn, self.cell_partials.data_ptr(),
+                                                   None, self.charge_f32.data_ptr(), self.stream()),
+              "deposit_sorted_f32")
+        return self.charge_f32
 
     def _rebind(self):
         self.parts = FempicParts(self.cap, self.pos.data_ptr(), self.vel.data_ptr(), self.lc.data_ptr(),
@@ -454,7 +495,7 @@ class NesoDevice:
 
     def step(self, dt, record_hops=False, do_update=True):
         check(lib().oppx_neso_step(self.n, ctypes.byref(self.parts), self.cells.data_ptr(), self.nc, dt, self.cw,
-                                   self.nx * self.cw, self.ny * self.cw, 1 if do_update else 0,
+                                   self.nx * self.cw, self.ny * self.cw, (1 if do_update else 0) | 2,
                                    self.counters.data_ptr(), self.hops.data_ptr() if record_hops else None,
                                    self.stream()), "neso_step")
 

@@ -48,3 +48,35 @@ def test_neso_mesh_periodic():
         assert c2c[i, 0] == (x - 1) % 5 + 5 * y and c2c[i, 1] == (x + 1) % 5 + 5 * y
         assert c2c[i, 2] == x + 5 * ((y - 1) % 4) and c2c[i, 3] == x + 5 * ((y + 1) % 4)
     assert np.array_equal(m["c_pos_ll"][7], [2 * 0.5, 1 * 0.5])
+
+
+OPP_API = [
+    "opp_init(int, char**)", "opp_exit()", "opp_decl_set(int, char const*)",
+    "opp_decl_particle_set(char const*, opp_set_core*)", "opp_decl_particle_set(int, char const*, opp_set_core*)",
+    "opp_decl_map(opp_set_core*, opp_set_core*, int, int*, char const*)",
+    "opp_decl_dat(opp_set_core*, int, opp_data_type, void*, char const*)",
+    "opp_decl_const_impl(int, int, char*, char const*)", "opp_increase_particle_count(opp_set_core*, int)",
+    "opp_reset_dat_impl(opp_dat_core*, char*, opp_reset)", "opp_particle_sort(opp_set_core*)",
+    "opp_init_particle_move(opp_set_core*, int, opp_arg*)", "opp_finalize_particle_move(opp_set_core*)",
+    "opp_par_loop_all__calculate_new_pos_vel_kernel(opp_set_core*, opp_arg, opp_arg, opp_arg)",
+    "opp_particle_move__move_kernel(opp_set_core*, opp_map_core*, opp_map_core*, opp_arg, opp_arg, opp_arg, opp_arg)",
+    "opp_par_loop_all__deposit_charge_on_nodes_kernel(opp_set_core*, opp_arg, opp_arg, opp_arg, opp_arg, opp_arg)",
+    "opp_init_direct_hop_cg(double, opp_dat_core*, opp::BoundingBox const&, opp_map_core*, opp_map_core*, opp_arg, "
+    "opp_arg, opp_arg, opp_arg)",
+    "opp_par_loop_all__update_pos_kernel(opp_set_core*, opp_arg, opp_arg)",
+    "opp_particle_move__move_kernel(opp_set_core*, opp_map_core*, opp_map_core*, opp_arg, opp_arg, opp_arg)",
+]
+
+
+def test_opp_lib_dropin_exports_api(root):
+    """libopp_mi355x.so defines the opp_lib entry points OP-PIC's generated FEM-PIC and NESO mains call."""
+    import subprocess
+
+    so = os.path.join(root, "op-pic_amd", "libopp_mi355x.so")
+    out = subprocess.run(["nm", "-DC", "--defined-only", so], check=True, capture_output=True, text=True).stdout
+    defined = {line.split(" ", 2)[2] for line in out.splitlines() if " T " in line}
+    missing = [s for s in OPP_API if s not in defined]
+    assert not missing, missing
+    # the drop-in links against the C-ABI library, not against any torch or python runtime
+    deps = subprocess.run(["ldd", so], capture_output=True, text=True).stdout
+    assert "liboppx.so" in deps and "torch" not in deps and "python" not in deps

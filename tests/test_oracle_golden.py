@@ -125,6 +125,7 @@ def test_overlay_builders_match_reference(name):
     ref = g["out_overlay"]
     assert np.array_equal(oppx.dh_dims(g["bmin"], g["bmax"], gs), g["out_overlay_dims"])
     assert np.array_equal(oppx.dh_overlay(m, g["bmin"], g["bmax"], gs), ref)  # product builder
+    assert np.array_equal(oppx.dh_overlay_geom(m, g["bmin"], g["bmax"], gs), ref)  # from c_det/c_volume only
     if m["n_cells"] <= 5000:
         assert np.array_equal(oracle.dh_overlay_bruteforce(m, g["bmin"], g["bmax"], gs), ref)
 
