@@ -77,6 +77,65 @@ def cpu_baseline(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, budget_s):
                        f"input ({n0} particles, full 6M-tet mesh and 129^3 overlay), {steps} steps, {t_tot:.1f}s")
 
 
+def load_traffic(workload_tag):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter summary
+    (profiles/<kernel>_<workload>.md holds a line `traffic_bytes_per_launch: <B>`), or None."""
+    path = os.path.join(ROOT, "profiles", f"fempic_first_kernel_{workload_tag}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_peak_gbs(torch, gib=1):
+    """Measured device copy rate (a plain torch copy of `gib` GiB, read + write)."""
+    n = gib * (1 << 30) // 8
+    x = torch.empty(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(5):
+        y.copy_(x)
+    b.record()
+    torch.cuda.synchronize()
+    rate = 5 * 2 * n * 8 / (a.elapsed_time(b) * 1e-3) / 1e9
+    del x, y
+    return rate
+
+
+def f32_deposit_report(torch, gpu):
+    """FP32 compensated sorted deposit against the FP64 atomic deposit of the same (final) state."""
+    gpu.sort(drop_removed=False)  # the segmented sum runs over cell-sorted particles
+    gpu.deposit_sorted_f32()      # first call builds the node -> (cell, vertex) table
+    gpu.charge.zero_()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record()
+    gpu.deposit()
+    ev[1].record()
+    f32 = gpu.deposit_sorted_f32()
+    ev[2].record()
+    torch.cuda.synchronize()
+    rho64 = gpu.charge.cpu().numpy()
+    rho32 = f32.cpu().numpy().astype(np.float64)
+    scale = max(np.abs(rho64).max(), 1e-300)
+    big = np.abs(rho64) > 1e-3 * scale
+    return {
+        "max_abs_err_over_max": float(np.abs(rho32 - rho64).max() / scale),
+        "max_rel_err_nodes_above_1e-3_max": float((np.abs(rho32 - rho64)[big] / np.abs(rho64[big])).max())
+        if big.any() else 0.0,
+        "fp64_atomic_ms": float(ev[0].elapsed_time(ev[1])),
+        "fp32_compensated_ms": float(ev[1].elapsed_time(ev[2])),
+        "particles": int(gpu.read_counters()["size"]),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,8 +146,13 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--order", choices=["cell", "random"], default="cell",
                     help="initial particle order in memory (cell-sorted or generation order)")
+    ap.add_argument("--fill", choices=["holefill", "sort"], default="sort",
+                    help="move finalize: hole fill, or the Sort_Periodic policy (sort by cell every --sort-every "
+                         "steps, hole fill in between)")
+    ap.add_argument("--sort-every", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tag", default="c2", help="workload tag of the profile summaries")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -108,7 +172,9 @@ def main():
     t0 = time.time()
     gpu = oppx.FempicDevice(m, ef, pos, vel, p2c, overlay=ov, bmin=bmin, bmax=bmax, gs=gs)
     torch.cuda.synchronize()
-    log(f"uploaded in {time.time() - t0:.1f}s")
+    log(f"uploaded in {time.time() - t0:.1f}s; tuning {oppx.get_tuning()}")
+    copy_gbs = copy_peak_gbs(torch)
+    log(f"device copy rate {copy_gbs:.0f} GB/s")
     nc, nn = m["n_cells"], m["n_nodes"]
     K, W = args.steps, args.warmup
 
@@ -126,7 +192,10 @@ def main():
         if ev is not None:
             ev[2].record()
         hops_log[s] = gpu.counters[4:6].view(torch.int64)[0]
-        gpu.holefill()
+        if args.fill == "sort" and args.sort_every > 0 and (s + 1) % args.sort_every == 0:
+            gpu.sort(drop_removed=True)
+        else:
+            gpu.holefill()
         if ev is not None:
             ev[3].record()
 
@@ -155,11 +224,22 @@ def main():
     value = float(timed_sizes.sum()) / wall
     step_ms = np.array([evs[s][0].elapsed_time(evs[s][3]) for s in range(K)])
     pass_ms = np.array([evs[s][1].elapsed_time(evs[s][2]) for s in range(K)])
+    fill_ms = np.array([evs[s][2].elapsed_time(evs[s][3]) for s in range(K)])
     alg = np.array([algorithmic_bytes(int(n), nc, nn) for n in timed_sizes], dtype=np.float64)
     achieved = float(alg.sum() / (pass_ms.sum() * 1e-3) / 1e9)
     avg_hops = float(hops[W:].sum() / max(timed_sizes.sum(), 1))
     log(f"timed {K} steps in {wall:.3f}s; live {timed_sizes[0]} -> {sizes[W + K]}; pass avg {pass_ms.mean():.3f} ms;"
         f" step p50 {np.percentile(step_ms, 50):.3f} ms; {avg_hops:.2f} hops/particle")
+    for s in range(0, K, max(1, K // 8)):
+        log(f"  step {W + s:3d}: live {timed_sizes[s]:9d}  pass {pass_ms[s]:.3f} ms  fill {fill_ms[s]:.3f} ms  "
+            f"({alg[s] / (pass_ms[s] * 1e-3) / 1e9:.0f} GB/s)")
+
+    f32 = None
+    try:
+        f32 = f32_deposit_report(torch, gpu)
+        log(f"fp32 compensated deposit: {f32}")
+    except Exception as e:  # reported, never required
+        log(f"fp32 deposit report unavailable: {e}")
 
     cpu = None
     if not args.no_cpu:
@@ -171,6 +251,7 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline unavailable: {e}")
 
+    fill = "hole-fill" if args.fill == "holefill" or args.sort_every <= 0 else f"hole-fill, sort by cell every {args.sort_every} steps"
     out = {
         "metric": METRIC,
         "value": value,
@@ -187,25 +268,27 @@ def main():
         "config": {
             "workload": f"FEM-PIC hot path, box {args.nx}^3 cubes x 6 tets ({nc} tets, {nn} nodes), "
                         f"{args.particles} particles seed {args.seed}, {int(np.prod(oppx.dh_dims(bmin, bmax, gs)))}-box "
-                        f"direct-hop overlay, FP64 push+move+atomic deposit+hole-fill",
+                        f"direct-hop overlay, FP64 push+move+atomic deposit, {fill}",
             "tets": nc, "nodes": nn, "particles": args.particles, "overlay_spacing": gs, "particle_order": args.order,
-            "steps_total": W + K, "deposit": "fp64-atomic", "parallelism": "single-gpu",
+            "steps_total": W + K, "deposit": "fp64-atomic", "fill": fill, "parallelism": "single-gpu",
         },
         "p50_step_ms": float(np.percentile(step_ms, 50)),
         "p99_step_ms": float(np.percentile(step_ms, 99)),
         "hops_per_particle": avg_hops,
         "live_first_last": [int(timed_sizes[0]), int(sizes[W + K])],
         "roofline": {
-            "kernel": "fempic_pass_kernel (push+DH+walk+deposit)",
+            "kernel": "fempic_first_kernel (push + direct hop + walk + FP64 deposit, one lane per particle)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
+            "traffic": load_traffic(args.tag),
             "avg_kernel_ms": float(pass_ms.mean()),
             "bytes_per_launch": float(alg.mean()),
-            "traffic": None,
+            "measured_copy_gbs": copy_gbs,
         },
+        "deposit_fp32_compensated": f32,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -87,20 +87,51 @@ typedef struct oppx_move_counters {
     int64_t total_hops;        /* kernel evaluations of the pass */
     int32_t faults;            /* particles whose walk hit the hop cap or an invalid cell index */
     int32_t pad;
+    int32_t queue[8];          /* per XCD: next block tile of the persistent walk, or worklist length of
+                                  the two-pass move (reset per launch) */
 } oppx_move_counters;
 
 /* Walks longer than this are stopped and counted in oppx_move_counters.faults (the
  * reference would loop forever on a cycling walk). */
 #define OPPX_MAX_HOPS 4096
 
-/* Launch shape of the wavefront-cooperative walk: each wave owns `tile` consecutive particles
- * (multiple of 64, <= 512) and refills its free lanes once `refill_min` of them are idle.
- * Process-wide; chosen by the autotuner (op-pic_amd/autotune.py) or oppx_set_tuning. */
+/* Launch shape of the wavefront-cooperative walk and the sort: each wave owns `tile` consecutive
+ * particles (multiple of 64, <= 512) and refills its free lanes once `refill_min` of them are idle;
+ * `split` = 1 runs push + overlay relocation as a separate coalesced pass before the walk;
+ * `persistent` = 1 launches a resident grid whose blocks take block tiles from per-XCD queues;
+ * `sort_items` (4, 8 or 16) is the elements per thread of the radix sort passes (a block sorts
+ * 256 * sort_items keys); `inline_hops` >= 1 selects the two-pass move (walk evaluations done in the
+ * first, one-lane-per-particle pass before a particle goes to the worklist walk), 0 the
+ * cooperative tile walk. Process-wide; chosen by the autotuner (op-pic_amd/autotune.py) or
+ * oppx_set_tuning. Results do not depend on it. */
 typedef struct oppx_tuning {
     int32_t tile;
     int32_t refill_min;
-    int32_t reserved[6];
+    int32_t split;
+    int32_t persistent;
+    int32_t sort_items;
+    int32_t inline_hops;
+    int32_t first_block;       /* threads per block of the first-hop pass: 256 or 1024 */
+    int32_t reserved;
 } oppx_tuning;
+
+/* One instance of a batched FEM-PIC ensemble. The instances' meshes are concatenated into one
+ * mesh (cells of instance i are [cell_base, cell_base + n_cells), its nodes
+ * [node_base, node_base + n_nodes); c2c, c2n and p2c hold concatenated indices) and their particles
+ * into one particle set. dh_cell (device, nullable) is the instance's overlay holding concatenated
+ * cell indices or MAX_CELL_INDEX. 112 B. */
+typedef struct oppx_batch_inst {
+    int64_t cell_base;
+    int64_t n_cells;
+    int64_t node_base;
+    int64_t n_nodes;
+    const int32_t* dh_cell;
+    int64_t dims[3];
+    double min[3];
+    double inv_gs;
+    double k;                  /* charge / mass * dt */
+    double dt;
+} oppx_batch_inst;
 
 /* One particle dat for compaction/permutation: SoA base, components, element bytes (4 or 8). */
 typedef struct oppx_dat_desc {
@@ -182,15 +213,20 @@ int oppx_tet_aux_build(const double* c_vol, const int32_t* c2c, int64_t n_cells,
 int oppx_fempic_push(int64_t n_max, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh,
                      double k, double dt, oppx_move_counters* counters, void* stream);
 
+/* Device workspace of the two-pass move (first-hop pass + worklist walk) for n_max particles. */
+size_t oppx_fempic_work_bytes(int64_t n_max);
+
 /* opp_particle_move__move_kernel (apps_codegen/app_fempic_cg/hip/move_kernel_loop.hpp:142-407):
  * optional direct-hop relocation (dh may be NULL; opp_lib/include/device_kernels/hip_inline_kernels.h:330-393)
  * then the multi-hop barycentric walk (apps/app_fempic/kernels.h:106-161). Reads counters->size,
- * writes counters->max_hops/total_hops/faults and the removal bitmap hole_bits
- * [oppx_hole_bits_words(n_max)] used by oppx_holefill (which sets counters->removed). `hops` (nullable) receives per-particle hop counts.
+ * writes counters->removed/max_hops/total_hops/faults and the removal bitmap hole_bits
+ * [oppx_hole_bits_words(n_max)] used by oppx_holefill. `hops` (nullable) receives per-particle hop counts.
+ * work (nullable, oppx_fempic_work_bytes(n_max) bytes) selects the two-pass move when
+ * tuning.inline_hops >= 1; without it the cooperative tile walk runs. Results are the same.
  * n_max bounds the live count (grid size); the live count itself is read on the device. */
 int oppx_fempic_move(int64_t n_max, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh,
                      const oppx_dh_grid* dh, oppx_move_counters* counters, uint64_t* hole_bits,
-                     int32_t* hops, void* stream);
+                     void* work, size_t work_bytes, int32_t* hops, void* stream);
 
 /* opp_par_loop_all__deposit_charge_on_nodes_kernel
  * (apps_codegen/app_fempic_cg/hip/deposit_charge_on_nodes_kernel_loop.hpp:39-95, 208-222;
@@ -204,13 +240,25 @@ int oppx_fempic_deposit_atomic(int64_t n_max, const oppx_fempic_parts* parts, co
  * target (opp_lib/src/hip/opp_hip.cpp:548-577) plus opp_init_particle_move's counter reset. */
 int oppx_fempic_step_begin(oppx_move_counters* counters, double* node_charge, int64_t n_nodes, void* stream);
 
-/* Fused step: push + (direct hop) + walk + FP64 atomic deposit in one pass over the particles.
- * Same per-particle results as the three loops above in sequence; node_charge (nullable)
- * is accumulated and the hop statistics add to counters, so a step starts with
- * oppx_fempic_step_begin. */
+/* Fused step: push + (direct hop) + walk + FP64 atomic deposit in one pass over the particles (plus
+ * the worklist walk of the particles still moving after tuning.inline_hops evaluations when `work`
+ * is given, as for oppx_fempic_move). Same per-particle results as the three loops above in
+ * sequence; node_charge (nullable) is accumulated and the hop statistics add to counters, so a
+ * step starts with oppx_fempic_step_begin. */
 int oppx_fempic_step_fused(int64_t n_max, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh,
                            const oppx_dh_grid* dh, double k, double dt, oppx_move_counters* counters,
-                           uint64_t* hole_bits, int32_t* hops, double* node_charge, void* stream);
+                           uint64_t* hole_bits, void* work, size_t work_bytes, int32_t* hops, double* node_charge,
+                           void* stream);
+
+/* Batched fused step over an ensemble of independent instances concatenated as described at
+ * oppx_batch_inst: one launch walks every instance's particles (no reference counterpart: OP-PIC
+ * runs one instance per process). cell_inst (device [mesh->n_cells]) is the instance of every cell,
+ * insts (device [n_insts]) the instance table. Per-particle results are those of running each
+ * instance alone through oppx_fempic_step_fused. */
+int oppx_fempic_batch_step_fused(int64_t n_max, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh,
+                                 const int32_t* cell_inst, const oppx_batch_inst* insts, int32_t n_insts,
+                                 oppx_move_counters* counters, uint64_t* hole_bits, void* work, size_t work_bytes,
+                                 int32_t* hops, double* node_charge, void* stream);
 
 /* ---------------------------------------------------------------------------------------
  * NESO-advection hot path (device)

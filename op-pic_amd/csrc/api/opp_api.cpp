@@ -7,6 +7,7 @@
 // are SoA with stride = set capacity and grow by device-to-device copies; mesh dats/maps keep the
 // host (AoS) layout on the device; move counters live on the device.
 
+#include <algorithm>
 #include <chrono>
 #include <fstream>
 #include <iostream>
@@ -109,7 +110,9 @@ void ensure_capacity_scratch(opp_set set) {
     dev_free(d.hole_bits);
     dev_free(d.scratch);
     d.hole_bits = (uint64_t*)dev_alloc((size_t)oppx_hole_bits_words(cap) * sizeof(uint64_t));
-    d.scratch_bytes = std::max(oppx_holefill_scratch_bytes(cap), oppx_sort_scratch_bytes(cap));
+    // one workspace serves the move's worklist, then the hole fill or the sort (never at once)
+    d.scratch_bytes = std::max({oppx_holefill_scratch_bytes(cap), oppx_sort_scratch_bytes(cap),
+                                oppx_fempic_work_bytes(cap)});
     d.scratch = dev_alloc(d.scratch_bytes);
     d.cap_scratch = cap;
 }

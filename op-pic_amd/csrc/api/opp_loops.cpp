@@ -177,7 +177,9 @@ void opp_particle_move__move_kernel(opp_set set, opp_map c2c_map, opp_map p2c_ma
         dh = &grid;
     }
     ParticleDev& pd = pdev(set);
-    check(oppx_fempic_move(set->size, &parts, &mesh, dh, pd.counters, pd.hole_bits, nullptr, nullptr), "move_kernel");
+    check(oppx_fempic_move(set->size, &parts, &mesh, dh, pd.counters, pd.hole_bits, pd.scratch, pd.scratch_bytes,
+                           nullptr, nullptr),
+          "move_kernel");
     arg1.dat->dirty_hd = Dirty::Host;
     p2c_map->p2c_dat->dirty_hd = Dirty::Host;
     opp_finalize_particle_move(set);

@@ -16,6 +16,14 @@ namespace oppx {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kMaxBlock = 1024;  // largest block of the first-hop pass
+
+// Direct-hop overlay as a kernel sees it (one per instance in batched mode).
+struct DhView {
+    const int32_t* __restrict__ cell;
+    int64_t dx, dy, dz;
+    double mx, my, mz, inv_gs;
+};
 
 struct StepArgs {
     int64_t n_cells;
@@ -28,14 +36,18 @@ struct StepArgs {
     double* __restrict__ vel;
     double* __restrict__ lc;
     int32_t* __restrict__ p2c;
-    const int32_t* __restrict__ dh_cell;
-    int64_t dx, dy, dz;
-    double mx, my, mz, inv_gs;
+    DhView dh;
     double k, dt;
     int64_t n_max;
     int tile;        // particles per wave tile (multiple of 64, <= kMaxTile)
     int refill_min;  // free lanes that trigger a refill
+    int persistent;  // 1: resident grid taking block tiles from the per-XCD queues
+    int n_insts;     // batched mode: instance table size
+    const int32_t* __restrict__ cell_inst;       // batched mode: instance of every cell
+    const oppx_batch_inst* __restrict__ insts;   // batched mode: per-instance overlay, k, dt
     oppx_move_counters* ctr;
+    struct StatSlot* slots;  // nullable: per-block statistics spread over kSlots lines, reduced after the pass
+    int32_t* qcount;         // two-pass move: worklist lengths, one per queue
     uint64_t* __restrict__ hole_bits;
     int32_t* __restrict__ hops;
     double* __restrict__ charge;
@@ -44,14 +56,84 @@ struct StepArgs {
 // findStructuredCellIndex3D (opp_lib/include/opp_direct_hop_core.h:410-426) with the x86
 // size_t conversion made explicit: an axis is inside iff -1 < t < dims (t in (-1,0) truncates
 // to 0); then the overlay owner (kMaxCell = no owner -> remove, opp_lib/include/opp_omp.h:206-212).
-__device__ __forceinline__ int dh_lookup(const StepArgs& P, double x, double y, double z) {
-    const double tx = (x - P.mx) * P.inv_gs;
-    const double ty = (y - P.my) * P.inv_gs;
-    const double tz = (z - P.mz) * P.inv_gs;
-    if (!(tx > -1.0 && tx < (double)P.dx && ty > -1.0 && ty < (double)P.dy && tz > -1.0 && tz < (double)P.dz))
+__device__ __forceinline__ int dh_lookup(const DhView& D, double x, double y, double z) {
+    const double tx = (x - D.mx) * D.inv_gs;
+    const double ty = (y - D.my) * D.inv_gs;
+    const double tz = (z - D.mz) * D.inv_gs;
+    if (!(tx > -1.0 && tx < (double)D.dx && ty > -1.0 && ty < (double)D.dy && tz > -1.0 && tz < (double)D.dz))
         return kMaxCell;
     const int64_t ix = (int64_t)tx, iy = (int64_t)ty, iz = (int64_t)tz;
-    return P.dh_cell[ix + P.dx * (iy + P.dy * iz)];
+    return D.cell[ix + D.dx * (iy + D.dy * iz)];
+}
+
+// Particle dats are touched once per pass: non-temporal loads and stores keep them from evicting the
+// cell records the walk re-reads from L2.
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
+template <typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
+// Per-block statistics go to one of kSlots slots (blockIdx modulo kSlots) instead of the single
+// counters record: thousands of blocks adding to one address serialise in one L2 channel. The
+// slots are zeroed before the pass and folded into the counters by slots_reduce_kernel after it.
+constexpr int kSlots = 256;
+struct StatSlot {
+    unsigned long long hops;
+    int max_hops, removed, faults, pad[3];
+};
+
+__device__ __forceinline__ void block_stats(const StepArgs& P, long long hsum, int hmax, int nrem, int nfault) {
+    __shared__ long long s_sum[kMaxBlock / kWave];
+    __shared__ int s_max[kMaxBlock / kWave], s_rem[kMaxBlock / kWave], s_flt[kMaxBlock / kWave];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const long long hs = wave_sum_i64(hsum);
+    const int hm = wave_max_i32(hmax), nr = wave_sum_i32(nrem), nf = wave_sum_i32(nfault);
+    if (lane == 0) { s_sum[w] = hs; s_max[w] = hm; s_rem[w] = nr; s_flt[w] = nf; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        int m = 0, r = 0, f = 0;
+        for (int q = 0; q < (int)(blockDim.x / kWave); q++) {
+            t += s_sum[q];
+            m = m > s_max[q] ? m : s_max[q];
+            r += s_rem[q];
+            f += s_flt[q];
+        }
+        if (P.slots) {
+            StatSlot* sl = P.slots + (blockIdx.x % kSlots);
+            if (t) atomicAdd(&sl->hops, (unsigned long long)t);
+            if (m) atomicMax(&sl->max_hops, m);
+            if (r) atomicAdd(&sl->removed, r);
+            if (f) atomicAdd(&sl->faults, f);
+        } else {
+            if (t) atomicAdd((unsigned long long*)&P.ctr->total_hops, (unsigned long long)t);
+            if (m) atomicMax(&P.ctr->max_hops, m);
+            if (r) atomicAdd(&P.ctr->removed, r);
+            if (f) atomicAdd(&P.ctr->faults, f);
+        }
+    }
+}
+
+__global__ void slots_reset_kernel(StatSlot* slots, int32_t* qcount) {
+    const int t = threadIdx.x;  // kSlots threads
+    slots[t].hops = 0ull;
+    slots[t].max_hops = 0;
+    slots[t].removed = 0;
+    slots[t].faults = 0;
+    if (qcount) qcount[t] = 0;
+}
+
+__global__ void slots_reduce_kernel(const StatSlot* slots, oppx_move_counters* ctr) {
+    const int t = threadIdx.x;  // kSlots threads
+    const StatSlot sl = slots[t];
+    const long long hs = wave_sum_i64((long long)sl.hops);
+    const int hm = wave_max_i32(sl.max_hops), nr = wave_sum_i32(sl.removed), nf = wave_sum_i32(sl.faults);
+    if ((t & (kWave - 1)) == 0) {
+        if (hs) atomicAdd((unsigned long long*)&ctr->total_hops, (unsigned long long)hs);
+        if (hm) atomicMax(&ctr->max_hops, hm);
+        if (nr) atomicAdd(&ctr->removed, nr);
+        if (nf) atomicAdd(&ctr->faults, nf);
+    }
 }
 
 enum WalkEnd { kWalkDone = 0, kWalkRemoved = 1, kWalkFault = 2 };
@@ -103,7 +185,7 @@ struct DepositTable {
 };
 
 __device__ __forceinline__ void table_clear(DepositTable& T) {
-    for (int q = threadIdx.x; q < kTable; q += kBlock) {
+    for (int q = threadIdx.x; q < kTable; q += blockDim.x) {
         T.key[q] = -1;
         T.val[q] = 0.0;
     }
@@ -130,7 +212,7 @@ __device__ __forceinline__ void table_add(DepositTable& T, double* __restrict__ 
 }
 
 __device__ __forceinline__ void table_flush(DepositTable& T, double* __restrict__ charge) {
-    for (int q = threadIdx.x; q < kTable; q += kBlock) {
+    for (int q = threadIdx.x; q < kTable; q += blockDim.x) {
         const int k = T.key[q];
         if (k >= 0) atomicAdd(charge + k, T.val[q]);
     }
@@ -142,18 +224,22 @@ __global__ void __launch_bounds__(kBlock) fempic_pass_kernel(StepArgs P, uint32_
     const int64_t i = (int64_t)b * kBlock + threadIdx.x;
     int64_t n = P.ctr->size;
     n = n < 0 ? 0 : (n > P.n_max ? P.n_max : n);  // never beyond the host's bound
+    if ((int64_t)b * kBlock >= n) return;         // whole block past the live count
     const bool live = i < n;
     const int64_t S = P.stride;
 
     int cell = live ? P.p2c[i] : kMaxCell;
-    bool fault = live && !((unsigned)cell < (unsigned)P.n_cells);
+    // a particle already holding MAX_CELL_INDEX is skipped, as the reference loops do
+    // (apps_codegen/app_fempic_cg/seq/move_kernel_loop.hpp:92-94)
+    const bool skip = live && cell == kMaxCell;
+    bool fault = live && !skip && !((unsigned)cell < (unsigned)P.n_cells);
     bool removed = false;
     int h = 0;
     double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
     __shared__ DepositTable s_table;
     if (DEPOSIT) table_clear(s_table);
 
-    if (live && !fault && (PUSH || MOVE)) {
+    if (live && !skip && !fault && (PUSH || MOVE)) {
         double x = P.pos[i], y = P.pos[S + i], z = P.pos[2 * S + i];
         if (PUSH) {
             // calculate_new_pos_vel_kernel: v = fma(k, E, v); x = fma(v, dt, x)
@@ -170,14 +256,14 @@ __global__ void __launch_bounds__(kBlock) fempic_pass_kernel(StepArgs P, uint32_
             P.pos[i] = x; P.pos[S + i] = y; P.pos[2 * S + i] = z;
         }
         if (!MOVE && DH) {  // relocation only: the walk runs in a second pass
-            const int c = dh_lookup(P, x, y, z);
+            const int c = dh_lookup(P.dh, x, y, z);
             P.p2c[i] = c;
             removed = (c == kMaxCell);
             if (!removed && !((unsigned)c < (unsigned)P.n_cells)) fault = true;
         }
         if (MOVE) {
             if (DH) {
-                cell = dh_lookup(P, x, y, z);
+                cell = dh_lookup(P.dh, x, y, z);
                 if (cell == kMaxCell) removed = true;
                 else if (!((unsigned)cell < (unsigned)P.n_cells)) fault = true;
             }
@@ -193,7 +279,7 @@ __global__ void __launch_bounds__(kBlock) fempic_pass_kernel(StepArgs P, uint32_
     }
     if (DEPOSIT) {
         __syncthreads();  // table cleared
-        if (live && !removed && !fault) {
+        if (live && !skip && !removed && !fault) {
             if (!MOVE) {
                 l0 = P.lc[i]; l1 = P.lc[S + i]; l2 = P.lc[2 * S + i]; l3 = P.lc[3 * S + i];
             }
@@ -213,24 +299,224 @@ __global__ void __launch_bounds__(kBlock) fempic_pass_kernel(StepArgs P, uint32_
         const unsigned long long rmask = __ballot(live && removed);
         if (lane == 0) P.hole_bits[i >> 6] = rmask;
     }
-    const unsigned long long fmask = __ballot(fault);
-    if (lane == 0 && fmask) atomicAdd(&P.ctr->faults, (int)__popcll(fmask));
-    if (STATS) {
-        __shared__ long long s_sum[kBlock / kWave];
-        __shared__ int s_max[kBlock / kWave];
-        const long long hs = wave_sum_i64(h);
-        const int hm = wave_max_i32(h);
-        const int w = threadIdx.x / kWave;
-        if (lane == 0) { s_sum[w] = hs; s_max[w] = hm; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            long long t = 0;
-            int m = 0;
-            for (int q = 0; q < kBlock / kWave; q++) { t += s_sum[q]; m = m > s_max[q] ? m : s_max[q]; }
-            if (t) atomicAdd((unsigned long long*)&P.ctr->total_hops, (unsigned long long)t);
-            if (m) atomicMax(&P.ctr->max_hops, m);
+    block_stats(P, STATS ? h : 0, STATS ? h : 0, (MOVE && live && removed) ? 1 : 0, fault ? 1 : 0);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Two-pass move. The first pass is one lane per particle over the whole set (full occupancy, every
+// load independent across lanes): push, overlay lookup and up to `inline_hops` evaluations of the
+// elemental move kernel. A particle still moving after that is appended to its XCD's worklist
+// (wave-aggregated append) with its current cell in p2c; the second pass walks only those, on a
+// resident grid, to completion. Per-particle results are those of the single walk.
+
+enum HopResult { kHopInside = 0, kHopMove = 1, kHopExit = 2, kHopBad = 3 };
+
+// One evaluation of apps/app_fempic/kernels.h:106-161 at `cell`: inside -> kHopInside; otherwise
+// cell becomes the face neighbour of the most negative weight (first index wins ties), or
+// kMaxCell when that face is on the boundary.
+__device__ __forceinline__ int hop_eval(const StepArgs& P, double x, double y, double z, int& cell, double& l0,
+                                        double& l1, double& l2, double& l3) {
+    const double2* __restrict__ d = reinterpret_cast<const double2*>(P.det + 16 * (int64_t)cell);
+    const double2 a0 = d[0], a1 = d[1], a2 = d[2], a3 = d[3], a4 = d[4], a5 = d[5], a6 = d[6], a7 = d[7];
+    const double4 au = *reinterpret_cast<const double4*>(P.aux + cell);
+    const double c2 = au.x;
+    l0 = c2 * fma(-a1.y, z, fma(a1.x, y, fma(-a0.y, x, a0.x)));
+    l1 = c2 * fma(-a3.y, z, fma(a3.x, y, fma(-a2.y, x, a2.x)));
+    l2 = c2 * fma(-a5.y, z, fma(a5.x, y, fma(-a4.y, x, a4.x)));
+    l3 = c2 * fma(-a7.y, z, fma(a7.x, y, fma(-a6.y, x, a6.x)));
+    if ((l0 > -1e-10) && ((l0 - 1.0) < 1e-10) && (l1 > -1e-10) && ((l1 - 1.0) < 1e-10) && (l2 > -1e-10) &&
+        ((l2 - 1.0) < 1e-10) && (l3 > -1e-10) && ((l3 - 1.0) < 1e-10))
+        return kHopInside;
+    const int2 n01 = __builtin_bit_cast(int2, au.y), n23 = __builtin_bit_cast(int2, au.z);
+    int nb = n01.x;
+    double ml = l0;
+    if (l1 < ml) { ml = l1; nb = n01.y; }
+    if (l2 < ml) { ml = l2; nb = n23.x; }
+    if (l3 < ml) { ml = l3; nb = n23.y; }
+    if (nb < 0) {
+        cell = kMaxCell;
+        return kHopExit;
+    }
+    if (nb >= P.n_cells) return kHopBad;
+    cell = nb;
+    return kHopMove;
+}
+
+// Worklist of the second pass: one list per XCD, `cap` entries each, {particle, hops so far}.
+struct WorkList {
+    int2* __restrict__ entry;
+    int64_t cap;
+};
+
+template <int BS, bool PUSH, bool DH, bool DEPOSIT, bool BATCH>
+__global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t nblocks, int inline_hops, WorkList W) {
+    const uint32_t b = xcd_block(blockIdx.x, nblocks);
+    const int64_t i = (int64_t)b * BS + threadIdx.x;
+    int64_t n = P.ctr->size;
+    n = n < 0 ? 0 : (n > P.n_max ? P.n_max : n);
+    // blocks past the live count (the grid is sized by the host's bound) leave at once; the
+    // removal-bitmap words past the live count are never read
+    if ((int64_t)b * BS >= n) return;
+    const bool live = i < n;
+    const int64_t S = P.stride;
+    const int lane = threadIdx.x & (kWave - 1);
+
+    int cell = live ? ld_stream(P.p2c + i) : kMaxCell;
+    // a particle already holding MAX_CELL_INDEX is skipped, as the reference loops do
+    // (apps_codegen/app_fempic_cg/seq/move_kernel_loop.hpp:92-94)
+    const bool skip = live && cell == kMaxCell;
+    bool fault = live && !skip && !((unsigned)cell < (unsigned)P.n_cells);
+    bool removed = false, pending = false;
+    int h = 0;
+    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+    __shared__ DepositTable s_table;
+    if (DEPOSIT) table_clear(s_table);
+
+    if (live && !skip && !fault) {
+        double kk = P.k, dt = P.dt;
+        DhView D = P.dh;
+        if (BATCH) {  // the instance of the particle's cell supplies k, dt and its overlay
+            const int in = P.cell_inst[cell];
+            if ((unsigned)in < (unsigned)P.n_insts) {
+                const oppx_batch_inst* I = P.insts + in;
+                kk = I->k;
+                dt = I->dt;
+                D.cell = I->dh_cell;
+                D.dx = I->dims[0]; D.dy = I->dims[1]; D.dz = I->dims[2];
+                D.mx = I->min[0]; D.my = I->min[1]; D.mz = I->min[2];
+                D.inv_gs = I->inv_gs;
+            } else {
+                fault = true;
+            }
+        }
+        if (!fault) {
+            double x = ld_stream(P.pos + i), y = ld_stream(P.pos + S + i), z = ld_stream(P.pos + 2 * S + i);
+            if (PUSH) {
+                // calculate_new_pos_vel_kernel: v = fma(k, E, v); x = fma(v, dt, x)
+                const double* e = P.ef + 3 * (int64_t)cell;
+                const double ex = e[0], ey = e[1], ez = e[2];
+                double vx = ld_stream(P.vel + i), vy = ld_stream(P.vel + S + i), vz = ld_stream(P.vel + 2 * S + i);
+                vx = fma(kk, ex, vx);
+                x = fma(vx, dt, x);
+                vy = fma(kk, ey, vy);
+                y = fma(vy, dt, y);
+                vz = fma(kk, ez, vz);
+                z = fma(vz, dt, z);
+                st_stream(P.vel + i, vx); st_stream(P.vel + S + i, vy); st_stream(P.vel + 2 * S + i, vz);
+                st_stream(P.pos + i, x); st_stream(P.pos + S + i, y); st_stream(P.pos + 2 * S + i, z);
+            }
+            if (DH && (!BATCH || D.cell != nullptr)) {
+                cell = dh_lookup(D, x, y, z);
+                if (cell == kMaxCell) removed = true;
+                else if (!((unsigned)cell < (unsigned)P.n_cells)) fault = true;
+            }
+            while (!removed && !fault) {
+                const int r = hop_eval(P, x, y, z, cell, l0, l1, l2, l3);
+                h++;
+                if (r == kHopInside) break;
+                if (r == kHopExit) { removed = true; break; }
+                if (r == kHopBad || h >= OPPX_MAX_HOPS) { fault = true; break; }
+                if (h >= inline_hops) { pending = true; break; }
+            }
+        }
+        if (!fault) {
+            if (!pending && !removed && h > 0) {
+                st_stream(P.lc + i, l0); st_stream(P.lc + S + i, l1);
+                st_stream(P.lc + 2 * S + i, l2); st_stream(P.lc + 3 * S + i, l3);
+            }
+            st_stream(P.p2c + i, removed ? kMaxCell : cell);  // a pending particle resumes from `cell`
+        }
+        if (P.hops) st_stream(P.hops + i, h);
+    }
+
+    // hand the still-moving particles to worklist (blockIdx mod kSlots): same XCD, spread counters
+    const unsigned long long pmask = __ballot(pending);
+    if (pmask) {
+        const uint32_t q = blockIdx.x % kSlots;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&P.qcount[q], __popcll(pmask));
+        base = __shfl(base, 0);
+        if (pending) {
+            const int64_t slot = base + __popcll(pmask & ((1ull << lane) - 1ull));
+            if (slot < W.cap) W.entry[q * W.cap + slot] = make_int2((int)i, h);
         }
     }
+    if (DEPOSIT) {
+        __syncthreads();  // table cleared
+        if (live && !skip && !fault && !removed && !pending) {
+            const int4 m = *reinterpret_cast<const int4*>(P.c2n + 4 * (int64_t)cell);
+            table_add(s_table, P.charge, m.x, l0);
+            table_add(s_table, P.charge, m.y, l1);
+            table_add(s_table, P.charge, m.z, l2);
+            table_add(s_table, P.charge, m.w, l3);
+        }
+        __syncthreads();
+        table_flush(s_table, P.charge);
+    }
+    // removal bitmap: one word per wave, written for every word the grid covers
+    const unsigned long long rmask = __ballot(live && removed);
+    if (lane == 0) P.hole_bits[i >> 6] = rmask;
+    const bool done = live && !skip && !pending;
+    block_stats(P, done ? h : 0, done ? h : 0, (live && removed) ? 1 : 0, fault ? 1 : 0);
+}
+
+template <bool DEPOSIT>
+__global__ void __launch_bounds__(kBlock) fempic_walk_kernel(StepArgs P, WorkList W) {
+    const uint32_t q = blockIdx.x % kSlots;
+    const uint32_t nb_q = gridDim.x / kSlots;  // blocks draining this queue (grid is a multiple of kSlots)
+    const uint32_t k_q = blockIdx.x / kSlots;
+    int64_t m = P.qcount[q];
+    m = m < 0 ? 0 : (m > W.cap ? W.cap : m);
+    const int64_t S = P.stride;
+    __shared__ DepositTable s_table;
+    if (DEPOSIT) table_clear(s_table);
+    __syncthreads();
+
+    long long hsum = 0;
+    int hmax = 0, nrem = 0, nfault = 0;
+    for (int64_t e = (int64_t)k_q * kBlock + threadIdx.x; e < m; e += (int64_t)nb_q * kBlock) {
+        const int2 w = W.entry[q * W.cap + e];
+        const int64_t i = w.x;
+        int h = w.y;
+        int cell = P.p2c[i];
+        const double x = P.pos[i], y = P.pos[S + i], z = P.pos[2 * S + i];
+        double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+        bool removed = false, fault = !((unsigned)cell < (unsigned)P.n_cells);
+        while (!fault) {
+            const int r = hop_eval(P, x, y, z, cell, l0, l1, l2, l3);
+            h++;
+            if (r == kHopInside) break;
+            if (r == kHopExit) { removed = true; break; }
+            if (r == kHopBad || h >= OPPX_MAX_HOPS) fault = true;
+        }
+        if (!fault) {
+            P.lc[i] = l0; P.lc[S + i] = l1; P.lc[2 * S + i] = l2; P.lc[3 * S + i] = l3;
+            P.p2c[i] = removed ? kMaxCell : cell;
+            if (removed) {
+                atomicOr((unsigned long long*)&P.hole_bits[i >> 6], 1ull << (i & 63));
+                nrem++;
+            } else if (DEPOSIT) {
+                const int4 c = *reinterpret_cast<const int4*>(P.c2n + 4 * (int64_t)cell);
+                table_add(s_table, P.charge, c.x, l0);
+                table_add(s_table, P.charge, c.y, l1);
+                table_add(s_table, P.charge, c.z, l2);
+                table_add(s_table, P.charge, c.w, l3);
+            }
+        } else {
+            nfault++;
+        }
+        if (P.hops) P.hops[i] = h;
+        hsum += h;
+        hmax = hmax > h ? hmax : h;
+    }
+    __syncthreads();
+    if (DEPOSIT) table_flush(s_table, P.charge);
+    block_stats(P, hsum, hmax, nrem, nfault);
+}
+
+// Zero the per-XCD worklist / queue counters before a launch that uses them.
+__global__ void reset_queue_kernel(oppx_move_counters* ctr) {
+    if (threadIdx.x < kXcds) ctr->queue[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -244,189 +530,213 @@ constexpr int kMaxTile = 512;
 constexpr int kMaxTileWords = kMaxTile / kWave;
 constexpr int kWavesPerBlock = kBlock / kWave;
 
-template <bool PUSH, bool DH, bool DEPOSIT, bool STATS, bool PRE>
+template <bool PUSH, bool DH, bool DEPOSIT, bool STATS, bool PRE, bool BATCH>
 __global__ void __launch_bounds__(kBlock) fempic_tile_kernel(StepArgs P, uint32_t nblocks) {
-    const uint32_t b = xcd_block(blockIdx.x, nblocks);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
     int64_t n = P.ctr->size;
     n = n < 0 ? 0 : (n > P.n_max ? P.n_max : n);
     const int tile = P.tile, tile_words = P.tile / kWave;
-    const int64_t tile0 = ((int64_t)b * kWavesPerBlock + wave) * tile;
-    const int64_t tile_end = (tile0 + tile < n) ? tile0 + tile : n;
     const int64_t S = P.stride;
 
     __shared__ unsigned long long s_holes[kWavesPerBlock][kMaxTileWords];
     __shared__ DepositTable s_table;
-    if (lane < tile_words) s_holes[wave][lane] = 0ull;
-    if (DEPOSIT) table_clear(s_table);
-    __syncthreads();
+    __shared__ uint32_t s_next;
+
+    // Block tiles (kWavesPerBlock consecutive wave tiles) are split into one contiguous range per
+    // XCD. A plain launch maps block ids onto those ranges; a persistent launch keeps a resident grid
+    // whose blocks take the next block tile of their XCD's range from a queue in the counters.
+    const uint32_t xcd = blockIdx.x % kXcds;
+    const uint32_t q8 = nblocks / kXcds, r8 = nblocks % kXcds;
+    const uint32_t range0 = xcd * q8 + (xcd < r8 ? xcd : r8), range_n = q8 + (xcd < r8 ? 1u : 0u);
 
     const unsigned long long below = (1ull << lane) - 1ull;
-    int64_t next = tile0 + kWave;  // wave-uniform: first particle no lane has taken yet
-    int64_t p = tile0 + lane;
-    bool have = false, walking = false, removed = false, fault = false;
-    int cell = 0, h = 0;
-    double x = 0.0, y = 0.0, z = 0.0, l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
     long long hsum = 0;
-    int hmax = 0, nfault = 0;
+    int hmax = 0, nfault = 0, nrem_total = 0;
 
-    auto begin = [&](int64_t q) {
-        have = true;
-        walking = false;
-        removed = false;
-        fault = false;
-        h = 0;
-        cell = P.p2c[q];
-        if (PRE && cell == kMaxCell) {  // removed by the relocation pass
-            removed = true;
-            return;
+    for (uint32_t round = 0;; round++) {
+        uint32_t b;
+        if (P.persistent) {
+            if (threadIdx.x == 0) s_next = (uint32_t)atomicAdd(&P.ctr->queue[xcd], 1);
+            __syncthreads();
+            const uint32_t t = s_next;
+            __syncthreads();
+            if (t >= range_n) break;  // uniform over the block: every wave leaves here
+            b = range0 + t;
+        } else {
+            if (round > 0) break;
+            b = xcd_block(blockIdx.x, nblocks);
         }
-        if (!((unsigned)cell < (unsigned)P.n_cells)) {
-            fault = true;
-            return;
-        }
-        x = P.pos[q];
-        y = P.pos[S + q];
-        z = P.pos[2 * S + q];
-        if (PUSH) {
-            const double* e = P.ef + 3 * (int64_t)cell;
-            const double ex = e[0], ey = e[1], ez = e[2];
-            double vx = P.vel[q], vy = P.vel[S + q], vz = P.vel[2 * S + q];
-            vx = fma(P.k, ex, vx);
-            x = fma(vx, P.dt, x);
-            vy = fma(P.k, ey, vy);
-            y = fma(vy, P.dt, y);
-            vz = fma(P.k, ez, vz);
-            z = fma(vz, P.dt, z);
-            P.vel[q] = vx; P.vel[S + q] = vy; P.vel[2 * S + q] = vz;
-            P.pos[q] = x; P.pos[S + q] = y; P.pos[2 * S + q] = z;
-        }
-        if (DH) {
-            cell = dh_lookup(P, x, y, z);
+        // block tiles past the live count hold no particle (and later ones of this range neither)
+        if ((int64_t)b * kWavesPerBlock * tile >= n) break;
+        const int64_t tile0 = ((int64_t)b * kWavesPerBlock + wave) * tile;
+        const int64_t tile_end = (tile0 + tile < n) ? tile0 + tile : n;
+
+        if (lane < tile_words) s_holes[wave][lane] = 0ull;
+        if (DEPOSIT) table_clear(s_table);
+        __syncthreads();
+
+        int64_t next = tile0 + kWave;  // wave-uniform: first particle no lane has taken yet
+        int64_t p = tile0 + lane;
+        bool have = false, walking = false, removed = false, fault = false, skip = false;
+        int cell = 0, h = 0;
+        double x = 0.0, y = 0.0, z = 0.0, l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+
+        auto begin = [&](int64_t q) {
+            have = true;
+            walking = false;
+            removed = false;
+            fault = false;
+            skip = false;
+            h = 0;
+            cell = P.p2c[q];
             if (cell == kMaxCell) {
-                removed = true;
+                if (PRE) removed = true;  // removed by the relocation pass
+                else skip = true;         // already removed: skipped, as the reference loops do
                 return;
             }
             if (!((unsigned)cell < (unsigned)P.n_cells)) {
                 fault = true;
                 return;
             }
-        }
-        walking = true;
-    };
-
-    if (p < tile_end) begin(p);
-
-    for (;;) {
-        if (walking) {
-            // one evaluation of the elemental move kernel (apps/app_fempic/kernels.h:106-161)
-            const double2* __restrict__ d = reinterpret_cast<const double2*>(P.det + 16 * (int64_t)cell);
-            const double2 a0 = d[0], a1 = d[1], a2 = d[2], a3 = d[3], a4 = d[4], a5 = d[5], a6 = d[6], a7 = d[7];
-            const double4 au = *reinterpret_cast<const double4*>(P.aux + cell);
-            const double c2 = au.x;
-            l0 = c2 * fma(-a1.y, z, fma(a1.x, y, fma(-a0.y, x, a0.x)));
-            l1 = c2 * fma(-a3.y, z, fma(a3.x, y, fma(-a2.y, x, a2.x)));
-            l2 = c2 * fma(-a5.y, z, fma(a5.x, y, fma(-a4.y, x, a4.x)));
-            l3 = c2 * fma(-a7.y, z, fma(a7.x, y, fma(-a6.y, x, a6.x)));
-            h++;
-            if ((l0 > -1e-10) && ((l0 - 1.0) < 1e-10) && (l1 > -1e-10) && ((l1 - 1.0) < 1e-10) && (l2 > -1e-10) &&
-                ((l2 - 1.0) < 1e-10) && (l3 > -1e-10) && ((l3 - 1.0) < 1e-10)) {
-                walking = false;
-            } else {
-                const int2 n01 = __builtin_bit_cast(int2, au.y), n23 = __builtin_bit_cast(int2, au.z);
-                int nb = n01.x;
-                double ml = l0;
-                if (l1 < ml) { ml = l1; nb = n01.y; }
-                if (l2 < ml) { ml = l2; nb = n23.x; }
-                if (l3 < ml) { ml = l3; nb = n23.y; }
-                if (nb < 0) {
-                    removed = true;
-                    cell = kMaxCell;
-                    walking = false;
-                } else if (nb >= P.n_cells || h >= OPPX_MAX_HOPS) {
+            double kk = P.k, dt = P.dt;
+            DhView D = P.dh;
+            if (BATCH) {  // the instance of the particle's cell supplies k, dt and its overlay
+                const int in = P.cell_inst[cell];
+                if (!((unsigned)in < (unsigned)P.n_insts)) {
                     fault = true;
+                    return;
+                }
+                const oppx_batch_inst* I = P.insts + in;
+                kk = I->k;
+                dt = I->dt;
+                D.cell = I->dh_cell;
+                D.dx = I->dims[0]; D.dy = I->dims[1]; D.dz = I->dims[2];
+                D.mx = I->min[0]; D.my = I->min[1]; D.mz = I->min[2];
+                D.inv_gs = I->inv_gs;
+            }
+            x = P.pos[q];
+            y = P.pos[S + q];
+            z = P.pos[2 * S + q];
+            if (PUSH) {
+                const double* e = P.ef + 3 * (int64_t)cell;
+                const double ex = e[0], ey = e[1], ez = e[2];
+                double vx = P.vel[q], vy = P.vel[S + q], vz = P.vel[2 * S + q];
+                vx = fma(kk, ex, vx);
+                x = fma(vx, dt, x);
+                vy = fma(kk, ey, vy);
+                y = fma(vy, dt, y);
+                vz = fma(kk, ez, vz);
+                z = fma(vz, dt, z);
+                P.vel[q] = vx; P.vel[S + q] = vy; P.vel[2 * S + q] = vz;
+                P.pos[q] = x; P.pos[S + q] = y; P.pos[2 * S + q] = z;
+            }
+            if (DH && (!BATCH || D.cell != nullptr)) {
+                cell = dh_lookup(D, x, y, z);
+                if (cell == kMaxCell) {
+                    removed = true;
+                    return;
+                }
+                if (!((unsigned)cell < (unsigned)P.n_cells)) {
+                    fault = true;
+                    return;
+                }
+            }
+            walking = true;
+        };
+
+        if (p < tile_end) begin(p);
+
+        for (;;) {
+            if (walking) {
+                // one evaluation of the elemental move kernel (apps/app_fempic/kernels.h:106-161)
+                const double2* __restrict__ d = reinterpret_cast<const double2*>(P.det + 16 * (int64_t)cell);
+                const double2 a0 = d[0], a1 = d[1], a2 = d[2], a3 = d[3], a4 = d[4], a5 = d[5], a6 = d[6], a7 = d[7];
+                const double4 au = *reinterpret_cast<const double4*>(P.aux + cell);
+                const double c2 = au.x;
+                l0 = c2 * fma(-a1.y, z, fma(a1.x, y, fma(-a0.y, x, a0.x)));
+                l1 = c2 * fma(-a3.y, z, fma(a3.x, y, fma(-a2.y, x, a2.x)));
+                l2 = c2 * fma(-a5.y, z, fma(a5.x, y, fma(-a4.y, x, a4.x)));
+                l3 = c2 * fma(-a7.y, z, fma(a7.x, y, fma(-a6.y, x, a6.x)));
+                h++;
+                if ((l0 > -1e-10) && ((l0 - 1.0) < 1e-10) && (l1 > -1e-10) && ((l1 - 1.0) < 1e-10) && (l2 > -1e-10) &&
+                    ((l2 - 1.0) < 1e-10) && (l3 > -1e-10) && ((l3 - 1.0) < 1e-10)) {
                     walking = false;
                 } else {
-                    cell = nb;
+                    const int2 n01 = __builtin_bit_cast(int2, au.y), n23 = __builtin_bit_cast(int2, au.z);
+                    int nb = n01.x;
+                    double ml = l0;
+                    if (l1 < ml) { ml = l1; nb = n01.y; }
+                    if (l2 < ml) { ml = l2; nb = n23.x; }
+                    if (l3 < ml) { ml = l3; nb = n23.y; }
+                    if (nb < 0) {
+                        removed = true;
+                        cell = kMaxCell;
+                        walking = false;
+                    } else if (nb >= P.n_cells || h >= OPPX_MAX_HOPS) {
+                        fault = true;
+                        walking = false;
+                    } else {
+                        cell = nb;
+                    }
                 }
+            }
+            if (have && !walking) {  // retire
+                if (skip) {
+                    // nothing to write: the particle keeps MAX_CELL_INDEX
+                } else if (!fault) {
+                    if (h > 0) {
+                        P.lc[p] = l0; P.lc[S + p] = l1; P.lc[2 * S + p] = l2; P.lc[3 * S + p] = l3;
+                    }
+                    P.p2c[p] = removed ? kMaxCell : cell;
+                    if (removed) {
+                        const int64_t r = p - tile0;
+                        atomicOr(&s_holes[wave][r >> 6], 1ull << (r & 63));
+                    } else if (DEPOSIT) {
+                        const int4 m = *reinterpret_cast<const int4*>(P.c2n + 4 * (int64_t)cell);
+                        table_add(s_table, P.charge, m.x, l0);
+                        table_add(s_table, P.charge, m.y, l1);
+                        table_add(s_table, P.charge, m.z, l2);
+                        table_add(s_table, P.charge, m.w, l3);
+                    }
+                } else {
+                    nfault++;
+                }
+                if (P.hops) P.hops[p] = h;
+                if (STATS) {
+                    hsum += h;
+                    hmax = hmax > h ? hmax : h;
+                }
+                have = false;
+            }
+            const unsigned long long free_mask = __ballot(!have);
+            const unsigned long long walk_mask = __ballot(walking);
+            const int nfree = __popcll(free_mask);
+            if (next < tile_end && (nfree >= P.refill_min || walk_mask == 0ull)) {
+                if (!have) {
+                    const int64_t q = next + __popcll(free_mask & below);
+                    if (q < tile_end) {
+                        p = q;
+                        begin(q);
+                    }
+                }
+                next += nfree;
+            } else if (walk_mask == 0ull && __ballot(have) == 0ull) {
+                break;
             }
         }
-        if (have && !walking) {  // retire
-            if (!fault) {
-                if (h > 0) {
-                    P.lc[p] = l0; P.lc[S + p] = l1; P.lc[2 * S + p] = l2; P.lc[3 * S + p] = l3;
-                }
-                P.p2c[p] = removed ? kMaxCell : cell;
-                if (removed) {
-                    const int64_t r = p - tile0;
-                    atomicOr(&s_holes[wave][r >> 6], 1ull << (r & 63));
-                } else if (DEPOSIT) {
-                    const int4 m = *reinterpret_cast<const int4*>(P.c2n + 4 * (int64_t)cell);
-                    table_add(s_table, P.charge, m.x, l0);
-                    table_add(s_table, P.charge, m.y, l1);
-                    table_add(s_table, P.charge, m.z, l2);
-                    table_add(s_table, P.charge, m.w, l3);
-                }
-            } else {
-                nfault++;
-            }
-            if (P.hops) P.hops[p] = h;
-            if (STATS) {
-                hsum += h;
-                hmax = hmax > h ? hmax : h;
-            }
-            have = false;
+
+        __syncthreads();
+        if (threadIdx.x < kWavesPerBlock * tile_words) {
+            const int w = threadIdx.x / tile_words, k = threadIdx.x % tile_words;
+            const unsigned long long bits = s_holes[w][k];
+            P.hole_bits[((int64_t)b * kWavesPerBlock + w) * tile_words + k] = bits;
+            nrem_total += __popcll(bits);
         }
-        const unsigned long long free_mask = __ballot(!have);
-        const unsigned long long walk_mask = __ballot(walking);
-        const int nfree = __popcll(free_mask);
-        if (next < tile_end && (nfree >= P.refill_min || walk_mask == 0ull)) {
-            if (!have) {
-                const int64_t q = next + __popcll(free_mask & below);
-                if (q < tile_end) {
-                    p = q;
-                    begin(q);
-                }
-            }
-            next += nfree;
-        } else if (walk_mask == 0ull && __ballot(have) == 0ull) {
-            break;
-        }
+        if (DEPOSIT) table_flush(s_table, P.charge);
+        __syncthreads();  // the next round clears the table and the hole words
     }
 
-    __syncthreads();
-    int nrem = 0;
-    if (threadIdx.x < kWavesPerBlock * tile_words) {
-        const int w = threadIdx.x / tile_words, k = threadIdx.x % tile_words;
-        const unsigned long long bits = s_holes[w][k];
-        P.hole_bits[((int64_t)b * kWavesPerBlock + w) * tile_words + k] = bits;
-        nrem = __popcll(bits);
-    }
-    if (threadIdx.x < kWave) {  // the hole words sit in the first wave's lanes
-        const int tot = wave_sum_i32(nrem);
-        if (threadIdx.x == 0 && tot) atomicAdd(&P.ctr->removed, tot);
-    }
-    if (DEPOSIT) table_flush(s_table, P.charge);
-    const unsigned long long fm = __ballot(nfault > 0);
-    if (fm) {
-        const int nf = wave_sum_i32(nfault);
-        if (lane == 0) atomicAdd(&P.ctr->faults, nf);
-    }
-    if (STATS) {
-        __shared__ long long s_sum[kWavesPerBlock];
-        __shared__ int s_max[kWavesPerBlock];
-        const long long hs = wave_sum_i64(hsum);
-        const int hm = wave_max_i32(hmax);
-        if (lane == 0) { s_sum[wave] = hs; s_max[wave] = hm; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            long long t = 0;
-            int m = 0;
-            for (int q = 0; q < kWavesPerBlock; q++) { t += s_sum[q]; m = m > s_max[q] ? m : s_max[q]; }
-            if (t) atomicAdd((unsigned long long*)&P.ctr->total_hops, (unsigned long long)t);
-            if (m) atomicMax(&P.ctr->max_hops, m);
-        }
-    }
+    block_stats(P, hsum, hmax, nrem_total, nfault);
 }
 
 // Per-move reset of the counters (a kernel, not a memset, so the step can be graph-captured).
@@ -435,6 +745,7 @@ __global__ void reset_counters_kernel(oppx_move_counters* ctr) {
     ctr->holes_low = 0;
     ctr->max_hops = 0;
     ctr->total_hops = 0;
+    for (int x = 0; x < kXcds; x++) ctr->queue[x] = 0;
 }
 
 // Start of a fused step: zero the deposit target and the per-step counters in one launch.
@@ -446,6 +757,7 @@ __global__ void step_begin_kernel(oppx_move_counters* __restrict__ ctr, double* 
         ctr->max_hops = 0;
         ctr->total_hops = 0;
     }
+    if (j < kXcds) ctr->queue[j] = 0;
     if (charge)
         for (int64_t q = j; q < n_nodes; q += (int64_t)gridDim.x * blockDim.x) charge[q] = 0.0;
 }
@@ -463,8 +775,6 @@ __global__ void tet_aux_kernel(const double* __restrict__ vol, const int32_t* __
     a.pad[0] = a.pad[1] = 0;
     aux[c] = a;
 }
-
-oppx_tuning g_tuning = {256, 16, {0, 0, 0, 0, 0, 0}};
 
 int fill_args(StepArgs& P, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh, const oppx_dh_grid* dh,
               oppx_move_counters* ctr) {
@@ -485,37 +795,126 @@ int fill_args(StepArgs& P, const oppx_fempic_parts* parts, const oppx_tet_mesh* 
     P.ctr = ctr;
     P.tile = g_tuning.tile;
     P.refill_min = g_tuning.refill_min;
+    P.persistent = g_tuning.persistent;
     if (dh) {
         OPPX_ARG_CHECK(dh->inv_gs > 0, "fempic: overlay grid spacing must be positive");
         OPPX_ARG_CHECK(dh->cell && dh->dims[0] > 0 && dh->dims[1] > 0 && dh->dims[2] > 0, "fempic: bad overlay");
-        P.dh_cell = dh->cell;
-        P.dx = dh->dims[0]; P.dy = dh->dims[1]; P.dz = dh->dims[2];
-        P.mx = dh->min[0]; P.my = dh->min[1]; P.mz = dh->min[2];
-        P.inv_gs = dh->inv_gs;
+        P.dh.cell = dh->cell;
+        P.dh.dx = dh->dims[0]; P.dh.dy = dh->dims[1]; P.dh.dz = dh->dims[2];
+        P.dh.mx = dh->min[0]; P.dh.my = dh->min[1]; P.dh.mz = dh->min[2];
+        P.dh.inv_gs = dh->inv_gs;
     }
     return OPPX_OK;
+}
+
+// Blocks of `kernel` resident on the whole device (occupancy query cached per kernel).
+unsigned resident_blocks(const void* kernel) {
+    static int cus = 0, per_cu_cache[64] = {0};
+    static const void* kern_cache[64] = {nullptr};
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    int per_cu = 0;
+    for (int i = 0; i < 64; i++) {
+        if (kern_cache[i] == kernel) { per_cu = per_cu_cache[i]; break; }
+        if (kern_cache[i] == nullptr) {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu <= 0)
+                per_cu = 1;
+            kern_cache[i] = kernel;
+            per_cu_cache[i] = per_cu;
+            break;
+        }
+    }
+    return (unsigned)cus * (unsigned)(per_cu > 0 ? per_cu : 1);
+}
+
+// Grid of a tile-kernel launch: one block per block tile, or (persistent) the resident grid,
+// a multiple of the XCD count so every XCD's queue has blocks draining it.
+unsigned tile_grid(const void* kernel, unsigned block_tiles, bool persistent) {
+    if (!persistent) return block_tiles;
+    unsigned g = resident_blocks(kernel);
+    const unsigned need = ((block_tiles + kXcds - 1) / kXcds) * kXcds;
+    if (g > need) g = need;
+    g = (g / kXcds) * kXcds;
+    return g < (unsigned)kXcds ? (unsigned)kXcds : g;
+}
+
+template <bool PUSH, bool DH, bool DEPOSIT, bool STATS, bool PRE, bool BATCH>
+void launch_tile(const StepArgs& P, int64_t n_max, hipStream_t s) {
+    const int64_t per_block = (int64_t)P.tile * kWavesPerBlock;
+    const unsigned nbt = (unsigned)((n_max + per_block - 1) / per_block);
+    auto kern = fempic_tile_kernel<PUSH, DH, DEPOSIT, STATS, PRE, BATCH>;
+    const unsigned g = tile_grid(reinterpret_cast<const void*>(kern), nbt, P.persistent != 0);
+    if (P.persistent) reset_queue_kernel<<<1, kWave, 0, s>>>(P.ctr);
+    kern<<<g, kBlock, 0, s>>>(P, nbt);
+}
+
+// Work buffer of the move: worklist lengths [kSlots] int32, statistics slots [kSlots], then kSlots
+// worklists of work_cap(n_max) {particle, hops} entries.
+constexpr size_t kWorkHeader = kSlots * sizeof(int32_t) + kSlots * sizeof(StatSlot);
+
+int64_t work_cap(int64_t n_max) {  // per queue, for either first-pass block size
+    const int64_t a = ((n_max + kBlock - 1) / kBlock / kSlots + 1) * kBlock;
+    const int64_t b = ((n_max + kMaxBlock - 1) / kMaxBlock / kSlots + 1) * kMaxBlock;
+    return a > b ? a : b;
+}
+
+size_t work_need(int64_t n_max) { return kWorkHeader + (size_t)kSlots * (size_t)work_cap(n_max) * sizeof(int2); }
+
+// Point P's statistics at the slots of the work buffer (when it holds them) and zero them.
+bool attach_slots(StepArgs& P, void* work, size_t work_bytes, hipStream_t s) {
+    if (!work || work_bytes < kWorkHeader) return false;
+    P.qcount = reinterpret_cast<int32_t*>(work);
+    P.slots = reinterpret_cast<StatSlot*>(reinterpret_cast<char*>(work) + kSlots * sizeof(int32_t));
+    slots_reset_kernel<<<1, kSlots, 0, s>>>(P.slots, P.qcount);
+    return true;
+}
+
+void fold_slots(const StepArgs& P, hipStream_t s) {
+    if (P.slots) slots_reduce_kernel<<<1, kSlots, 0, s>>>(P.slots, P.ctr);
+}
+
+// Two-pass move (first-hop pass + worklist walk) when the work buffer is large enough and the
+// tuning asks for it; returns false when the caller should use the cooperative tile walk instead.
+template <bool PUSH, bool DH, bool DEPOSIT, bool BATCH>
+bool launch_two_pass(const StepArgs& P, int64_t n_max, void* work, size_t work_bytes, hipStream_t s) {
+    if (!P.slots || g_tuning.inline_hops < 1 || work_bytes < work_need(n_max)) return false;
+    WorkList W{reinterpret_cast<int2*>(reinterpret_cast<char*>(work) + kWorkHeader), work_cap(n_max)};
+    if (g_tuning.first_block == kMaxBlock) {
+        const unsigned g = grid_for(n_max, kMaxBlock);
+        fempic_first_kernel<kMaxBlock, PUSH, DH, DEPOSIT, BATCH><<<g, kMaxBlock, 0, s>>>(P, g, g_tuning.inline_hops, W);
+    } else {
+        const unsigned g = grid_for(n_max, kBlock);
+        fempic_first_kernel<kBlock, PUSH, DH, DEPOSIT, BATCH><<<g, kBlock, 0, s>>>(P, g, g_tuning.inline_hops, W);
+    }
+    auto walk = fempic_walk_kernel<DEPOSIT>;
+    unsigned wg = resident_blocks(reinterpret_cast<const void*>(walk));
+    wg = wg < (unsigned)kSlots ? (unsigned)kSlots : (wg / kSlots) * kSlots;
+    walk<<<wg, kBlock, 0, s>>>(P, W);
+    return true;
 }
 
 template <bool PUSH, bool MOVE, bool DEPOSIT>
 void launch_pass(const StepArgs& P, bool dh, bool stats, int64_t n_max, hipStream_t s) {
     if constexpr (MOVE) {
-        const int64_t per_block = (int64_t)P.tile * kWavesPerBlock;
-        const unsigned g = (unsigned)((n_max + per_block - 1) / per_block);
-#define OPPX_TILE(DHV, STV) fempic_tile_kernel<PUSH, DHV, DEPOSIT, STV, false><<<g, kBlock, 0, s>>>(P, g)
         if (dh) {
-            if (stats) OPPX_TILE(true, true); else OPPX_TILE(true, false);
+            if (stats) launch_tile<PUSH, true, DEPOSIT, true, false, false>(P, n_max, s);
+            else launch_tile<PUSH, true, DEPOSIT, false, false, false>(P, n_max, s);
         } else {
-            if (stats) OPPX_TILE(false, true); else OPPX_TILE(false, false);
+            if (stats) launch_tile<PUSH, false, DEPOSIT, true, false, false>(P, n_max, s);
+            else launch_tile<PUSH, false, DEPOSIT, false, false, false>(P, n_max, s);
         }
-#undef OPPX_TILE
     } else {
-    const unsigned g = grid_for(n_max, kBlock);
+        const unsigned g = grid_for(n_max, kBlock);
 #define OPPX_PASS(DHV, STV) fempic_pass_kernel<PUSH, MOVE, DHV, DEPOSIT, STV><<<g, kBlock, 0, s>>>(P, g)
-    if (dh) {
-        if (stats) OPPX_PASS(true, true); else OPPX_PASS(true, false);
-    } else {
-        if (stats) OPPX_PASS(false, true); else OPPX_PASS(false, false);
-    }
+        if (dh) {
+            if (stats) OPPX_PASS(true, true); else OPPX_PASS(true, false);
+        } else {
+            if (stats) OPPX_PASS(false, true); else OPPX_PASS(false, false);
+        }
 #undef OPPX_PASS
     }
 }
@@ -550,8 +949,8 @@ int oppx_fempic_push(int64_t n_max, const oppx_fempic_parts* parts, const oppx_t
 }
 
 int oppx_fempic_move(int64_t n_max, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh,
-                     const oppx_dh_grid* dh, oppx_move_counters* counters, uint64_t* hole_bits, int32_t* hops,
-                     void* stream) {
+                     const oppx_dh_grid* dh, oppx_move_counters* counters, uint64_t* hole_bits, void* work,
+                     size_t work_bytes, int32_t* hops, void* stream) {
     StepArgs P;
     int rc = fill_args(P, parts, mesh, dh, counters);
     if (rc) return rc;
@@ -564,7 +963,11 @@ int oppx_fempic_move(int64_t n_max, const oppx_fempic_parts* parts, const oppx_t
     P.n_max = n_max;
     P.hole_bits = hole_bits;
     P.hops = hops;
-    launch_pass<false, true, false>(P, dh != nullptr, true, n_max, s);
+    attach_slots(P, work, work_bytes, s);
+    const bool two = dh ? launch_two_pass<false, true, false, false>(P, n_max, work, work_bytes, s)
+                        : launch_two_pass<false, false, false, false>(P, n_max, work, work_bytes, s);
+    if (!two) launch_pass<false, true, false>(P, dh != nullptr, true, n_max, s);
+    fold_slots(P, s);
     OPPX_LAUNCH_CHECK("fempic_move");
     return OPPX_OK;
 }
@@ -586,7 +989,8 @@ int oppx_fempic_deposit_atomic(int64_t n_max, const oppx_fempic_parts* parts, co
 
 int oppx_fempic_step_fused(int64_t n_max, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh,
                            const oppx_dh_grid* dh, double k, double dt, oppx_move_counters* counters,
-                           uint64_t* hole_bits, int32_t* hops, double* node_charge, void* stream) {
+                           uint64_t* hole_bits, void* work, size_t work_bytes, int32_t* hops, double* node_charge,
+                           void* stream) {
     StepArgs P;
     int rc = fill_args(P, parts, mesh, dh, counters);
     if (rc) return rc;
@@ -600,29 +1004,74 @@ int oppx_fempic_step_fused(int64_t n_max, const oppx_fempic_parts* parts, const 
     P.hole_bits = hole_bits;
     P.hops = hops;
     P.charge = node_charge;
-    if (g_tuning.reserved[0] == 1 && dh != nullptr) {
+    attach_slots(P, work, work_bytes, s);
+    bool two;
+    if (dh) two = node_charge ? launch_two_pass<true, true, true, false>(P, n_max, work, work_bytes, s)
+                              : launch_two_pass<true, true, false, false>(P, n_max, work, work_bytes, s);
+    else two = node_charge ? launch_two_pass<true, false, true, false>(P, n_max, work, work_bytes, s)
+                           : launch_two_pass<true, false, false, false>(P, n_max, work, work_bytes, s);
+    if (two) {
+    } else if (g_tuning.split == 1 && dh != nullptr) {
         // two passes: coalesced push + overlay relocation, then the cooperative walk (+ deposit)
         fempic_pass_kernel<true, false, true, false, false><<<grid_for(n_max, kBlock), kBlock, 0, s>>>(
             P, grid_for(n_max, kBlock));
         OPPX_LAUNCH_CHECK("fempic_push_relocate");
-        const int64_t per_block = (int64_t)P.tile * kWavesPerBlock;
-        const unsigned g = (unsigned)((n_max + per_block - 1) / per_block);
-        if (node_charge) fempic_tile_kernel<false, false, true, true, true><<<g, kBlock, 0, s>>>(P, g);
-        else fempic_tile_kernel<false, false, false, true, true><<<g, kBlock, 0, s>>>(P, g);
+        if (node_charge) launch_tile<false, false, true, true, true, false>(P, n_max, s);
+        else launch_tile<false, false, false, true, true, false>(P, n_max, s);
     } else if (node_charge) {
         launch_pass<true, true, true>(P, dh != nullptr, true, n_max, s);
     } else {
         launch_pass<true, true, false>(P, dh != nullptr, true, n_max, s);
     }
+    fold_slots(P, s);
     OPPX_LAUNCH_CHECK("fempic_step_fused");
     return OPPX_OK;
 }
+
+int oppx_fempic_batch_step_fused(int64_t n_max, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh,
+                                 const int32_t* cell_inst, const oppx_batch_inst* insts, int32_t n_insts,
+                                 oppx_move_counters* counters, uint64_t* hole_bits, void* work, size_t work_bytes,
+                                 int32_t* hops, double* node_charge, void* stream) {
+    StepArgs P;
+    int rc = fill_args(P, parts, mesh, nullptr, counters);
+    if (rc) return rc;
+    OPPX_ARG_CHECK(cell_inst && insts && n_insts > 0, "oppx_fempic_batch_step_fused: instance tables required");
+    OPPX_ARG_CHECK(n_max >= 0 && n_max <= parts->stride, "oppx_fempic_batch_step_fused: n_max beyond capacity");
+    OPPX_ARG_CHECK(hole_bits != nullptr, "oppx_fempic_batch_step_fused: hole_bits required");
+    if (n_max == 0) return OPPX_OK;
+    P.n_max = n_max;
+    P.hole_bits = hole_bits;
+    P.hops = hops;
+    P.charge = node_charge;
+    P.cell_inst = cell_inst;
+    P.insts = insts;
+    P.n_insts = n_insts;
+    hipStream_t s = as_stream(stream);
+    attach_slots(P, work, work_bytes, s);
+    const bool two = node_charge ? launch_two_pass<true, true, true, true>(P, n_max, work, work_bytes, s)
+                                 : launch_two_pass<true, true, false, true>(P, n_max, work, work_bytes, s);
+    if (!two) {
+        if (node_charge) launch_tile<true, true, true, true, false, true>(P, n_max, s);
+        else launch_tile<true, true, false, true, false, true>(P, n_max, s);
+    }
+    fold_slots(P, s);
+    OPPX_LAUNCH_CHECK("fempic_batch_step_fused");
+    return OPPX_OK;
+}
+
+size_t oppx_fempic_work_bytes(int64_t n_max) { return n_max < 0 ? 0 : work_need(n_max); }
 
 int oppx_set_tuning(const oppx_tuning* t) {
     OPPX_ARG_CHECK(t != nullptr, "oppx_set_tuning: null");
     OPPX_ARG_CHECK(t->tile >= kWave && t->tile <= kMaxTile && t->tile % kWave == 0,
                    "oppx_set_tuning: tile must be a multiple of 64 in [64, 512]");
     OPPX_ARG_CHECK(t->refill_min >= 1 && t->refill_min <= kWave, "oppx_set_tuning: refill_min in [1, 64]");
+    OPPX_ARG_CHECK((t->split == 0 || t->split == 1) && (t->persistent == 0 || t->persistent == 1),
+                   "oppx_set_tuning: split and persistent are 0 or 1");
+    OPPX_ARG_CHECK(t->sort_items == 4 || t->sort_items == 8 || t->sort_items == 16,
+                   "oppx_set_tuning: sort_items must be 4, 8 or 16");
+    OPPX_ARG_CHECK(t->inline_hops >= 0 && t->inline_hops <= 64, "oppx_set_tuning: inline_hops in [0, 64]");
+    OPPX_ARG_CHECK(t->first_block == kBlock || t->first_block == kMaxBlock, "oppx_set_tuning: first_block 256 or 1024");
     g_tuning = *t;
     return OPPX_OK;
 }

@@ -12,6 +12,9 @@ namespace oppx {
 
 void set_error(const std::string& msg);
 
+// Process-wide launch shape (oppx_set_tuning), defined in oppx_runtime.hip.
+extern oppx_tuning g_tuning;
+
 // Returns from the enclosing C-ABI function with OPPX_ERR_HIP when a HIP call fails.
 #define OPPX_HIP_TRY(call)                                                                       \
     do {                                                                                        \

@@ -10,6 +10,8 @@ namespace oppx {
 
 static thread_local std::string g_last_error;
 
+oppx_tuning g_tuning = {256, 16, 0, 0, 16, 64, 256, 0};
+
 void set_error(const std::string& msg) { g_last_error = msg; }
 
 namespace {

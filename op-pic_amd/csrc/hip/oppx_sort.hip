@@ -6,7 +6,7 @@
 // particles by p2c with removed particles (MAX_CELL_INDEX) last. Their tie order differs between backends
 // (std::sort / thrust radix); this build's contract is stable by (p2c, index).
 //
-// Each pass sorts by one 8-bit digit. A block owns kElems consecutive elements; ranks inside the block
+// Each pass sorts by one 8-bit digit. A block owns 256 * sort_items consecutive elements (tuning); ranks inside the block
 // come from wave-level digit matching (8 ballots give the lanes holding the same digit), so equal digits
 // keep their index order and no LDS atomic is contended by nearly sorted keys. Per-block digit counts
 // are scanned digit-major across blocks to give every (digit, block) its output offset.
@@ -17,8 +17,9 @@ namespace oppx {
 namespace {
 
 constexpr int kB = 256;
-constexpr int kItems = 16;
-constexpr int kElems = kB * kItems;  // elements per block
+constexpr int kItems = 16;           // elements per thread of the scan kernels
+constexpr int kElems = kB * kItems;
+constexpr int kMinSortItems = 4;     // smallest radix block (elements per thread) the scratch is sized for
 constexpr int kBins = 256;
 constexpr int kMaxDats = 16;
 
@@ -64,6 +65,7 @@ __global__ void __launch_bounds__(kB) sort_prepare(const int32_t* __restrict__ p
     if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(n_removed, (unsigned long long)__popcll(m));
 }
 
+template <int ITEMS>
 __global__ void __launch_bounds__(kB) sort_hist(const uint32_t* __restrict__ keys, const oppx_move_counters* ctr,
                                                 int64_t n_max, int shift, uint32_t* __restrict__ hist, int64_t nblocks) {
     __shared__ uint32_t s_h[kBins];
@@ -71,8 +73,8 @@ __global__ void __launch_bounds__(kB) sort_hist(const uint32_t* __restrict__ key
     __syncthreads();
     const int64_t n = live_n(ctr, n_max);
     const int lane = threadIdx.x & (kWave - 1);
-    for (int r = 0; r < kItems; r++) {
-        const int64_t i = (int64_t)blockIdx.x * kElems + r * kB + threadIdx.x;
+    for (int r = 0; r < ITEMS; r++) {
+        const int64_t i = (int64_t)blockIdx.x * (kB * ITEMS) + r * kB + threadIdx.x;
         const bool ok = i < n;
         const uint32_t d = ok ? (keys[i] >> shift) & 0xFFu : 0u;
         const unsigned long long act = __ballot(ok);
@@ -83,6 +85,7 @@ __global__ void __launch_bounds__(kB) sort_hist(const uint32_t* __restrict__ key
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = s_h[threadIdx.x];
 }
 
+template <int ITEMS>
 __global__ void __launch_bounds__(kB) sort_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                    const oppx_move_counters* ctr, int64_t n_max, int shift,
                                                    const uint32_t* __restrict__ offs, int64_t nblocks,
@@ -93,10 +96,10 @@ __global__ void __launch_bounds__(kB) sort_scatter(const uint32_t* __restrict__ 
     const unsigned long long below = (1ull << lane) - 1ull;
     s_run[threadIdx.x] = 0;
     const int64_t n = live_n(ctr, n_max);
-    for (int r = 0; r < kItems; r++) {
+    for (int r = 0; r < ITEMS; r++) {
         for (int q = 0; q < kB / kWave; q++) s_wave[q][threadIdx.x] = 0;
         __syncthreads();
-        const int64_t i = (int64_t)blockIdx.x * kElems + r * kB + threadIdx.x;
+        const int64_t i = (int64_t)blockIdx.x * (kB * ITEMS) + r * kB + threadIdx.x;
         const bool ok = i < n;
         const uint32_t key = ok ? kin[i] : 0u;
         const uint32_t val = ok ? vin[i] : 0u;
@@ -221,9 +224,10 @@ struct SortScratch {
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-SortScratch carve(void* base, int64_t n_max) {
+SortScratch carve(void* base, int64_t n_max, int items) {
     SortScratch s;
-    s.nblocks = (n_max + kElems - 1) / kElems;
+    const int64_t per_block = (int64_t)kB * items;
+    s.nblocks = (n_max + per_block - 1) / per_block;
     if (s.nblocks < 1) s.nblocks = 1;
     s.hist_n = s.nblocks * kBins;
     s.scan_blocks = (s.hist_n + kElems - 1) / kElems;
@@ -240,7 +244,8 @@ SortScratch carve(void* base, int64_t n_max) {
 }
 
 size_t scratch_need(int64_t n_max) {
-    const int64_t nb = n_max > 0 ? (n_max + kElems - 1) / kElems : 1;
+    const int64_t per_block = (int64_t)kB * kMinSortItems;  // the largest histogram any sort_items needs
+    const int64_t nb = n_max > 0 ? (n_max + per_block - 1) / per_block : 1;
     const int64_t hist_n = nb * kBins;
     const int64_t scan_blocks = (hist_n + kElems - 1) / kElems;
     return 4 * align256((size_t)(n_max > 0 ? n_max : 1) * 4) + align256((size_t)hist_n * 4) +
@@ -280,7 +285,9 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
         T.eb[q] = dats[q].elem_bytes;
     }
     hipStream_t s = as_stream(stream);
-    SortScratch sc = carve(scratch, n_max);
+    const int items = g_tuning.sort_items;
+    OPPX_ARG_CHECK(items == 4 || items == 8 || items == 16, "oppx_sort_by_cell: tuning.sort_items must be 4, 8 or 16");
+    SortScratch sc = carve(scratch, n_max, items);
     zero_u64<<<1, 1, 0, s>>>(sc.n_removed);  // a kernel, not a memset node, so the sort can be captured
     if (n_max > 0) {
         sort_prepare<<<grid_for(n_max, kB), kB, 0, s>>>(p2c, counters, n_max, n_cells, sc.ka, sc.va, sc.n_removed);
@@ -289,14 +296,20 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
         while (bits < 32 && ((int64_t)1 << bits) <= n_cells) bits++;  // keys are in [0, n_cells]
         uint32_t *kin = sc.ka, *vin = sc.va, *kout = sc.kb, *vout = sc.vb;
         for (int shift = 0; shift < bits; shift += 8) {
-            sort_hist<<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, counters, n_max, shift, sc.hist, sc.nblocks);
+            if (items == 4) sort_hist<4><<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, counters, n_max, shift, sc.hist, sc.nblocks);
+            else if (items == 8) sort_hist<8><<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, counters, n_max, shift, sc.hist, sc.nblocks);
+            else sort_hist<16><<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, counters, n_max, shift, sc.hist, sc.nblocks);
             OPPX_LAUNCH_CHECK("sort_hist");
             scan_reduce<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
             scan_sums<<<1, kB, 0, s>>>(sc.sums, sc.scan_blocks);
             scan_apply<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
             OPPX_LAUNCH_CHECK("sort_scan");
-            sort_scatter<<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, vin, counters, n_max, shift, sc.hist, sc.nblocks,
-                                                              kout, vout);
+#define OPPX_SCATTER(I) sort_scatter<I><<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, vin, counters, n_max, shift, sc.hist, \
+                                                                              sc.nblocks, kout, vout)
+            if (items == 4) OPPX_SCATTER(4);
+            else if (items == 8) OPPX_SCATTER(8);
+            else OPPX_SCATTER(16);
+#undef OPPX_SCATTER
             OPPX_LAUNCH_CHECK("sort_scatter");
             uint32_t* t = kin; kin = kout; kout = t;
             t = vin; vin = vout; vout = t;

@@ -39,7 +39,10 @@ class FempicParts(Structure):
 
 class MoveCounters(Structure):
     _fields_ = [("size", c_int32), ("removed", c_int32), ("holes_low", c_int32), ("max_hops", c_int32),
-                ("total_hops", c_int64), ("faults", c_int32), ("pad", c_int32)]
+                ("total_hops", c_int64), ("faults", c_int32), ("pad", c_int32), ("queue", c_int32 * 8)]
+
+
+COUNTER_WORDS = 16  # int32 words of one oppx_move_counters
 
 
 class QuadCell(Structure):
@@ -51,14 +54,22 @@ class NesoParts(Structure):
 
 
 class Tuning(Structure):
-    _fields_ = [("tile", c_int32), ("refill_min", c_int32), ("reserved", c_int32 * 6)]
+    _fields_ = [("tile", c_int32), ("refill_min", c_int32), ("split", c_int32), ("persistent", c_int32),
+                ("sort_items", c_int32), ("inline_hops", c_int32), ("first_block", c_int32), ("reserved", c_int32)]
+
+
+class BatchInst(Structure):
+    _fields_ = [("cell_base", c_int64), ("n_cells", c_int64), ("node_base", c_int64), ("n_nodes", c_int64),
+                ("dh_cell", c_void_p), ("dims", c_int64 * 3), ("min", c_double * 3), ("inv_gs", c_double),
+                ("k", c_double), ("dt", c_double)]
 
 
 class DatDesc(Structure):
     _fields_ = [("base", c_void_p), ("dim", c_int32), ("elem_bytes", c_int32)]
 
 
-assert ctypes.sizeof(TetAux) == 32 and ctypes.sizeof(MoveCounters) == 32
+assert ctypes.sizeof(TetAux) == 32 and ctypes.sizeof(MoveCounters) == 4 * COUNTER_WORDS
+assert ctypes.sizeof(BatchInst) == 112 and ctypes.sizeof(Tuning) == 32
 
 # name -> (restype, argtypes): every symbol include/oppx.h declares
 _SIGS = {
@@ -91,12 +102,17 @@ _SIGS = {
     "oppx_tet_aux_build": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "oppx_fempic_push": (c_int, [c_int64, POINTER(FempicParts), POINTER(TetMesh), c_double, c_double,
                                  c_void_p, c_void_p]),
+    "oppx_fempic_work_bytes": (ctypes.c_size_t, [c_int64]),
     "oppx_fempic_move": (c_int, [c_int64, POINTER(FempicParts), POINTER(TetMesh), POINTER(DhGrid),
-                                 c_void_p, c_void_p, c_void_p, c_void_p]),
+                                 c_void_p, c_void_p, c_void_p, ctypes.c_size_t, c_void_p, c_void_p]),
     "oppx_fempic_deposit_atomic": (c_int, [c_int64, POINTER(FempicParts), POINTER(TetMesh), c_void_p,
                                            c_void_p, c_void_p]),
     "oppx_fempic_step_fused": (c_int, [c_int64, POINTER(FempicParts), POINTER(TetMesh), POINTER(DhGrid),
-                                       c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+                                       c_double, c_double, c_void_p, c_void_p, c_void_p, ctypes.c_size_t, c_void_p,
+                                       c_void_p, c_void_p]),
+    "oppx_fempic_batch_step_fused": (c_int, [c_int64, POINTER(FempicParts), POINTER(TetMesh), c_void_p, c_void_p,
+                                             c_int32, c_void_p, c_void_p, c_void_p, ctypes.c_size_t, c_void_p,
+                                             c_void_p, c_void_p]),
     "oppx_quad_cells_build": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "oppx_neso_step": (c_int, [c_int64, POINTER(NesoParts), c_void_p, c_int64, c_double, c_double, c_double, c_double,
                                c_int, c_void_p, c_void_p, c_void_p]),
@@ -228,14 +244,19 @@ def neso_particles(n: int, seed: int, m: dict, vmax: float):
     return pos, vel, p2c
 
 
-def set_tuning(tile: int, refill_min: int):
-    check(lib().oppx_set_tuning(ctypes.byref(Tuning(tile, refill_min))), "set_tuning")
+TUNING_FIELDS = ("tile", "refill_min", "split", "persistent", "sort_items", "inline_hops", "first_block")
 
 
-def get_tuning():
+def set_tuning(tile=256, refill_min=16, split=0, persistent=0, sort_items=16, inline_hops=64, first_block=256):
+    """Process-wide launch shape of the walk and the sort (results do not depend on it)."""
+    t = Tuning(tile, refill_min, split, persistent, sort_items, inline_hops, first_block)
+    check(lib().oppx_set_tuning(ctypes.byref(t)), "set_tuning")
+
+
+def get_tuning() -> dict:
     t = Tuning()
     check(lib().oppx_get_tuning(ctypes.byref(t)), "get_tuning")
-    return int(t.tile), int(t.refill_min)
+    return {f: int(getattr(t, f)) for f in TUNING_FIELDS}
 
 
 def node_cell_csr(m: dict):
@@ -313,9 +334,11 @@ class FempicDevice:
         self.p2c[:n] = t(p2c, torch.int32)
         self.parts = FempicParts(cap, self.pos.data_ptr(), self.vel.data_ptr(), self.lc.data_ptr(),
                                  self.p2c.data_ptr())
-        self.counters = torch.zeros(8, dtype=torch.int32, device=self.dev)
+        self.counters = torch.zeros(COUNTER_WORDS, dtype=torch.int32, device=self.dev)
         self.counters[0] = n
         self.hole_bits = torch.zeros(int(L.oppx_hole_bits_words(cap)), dtype=torch.int64, device=self.dev)
+        self.work_bytes = int(L.oppx_fempic_work_bytes(cap))
+        self.work = torch.zeros(max(self.work_bytes, 8), dtype=torch.uint8, device=self.dev)
         self.scratch_bytes = int(L.oppx_holefill_scratch_bytes(cap))
         self.scratch = torch.zeros(self.scratch_bytes, dtype=torch.uint8, device=self.dev)
         self.charge = torch.zeros(nn, dtype=torch.float64, device=self.dev)
@@ -347,8 +370,9 @@ class FempicDevice:
 
     def move(self, record_hops=True):
         check(lib().oppx_fempic_move(self.n_max, ctypes.byref(self.parts), ctypes.byref(self.mesh), self._dh_ref(),
-                                     self.counters.data_ptr(), self.hole_bits.data_ptr(),
-                                     self.hops.data_ptr() if record_hops else None, self.stream()), "fempic_move")
+                                     self.counters.data_ptr(), self.hole_bits.data_ptr(), self.work.data_ptr(),
+                                     self.work_bytes, self.hops.data_ptr() if record_hops else None, self.stream()),
+              "fempic_move")
 
     def holefill(self):
         check(lib().oppx_holefill(self.n_max, self.cap, self.counters.data_ptr(), self.hole_bits.data_ptr(),
@@ -405,12 +429,14 @@ class FempicDevice:
         check(lib().oppx_fempic_step_begin(self.counters.data_ptr(), self.charge.data_ptr(), self.nn, self.stream()),
               "fempic_step_begin")
 
-    def pass_fused(self, charge, mass, dt, record_hops=False):
+    def pass_fused(self, charge, mass, dt, record_hops=False, deposit=True):
         k = charge / mass * dt
         check(lib().oppx_fempic_step_fused(self.n_max, ctypes.byref(self.parts), ctypes.byref(self.mesh),
                                            self._dh_ref(), k, dt, self.counters.data_ptr(),
-                                           self.hole_bits.data_ptr(), self.hops.data_ptr() if record_hops else None,
-                                           self.charge.data_ptr(), self.stream()), "fempic_step_fused")
+                                           self.hole_bits.data_ptr(), self.work.data_ptr(), self.work_bytes,
+                                           self.hops.data_ptr() if record_hops else None,
+                                           self.charge.data_ptr() if deposit else None, self.stream()),
+              "fempic_step_fused")
 
     def step_fused(self, charge, mass, dt, record_hops=False, fill="holefill"):
         """push + direct hop + walk + deposit in one pass, then hole-fill (or a sort finalize)."""
@@ -450,6 +476,122 @@ class FempicDevice:
                     charge=self.charge.cpu().numpy().copy())
 
 
+def concat_instances(instances) -> dict:
+    """Host side of a batched ensemble: one mesh, field and particle set holding every instance,
+    cell / node / p2c indices offset by the instance's base (c2c = -1 stays a boundary), the
+    instance of every cell, and the overlays concatenated with offset cell indices."""
+    if not instances:
+        raise ValueError("empty batch")
+    nc = np.array([int(d["mesh"]["n_cells"]) for d in instances], np.int64)
+    nn = np.array([int(d["mesh"]["n_nodes"]) for d in instances], np.int64)
+    if nc.sum() >= MAX_CELL_INDEX:
+        raise ValueError("concatenated mesh too large for 32-bit cell indices")
+    cb = np.concatenate([[0], np.cumsum(nc)[:-1]]).astype(np.int64)
+    nb = np.concatenate([[0], np.cumsum(nn)[:-1]]).astype(np.int64)
+    cat = np.concatenate
+    for d, n in zip(instances, nc):
+        if len(d["p2c"]) and not ((d["p2c"] >= 0) & (d["p2c"] < n)).all():
+            raise ValueError("instance p2c out of its mesh")
+    mesh = dict(
+        n_cells=int(nc.sum()), n_nodes=int(nn.sum()),
+        c_det=cat([d["mesh"]["c_det"].reshape(-1, 16) for d in instances]),
+        c_vol=cat([d["mesh"]["c_vol"] for d in instances]),
+        c2c=cat([np.where(d["mesh"]["c2c"] >= 0, d["mesh"]["c2c"] + b, d["mesh"]["c2c"]).astype(np.int32)
+                 for d, b in zip(instances, cb)]),
+        c2n=cat([(d["mesh"]["c2n"] + b).astype(np.int32) for d, b in zip(instances, nb)]),
+    )
+    ovs, offs, off = [], [], 0
+    for d, b in zip(instances, cb):
+        ov = d.get("overlay")
+        if ov is None:
+            offs.append(-1)
+            continue
+        if ov.size != int(np.prod(dh_dims(d["bmin"], d["bmax"], d["gs"]))):
+            raise ValueError("overlay size does not match its grid dims")
+        n_i = int(d["mesh"]["n_cells"])
+        if not (((ov >= 0) & (ov < n_i)) | (ov == MAX_CELL_INDEX)).all():
+            raise ValueError("overlay entries must be cells of the instance or MAX_CELL_INDEX")
+        ovs.append(np.where(ov == MAX_CELL_INDEX, ov, ov + b).astype(np.int32))
+        offs.append(off)
+        off += ov.size
+    return dict(mesh=mesh, ef=cat([d["ef"].reshape(-1, 3) for d in instances]),
+                pos=cat([d["pos"] for d in instances]), vel=cat([d["vel"] for d in instances]),
+                p2c=cat([(d["p2c"] + b).astype(np.int32) for d, b in zip(instances, cb)]),
+                cell_base=cb, n_cells=nc, node_base=nb, n_nodes=nn,
+                cell_inst=np.repeat(np.arange(len(instances), dtype=np.int32), nc),
+                overlay=cat(ovs) if ovs else np.zeros(1, np.int32), overlay_off=np.array(offs, np.int64))
+
+
+class FempicBatch(FempicDevice):
+    """An ensemble of independent FEM-PIC instances resident on the GPU as ONE concatenated mesh and
+    ONE particle set, stepped by single launches (oppx_fempic_batch_step_fused, the sort, the FP32
+    deposit) that cover every instance.
+
+    instances: list of dicts with mesh, ef, pos, vel, p2c, charge, mass, dt and optionally overlay,
+    bmin, bmax, gs (direct hop). Cell, node and p2c indices are offset by each instance's base; the
+    Sort_All finalize (stable by concatenated cell) keeps each instance's particles contiguous and in
+    the order the instance would have alone, so every per-instance result equals its alone run.
+    """
+
+    def __init__(self, instances, device="cuda"):
+        import torch
+        u = concat_instances(instances)
+        super().__init__(u["mesh"], u["ef"], u["pos"], u["vel"], u["p2c"], device=device)
+        self.n_inst = len(instances)
+        self.cell_base, self.n_cells_i = u["cell_base"], u["n_cells"]
+        self.node_base, self.n_nodes_i = u["node_base"], u["n_nodes"]
+        self.cell_inst = torch.from_numpy(u["cell_inst"]).to(self.dev)
+        self.overlay_cat = torch.from_numpy(u["overlay"]).to(self.dev)
+        table = (BatchInst * self.n_inst)()
+        for i, d in enumerate(instances):
+            t = table[i]
+            t.cell_base, t.n_cells = int(u["cell_base"][i]), int(u["n_cells"][i])
+            t.node_base, t.n_nodes = int(u["node_base"][i]), int(u["n_nodes"][i])
+            if u["overlay_off"][i] >= 0:
+                dims = dh_dims(d["bmin"], d["bmax"], d["gs"])
+                t.dh_cell = self.overlay_cat.data_ptr() + 4 * int(u["overlay_off"][i])
+                for a in range(3):
+                    t.dims[a] = int(dims[a])
+                    t.min[a] = float(d["bmin"][a])
+                t.inv_gs = 1.0 / float(d["gs"])
+            t.dt = float(d["dt"])
+            t.k = float(d["charge"]) / float(d["mass"]) * float(d["dt"])
+        raw = np.frombuffer(bytes(table), np.uint8).copy()
+        self.inst_table = torch.from_numpy(raw).to(self.dev)
+
+    def pass_batch(self, deposit=False, record_hops=False):
+        """push + direct hop + walk (+ FP64 deposit into the concatenated node charge) of every instance."""
+        check(lib().oppx_fempic_batch_step_fused(
+            self.n_max, ctypes.byref(self.parts), ctypes.byref(self.mesh), self.cell_inst.data_ptr(),
+            self.inst_table.data_ptr(), self.n_inst, self.counters.data_ptr(), self.hole_bits.data_ptr(),
+            self.work.data_ptr(), self.work_bytes, self.hops.data_ptr() if record_hops else None,
+            self.charge.data_ptr() if deposit else None, self.stream()), "fempic_batch_step_fused")
+
+    def step_batch(self, deposit="f32"):
+        """One ensemble step: reset, batched push/move, Sort_All finalize, deposit.
+        deposit: "f32" (compensated FP32 over the sorted particles, into self.charge_f32) or "f64"
+        (FP64 atomics inside the walk, into self.charge)."""
+        self.step_begin()
+        self.pass_batch(deposit=(deposit == "f64"))
+        self.sort(drop_removed=True)
+        if deposit == "f32":
+            self.deposit_sorted_f32()
+
+    def split(self):
+        """Per-instance state (host): the concatenated particles are sorted by concatenated cell."""
+        out = self.download()
+        f32 = self.charge_f32.cpu().numpy() if hasattr(self, "charge_f32") else None
+        res = []
+        for i in range(self.n_inst):
+            c0, c1 = int(self.cell_base[i]), int(self.cell_base[i] + self.n_cells_i[i])
+            lo, hi = np.searchsorted(out["p2c"], [c0, c1])
+            n0, n1 = int(self.node_base[i]), int(self.node_base[i] + self.n_nodes_i[i])
+            res.append(dict(n=int(hi - lo), pos=out["pos"][lo:hi], vel=out["vel"][lo:hi], lc=out["lc"][lo:hi],
+                            p2c=out["p2c"][lo:hi] - c0, charge=out["charge"][n0:n1],
+                            charge_f32=None if f32 is None else f32[n0:n1]))
+        return res
+
+
 class NesoDevice:
     """NESO-advection particles + periodic quad mesh resident on the GPU (C-ABI driven)."""
 
@@ -486,7 +628,7 @@ class NesoDevice:
             if mdir is not None:
                 self.mdir[:, :n] = t(mdir.T, torch.int32)
         self.parts = NesoParts(cap, self.pos.data_ptr(), self.vel.data_ptr(), self.mdir.data_ptr(), self.p2c.data_ptr())
-        self.counters = torch.zeros(8, dtype=torch.int32, device=self.dev)
+        self.counters = torch.zeros(COUNTER_WORDS, dtype=torch.int32, device=self.dev)
         self.counters[0] = n
         self.hops = torch.zeros(cap, dtype=torch.int32, device=self.dev)
 

@@ -78,15 +78,15 @@ def test_gpu_matches_reference_fixture(name, mode):
     assert charge_close(out["charge"], g["out_final_charge"])
 
 
-@pytest.mark.parametrize("tile,refill,split", [(64, 64, 0), (128, 8, 0), (512, 32, 0), (256, 16, 1), (512, 1, 1)])
-def test_gpu_walk_launch_shapes_bit_exact(tile, refill, split):
+@pytest.mark.parametrize("tile,refill,split,persistent", [(64, 64, 0, 0), (128, 8, 0, 0), (512, 32, 0, 0),
+                                                          (256, 16, 1, 0), (512, 1, 1, 0), (256, 16, 0, 1),
+                                                          (64, 4, 1, 1)])
+def test_gpu_walk_launch_shapes_bit_exact(tile, refill, split, persistent):
     # every launch shape of the cooperative walk gives the same cells, hops, permutation and lc
     g = load("fempic_small_dh_omp1.npz")
     m, ef, pos, vel, p2c, kw = case(g)
-    t = oppx.Tuning(tile, refill)
-    t.reserved[0] = split
     saved = oppx.get_tuning()
-    oppx.check(oppx.lib().oppx_set_tuning(ctypes.byref(t)), "set_tuning")
+    oppx.set_tuning(tile, refill, split=split, persistent=persistent)
     try:
         gpu = oppx.FempicDevice(m, ef, pos, vel, p2c, **kw)
         q, ms, dt = float(g["meta_charge"]), float(g["meta_mass"]), float(g["meta_dt"])
@@ -102,7 +102,7 @@ def test_gpu_walk_launch_shapes_bit_exact(tile, refill, split):
         assert np.array_equal(out["p2c"], g["out_final_p2c"]) and np.array_equal(out["lc"].ravel(), g["out_final_lc"])
         assert charge_close(out["charge"], g["out_final_charge"])
     finally:
-        oppx.set_tuning(*saved)
+        oppx.set_tuning(**saved)
 
 
 def test_gpu_c1_scale_matches_reference():
@@ -187,7 +187,7 @@ def test_gpu_holefill_closed_form(n, frac, seed):
     bits[:n] = removed
     packed = np.packbits(bits, bitorder="little").view("<i8")  # bit j of word w = particle 64w + j
     hole_bits = torch.from_numpy(packed.ravel()).to("cuda")
-    ctr = torch.zeros(8, dtype=torch.int32, device="cuda")
+    ctr = torch.zeros(oppx.COUNTER_WORDS, dtype=torch.int32, device="cuda")
     ctr[0] = n
     sb = int(L.oppx_holefill_scratch_bytes(n))
     scratch = torch.zeros(sb, dtype=torch.uint8, device="cuda")

@@ -14,10 +14,20 @@ from conftest import golden_path
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n,n_cells,frac_removed,seed", [(1, 5, 0.0, 0), (1000, 48, 0.1, 1), (70001, 300, 0.3, 2),
-                                                         (300000, 6_000_000, 0.05, 3), (5000, 7, 0.0, 4),
-                                                         (4096, 100000, 1.0, 5)])
-def test_gpu_sort_matches_oracle(n, n_cells, frac_removed, seed):
+@pytest.mark.parametrize("n,n_cells,frac_removed,seed,items", [(1, 5, 0.0, 0, 16), (1000, 48, 0.1, 1, 4),
+                                                               (70001, 300, 0.3, 2, 8), (300000, 6_000_000, 0.05, 3, 16),
+                                                               (300000, 6_000_000, 0.05, 3, 4), (5000, 7, 0.0, 4, 8),
+                                                               (4096, 100000, 1.0, 5, 16)])
+def test_gpu_sort_matches_oracle(n, n_cells, frac_removed, seed, items):
+    saved = oppx.get_tuning()
+    oppx.set_tuning(**dict(saved, sort_items=items))
+    try:
+        _sort_case(n, n_cells, frac_removed, seed)
+    finally:
+        oppx.set_tuning(**saved)
+
+
+def _sort_case(n, n_cells, frac_removed, seed):
     rng = np.random.default_rng(seed)
     p2c = rng.integers(0, n_cells, n).astype(np.int32)
     p2c[rng.random(n) < frac_removed] = oppx.MAX_CELL_INDEX
@@ -33,7 +43,7 @@ def test_gpu_sort_matches_oracle(n, n_cells, frac_removed, seed):
     k_in[:n] = torch.from_numpy(p2c)
     v_out, k_out = torch.zeros_like(v_in), torch.zeros_like(k_in)
     perm = torch.zeros(cap, dtype=torch.int32, device=dev)
-    ctr = torch.zeros(8, dtype=torch.int32, device=dev)
+    ctr = torch.zeros(oppx.COUNTER_WORDS, dtype=torch.int32, device=dev)
     ctr[0] = n
     sb = int(L.oppx_sort_scratch_bytes(n))
     scratch = torch.zeros(sb, dtype=torch.uint8, device=dev)
