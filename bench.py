@@ -136,6 +136,224 @@ def f32_deposit_report(torch, gpu):
     }
 
 
+def bench_ensemble(args):
+    """Batched ensemble: N independent tet-box instances (tets log-uniform 2k-60k, particles log-uniform
+    10k-2M, sigma uniform [0.005, 0.02], seed 4) stepped 20 times as one concatenated set: batched walk,
+    Sort_All finalize and FP32 compensated deposit, launch shape from the autotuner, the 20 steps
+    captured in one hipGraph and launched once. Compared with the plain path: every instance stepped
+    by its own launches, one after the other."""
+    import torch
+
+    import autotune
+    import oppx
+    import workloads
+
+    t0 = time.time()
+    inst = workloads.ensemble(args.instances, seed=4, particle_scale=args.particle_scale)
+    n_part = sum(len(d["p2c"]) for d in inst)
+    n_tets = sum(d["mesh"]["n_cells"] for d in inst)
+    log(f"ensemble built in {time.time() - t0:.1f}s: {len(inst)} instances, {n_tets} tets, {n_part} particles")
+    S = args.ensemble_steps
+
+    b = oppx.FempicBatch(inst)
+    snap = autotune.StateSnapshot(b)
+    best, table = autotune.autotune(lambda: b.step_batch("f32"), snap, autotune.SHORT_CANDIDATES, steps=2,
+                                    use_cache=False, log=log)
+    log(f"autotuned launch shape {best}")
+
+    # eager steps: per-step latency and the live counts of the trajectory (deterministic)
+    snap.restore()
+    torch.cuda.synchronize()
+    lat, live = [], []
+    for _ in range(S):
+        live.append(b.read_counters()["size"])
+        a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        b.step_batch("f32")
+        e.record()
+        torch.cuda.synchronize()
+        lat.append(a.elapsed_time(e))
+    final_eager = b.split()
+
+    # the S steps as one graph launch (S even: the sort's buffer swap returns to the start buffers)
+    snap.restore()
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph, stream=side):
+            for _ in range(S):
+                b.step_batch("f32")
+    snap.restore()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    graph.replay()
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t1
+    final_graph = b.split()
+    same = all(np.array_equal(x["p2c"], y["p2c"]) and np.array_equal(x["pos"], y["pos"])
+               and np.array_equal(x["charge_f32"], y["charge_f32"]) for x, y in zip(final_eager, final_graph))
+    # FP32 compensated deposit against the FP64 atomic deposit of the final state
+    b.charge.zero_()
+    b.deposit()
+    rho64 = b.charge.cpu().numpy()
+    rho32 = b.charge_f32.cpu().numpy().astype(np.float64)
+    f32_err = float(np.abs(rho32 - rho64).max() / max(np.abs(rho64).max(), 1e-300))
+    value = float(sum(live)) / t_graph
+    log(f"batched: {S} steps in {t_graph * 1e3:.1f} ms (graph) -> {value:.3e} particles/s; eager == graph: {same}")
+    del graph, b, snap
+    torch.cuda.empty_cache()
+
+    # plain path: one instance at a time, its own launches
+    solo = []
+    for d in inst:
+        solo.append(oppx.FempicDevice(d["mesh"], d["ef"], d["pos"], d["vel"], d["p2c"], overlay=d["overlay"],
+                                      bmin=d["bmin"], bmax=d["bmax"], gs=d["gs"]))
+    oppx.set_tuning()
+
+    def plain_step(g, d):
+        g.step_begin()
+        g.pass_fused(d["charge"], d["mass"], d["dt"], deposit=False)
+        g.sort(drop_removed=True)
+        g.deposit_sorted_f32()
+
+    for g, d in zip(solo, inst):  # allocations and tables outside the timed region
+        snap_g = autotune.StateSnapshot(g)
+        plain_step(g, d)
+        snap_g.restore()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(S):
+        for g, d in zip(solo, inst):
+            plain_step(g, d)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t1
+    plain = float(sum(live)) / t_plain
+    log(f"plain per-instance launches: {t_plain * 1e3:.1f} ms -> {plain:.3e} particles/s")
+    return {
+        "metric": "ensemble particles/s (batched graph launch vs plain per-instance launches)",
+        "value": value,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": S,
+        "higher_is_better": True,
+        "dtype": "f64 move, f32 compensated deposit",
+        "data": "synthetic",
+        "config": {"workload": f"{len(inst)} box instances, {n_tets} tets, {n_part} particles (seed 4)",
+                   "instances": len(inst), "tets": n_tets, "particles": n_part, "tuning": best},
+        "plain_value": plain,
+        "speedup_vs_plain": value / plain,
+        "p50_step_ms": float(np.percentile(lat, 50)),
+        "p99_step_ms": float(np.percentile(lat, 99)),
+        "graph_equals_eager": bool(same),
+        "fp32_deposit_err_vs_fp64": f32_err,
+        "autotune_table": [[c, round(ms, 4)] for c, ms in table],
+    }
+
+
+def bench_neso(args):
+    """NESO-advection pure move: 1024^2 periodic quads, 64M particles (seed 3) at uniform positions,
+    velocities uniform in [-v, v]^2 with v = 4 cells per step, FP64 update + multi-hop move."""
+    import torch
+
+    import oppx
+
+    nx, cw, dt = args.neso_nx, 0.5, 0.5
+    t0 = time.time()
+    m = oppx.neso_mesh(nx, nx, cw)
+    pos, vel, p2c = oppx.neso_particles(args.neso_particles, 3, m, 4.0 * cw / dt)
+    log(f"neso inputs in {time.time() - t0:.1f}s")
+    g = oppx.NesoDevice(m, pos, vel, p2c)
+    del pos, vel, p2c
+    for _ in range(args.warmup):
+        g.step(dt)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t1 = time.perf_counter()
+    for a, b in evs:
+        a.record()
+        g.step(dt)
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    ms = np.array([a.elapsed_time(b) for a, b in evs])
+    n = args.neso_particles
+    alg = 64.0 * n + 32.0 * nx * nx  # R pos 16, vel 16, mdir 8, p2c 4; W pos 16, mdir 8, p2c 4 (B/particle)
+    achieved = alg / (ms.mean() * 1e-3) / 1e9
+    c = g.read_counters()
+    return {
+        "metric": "NESO-advection particle moves/s (update + multi-hop move)",
+        "value": n * args.steps / wall,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"NESO advection, {nx}x{nx} periodic quads, {n} particles seed 3, 4 cells/step max"},
+        "p50_step_ms": float(np.percentile(ms, 50)),
+        "p99_step_ms": float(np.percentile(ms, 99)),
+        "max_hops_last_step": c["max_hops"],
+        "roofline": {"kernel": "neso_step_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+    }
+
+
+def bench_serve(args):
+    """Serving stream: Poisson arrivals of 10-step segments, sizes as the ensemble's plus 5% large
+    requests, FP32 compensated deposit, admission by an HBM budget; continuous batching against the
+    plain per-request launch path on the same arrival sequence."""
+    import serve
+    import workloads
+
+    t0 = time.time()
+    small = [serve.Template.of(workloads.box_instance(nx, n, s, sg))
+             for (nx, n, s, sg) in workloads.ensemble_specs(args.serve_pool, seed=5)]
+    large = [serve.Template.of(workloads.box_instance(nx, n, 5_000_000 + i, 0.01))
+             for i, (nx, n) in enumerate(args.serve_large)]
+    log(f"serve templates in {time.time() - t0:.1f}s: {len(small)} small, {len(large)} large")
+    rng = np.random.default_rng(5)
+    reqs, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / args.serve_rate)
+        if t > args.serve_seconds:
+            break
+        tpl = large[rng.integers(len(large))] if (large and rng.random() < 0.05) else small[rng.integers(len(small))]
+        reqs.append(serve.Request(len(reqs), t, tpl, 10))
+    budget = args.serve_budget_gb * 1e9
+    arena = serve.FempicArena.for_budget(min(budget, args.serve_arena_gb * 1e9))
+    _, st = serve.serve(arena, reqs, budget, log=log)
+    log(f"continuous batching: {st}")
+    del arena
+    import torch
+    torch.cuda.empty_cache()
+    reqs_plain = [serve.Request(r.rid, r.arrival, r.tpl, r.steps) for r in reqs]
+    _, sp = serve.serve_plain(reqs_plain)
+    log(f"plain per-request launches: {sp}")
+    return {
+        "metric": "serving particles/s (continuous batching), p50/p99 segment latency",
+        "value": st["value"],
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64 move, f32 compensated deposit",
+        "data": "synthetic",
+        "config": {"workload": f"Poisson {args.serve_rate}/s for {args.serve_seconds}s, {len(reqs)} requests of 10 "
+                               f"steps, {len(small)} small + {len(large)} large instance templates, budget "
+                               f"{args.serve_budget_gb} GB"},
+        "p50_latency_ms": st["p50_latency_ms"],
+        "p99_latency_ms": st["p99_latency_ms"],
+        "iterations": st["iterations"],
+        "peak_resident_gb": st["peak_resident_bytes"] / 1e9,
+        "plain_value": sp["value"],
+        "plain_p50_latency_ms": sp["p50_latency_ms"],
+        "plain_p99_latency_ms": sp["p99_latency_ms"],
+        "speedup_vs_plain": st["value"] / sp["value"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,12 +371,38 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tag", default="c2", help="workload tag of the profile summaries")
+    ap.add_argument("--workload", choices=["single", "ensemble", "serve", "neso"], default="single",
+                    help="single: the large single instance (the headline line); ensemble: the batched ensemble; "
+                         "serve: the serving stream; neso: the NESO-advection pure move")
+    ap.add_argument("--neso-nx", type=int, default=1024)
+    ap.add_argument("--neso-particles", type=int, default=64_000_000)
+    ap.add_argument("--serve-rate", type=float, default=400.0)
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--serve-pool", type=int, default=48)
+    ap.add_argument("--serve-large", type=lambda v: [tuple(int(x) for x in p.split(":")) for p in v.split(",") if p],
+                    default=[(56, 5_000_000), (80, 20_000_000)], help="large templates as nx:particles,...")
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-arena-gb", type=float, default=64.0)
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--particle-scale", type=float, default=1.0)
+    ap.add_argument("--ensemble-steps", type=int, default=20)
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "neso":
+        print(json.dumps(bench_neso(args)), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps(bench_serve(args)), flush=True)
+        return
+    if args.workload == "ensemble":
+        if args.ensemble_steps % 2:
+            raise SystemExit("--ensemble-steps must be even (the captured steps end on the starting buffers)")
+        print(json.dumps(bench_ensemble(args)), flush=True)
+        return
 
     import torch
 

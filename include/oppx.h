@@ -320,14 +320,16 @@ int oppx_holefill(int64_t n_max, int64_t stride, oppx_move_counters* counters, c
 
 /* FP32 compensated deposit over cell-sorted particles (p2c ascending, e.g. after oppx_sort_by_cell):
  * per-cell Kahan sums of lc in index order into cell_partials [4*n_cells] (scratch, zeroed here), then per
- * node a Kahan sum of its (cell, vertex) partials in node_ref order. Writes node_charge (double, nullable)
- * and/or node_charge_f32 (nullable). The OPP_INC deposit of apps/app_fempic/kernels.h:164-175 in the
- * segmented-reduction form of apps_codegen/app_fempic_cg/hip/deposit_charge_on_nodes_kernel_loop.hpp:98-158;
- * every addition order is fixed by the data (deterministic). */
+ * node a Kahan sum of its (cell, vertex) partials in node_ref order: entries [node_start[v], node_end[v])
+ * of node_ref (4*cell + vertex), or [node_start[v], node_start[v+1]) when node_end is NULL. Writes
+ * node_charge (double, nullable) and/or node_charge_f32 (nullable). The OPP_INC deposit of
+ * apps/app_fempic/kernels.h:164-175 in the segmented-reduction form of
+ * apps_codegen/app_fempic_cg/hip/deposit_charge_on_nodes_kernel_loop.hpp:98-158; every addition order
+ * is fixed by the data (deterministic). */
 int oppx_fempic_deposit_sorted_f32(int64_t n_max, int64_t stride, oppx_move_counters* counters, const int32_t* p2c,
-                                   const double* lc, int64_t n_cells, const int32_t* node_start, const int32_t* node_ref,
-                                   int64_t n_nodes, float* cell_partials, double* node_charge, float* node_charge_f32,
-                                   void* stream);
+                                   const double* lc, int64_t n_cells, const int32_t* node_start, const int32_t* node_end,
+                                   const int32_t* node_ref, int64_t n_nodes, float* cell_partials, double* node_charge,
+                                   float* node_charge_f32, void* stream);
 
 /* ---------------------------------------------------------------------------------------
  * Sort
@@ -345,6 +347,13 @@ size_t oppx_sort_scratch_bytes(int64_t n_max);
 int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counters, int64_t n_cells,
                       const int32_t* p2c, const oppx_dat_desc* dats, const oppx_dat_desc* out_dats, int n_dats,
                       int32_t* perm_out, int drop_removed, void* scratch, size_t scratch_bytes, void* stream);
+
+/* Range lookup in cell-sorted particles: out[j] = the first index in [0, counters->size) whose p2c is
+ * >= keys[j] (device arrays, k keys). With keys = {first cell, end cell} of a mesh region this gives
+ * the region's particle range (no reference counterpart: used to hand back one instance of a
+ * batched set). */
+int oppx_lower_bound(int64_t n_max, const oppx_move_counters* counters, const int32_t* p2c, const int32_t* keys,
+                     int64_t k, int64_t* out, void* stream);
 
 #ifdef __cplusplus
 }

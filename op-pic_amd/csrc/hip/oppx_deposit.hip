@@ -55,13 +55,15 @@ __global__ void __launch_bounds__(kB) cell_runs(const int32_t* __restrict__ p2c,
 }
 
 __global__ void __launch_bounds__(kB) node_gather(const int32_t* __restrict__ node_start,
+                                                  const int32_t* __restrict__ node_end,
                                                   const int32_t* __restrict__ node_ref, int64_t n_nodes,
                                                   const float* __restrict__ part, double* __restrict__ charge,
                                                   float* __restrict__ charge_f32) {
     const int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x;
     if (v >= n_nodes) return;
     Kahan k;
-    for (int32_t e = node_start[v]; e < node_start[v + 1]; e++) k.add(part[node_ref[e]]);
+    const int32_t e1 = node_end ? node_end[v] : node_start[v + 1];
+    for (int32_t e = node_start[v]; e < e1; e++) k.add(part[node_ref[e]]);
     if (charge) charge[v] = (double)k.s;
     if (charge_f32) charge_f32[v] = k.s;
 }
@@ -74,9 +76,9 @@ using namespace oppx;
 extern "C" {
 
 int oppx_fempic_deposit_sorted_f32(int64_t n_max, int64_t stride, oppx_move_counters* counters, const int32_t* p2c,
-                                   const double* lc, int64_t n_cells, const int32_t* node_start, const int32_t* node_ref,
-                                   int64_t n_nodes, float* cell_partials, double* node_charge, float* node_charge_f32,
-                                   void* stream) {
+                                   const double* lc, int64_t n_cells, const int32_t* node_start, const int32_t* node_end,
+                                   const int32_t* node_ref, int64_t n_nodes, float* cell_partials, double* node_charge,
+                                   float* node_charge_f32, void* stream) {
     OPPX_ARG_CHECK(counters && p2c && lc && node_start && node_ref && cell_partials,
                    "oppx_fempic_deposit_sorted_f32: null pointer");
     OPPX_ARG_CHECK(node_charge || node_charge_f32, "oppx_fempic_deposit_sorted_f32: no output");
@@ -92,8 +94,8 @@ int oppx_fempic_deposit_sorted_f32(int64_t n_max, int64_t stride, oppx_move_coun
         cell_runs<<<grid_for(n_max, kB), kB, 0, s>>>(p2c, lc, stride, counters, n_max, n_cells, cell_partials);
         OPPX_LAUNCH_CHECK("cell_runs");
     }
-    node_gather<<<grid_for(n_nodes, kB), kB, 0, s>>>(node_start, node_ref, n_nodes, cell_partials, node_charge,
-                                                     node_charge_f32);
+    node_gather<<<grid_for(n_nodes, kB), kB, 0, s>>>(node_start, node_end, node_ref, n_nodes, cell_partials,
+                                                     node_charge, node_charge_f32);
     OPPX_LAUNCH_CHECK("node_gather");
     return OPPX_OK;
 }

@@ -208,6 +208,21 @@ __global__ void __launch_bounds__(kB) sort_gather(const uint32_t* __restrict__ p
 
 __global__ void zero_u64(unsigned long long* p) { *p = 0ull; }
 
+__global__ void lower_bound_kernel(const int32_t* __restrict__ p2c, const oppx_move_counters* ctr, int64_t n_max,
+                                   const int32_t* __restrict__ keys, int64_t k, int64_t* __restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= k) return;
+    const int64_t n = live_n(ctr, n_max);
+    const int32_t key = keys[j];
+    int64_t lo = 0, hi = n;  // first index with p2c >= key in [lo, hi]
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (p2c[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    out[j] = lo;
+}
+
 __global__ void sort_finalize(oppx_move_counters* ctr, int64_t n_max, const unsigned long long* n_removed, int drop) {
     const int64_t n = live_n(ctr, n_max);
     const int64_t r = (int64_t)*n_removed;
@@ -319,6 +334,15 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
     }
     sort_finalize<<<1, 1, 0, s>>>(counters, n_max, sc.n_removed, drop_removed);
     OPPX_LAUNCH_CHECK("sort_finalize");
+    return OPPX_OK;
+}
+
+int oppx_lower_bound(int64_t n_max, const oppx_move_counters* counters, const int32_t* p2c, const int32_t* keys,
+                     int64_t k, int64_t* out, void* stream) {
+    OPPX_ARG_CHECK(counters && p2c && keys && out && n_max >= 0 && k >= 0, "oppx_lower_bound: bad arguments");
+    if (k == 0) return OPPX_OK;
+    lower_bound_kernel<<<grid_for(k, kB), kB, 0, as_stream(stream)>>>(p2c, counters, n_max, keys, k, out);
+    OPPX_LAUNCH_CHECK("lower_bound");
     return OPPX_OK;
 }
 

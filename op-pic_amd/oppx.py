@@ -120,7 +120,8 @@ _SIGS = {
     "oppx_hole_bits_words": (c_int64, [c_int64]),
     "oppx_node_cell_csr": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
     "oppx_fempic_deposit_sorted_f32": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
-                                               c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+                                               c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "oppx_lower_bound": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "oppx_sort_scratch_bytes": (c_size_t, [c_int64]),
     "oppx_sort_by_cell": (c_int, [c_int64, c_int64, c_void_p, c_int64, c_void_p, POINTER(DatDesc), POINTER(DatDesc),
                                   c_int, c_void_p, c_int, c_void_p, c_size_t, c_void_p]),
@@ -408,7 +409,7 @@ class FempicDevice:
             self.cell_partials = torch.zeros(4 * self.nc, dtype=torch.float32, device=self.dev)
             self.charge_f32 = torch.zeros(self.nn, dtype=torch.float32, device=self.dev)
         check(lib().oppx_fempic_deposit_sorted_f32(self.n_max, self.cap, self.counters.data_ptr(), self.p2c.data_ptr(),
-                                                   self.lc.data_ptr(), self.nc, self.node_start.data_ptr(),
+                                                   self.lc.data_ptr(), self.nc, self.node_start.data_ptr(), None,
                                                    self.node_ref.data_ptr(), self.nn, self.cell_partials.data_ptr(),
                                                    None, self.charge_f32.data_ptr(), self.stream()),
               "deposit_sorted_f32")

@@ -1,0 +1,371 @@
+"""Serving loop for independent FEM-PIC simulation segments on one GPU: continuous batching.
+
+Requests (an instance plus a number of steps) are admitted into a resident *arena* as soon as they
+have arrived and their state fits both the arena's free ranges and the HBM budget; every server
+iteration then runs ONE batched step over all resident requests (batched push/move, Sort_All
+finalize, FP32 compensated deposit: a fixed number of launches whatever the mix), and requests that
+have done their steps are handed back and their ranges freed at once, so later arrivals join the
+running batch between iterations instead of waiting for it to drain.
+
+Arena layout (device): cells, nodes and overlay words are ranges allocated first-fit per request;
+every cell, node and p2c index is stored concatenated (offset by the request's base); particles of
+all requests share one particle set that the per-step sort keeps ordered by concatenated cell, so a
+request's particles are one contiguous range found by a lower-bound search. A retired request's
+particles are marked removed and leave the set at the next sort. Per-request results are identical
+to the request run alone (tests/test_gpu_serve.py).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import oppx
+from oppx import MAX_CELL_INDEX, BatchInst, DatDesc, FempicParts, TetMesh, check, lib
+
+
+class RangeAlloc:
+    """First-fit allocator of index ranges in [0, cap)."""
+
+    def __init__(self, cap: int):
+        self.cap = int(cap)
+        self.free = [(0, self.cap)]  # sorted, disjoint (start, length)
+        self.used = {}               # start -> length
+
+    def alloc(self, n: int):
+        n = int(n)
+        if n <= 0:
+            return None
+        for k, (s, ln) in enumerate(self.free):
+            if ln >= n:
+                if ln == n:
+                    self.free.pop(k)
+                else:
+                    self.free[k] = (s + n, ln - n)
+                self.used[s] = n
+                return s
+        return None
+
+    def release(self, start: int):
+        n = self.used.pop(start)
+        self.free.append((start, n))
+        self.free.sort()
+        merged = []
+        for s, ln in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == s:
+                merged[-1] = (merged[-1][0], merged[-1][1] + ln)
+            else:
+                merged.append((s, ln))
+        self.free = merged
+
+    def extent(self) -> int:
+        """End of the highest allocated range (kernels cover [0, extent))."""
+        return max((s + n for s, n in self.used.items()), default=0)
+
+    def in_use(self) -> int:
+        return sum(self.used.values())
+
+
+def _aux_bytes(vol: np.ndarray, c2c_global: np.ndarray) -> np.ndarray:
+    """oppx_tet_aux records {c2 = (1/6)/volume, c2c[4], pad[2]} (32 B each) as float64 words."""
+    n = len(vol)
+    rec = np.zeros(n, dtype=[("c2", "<f8"), ("c2c", "<i4", (4,)), ("pad", "<i4", (2,))])
+    rec["c2"] = (1.0 / 6.0) / vol  # the same IEEE division oppx_tet_aux_build performs
+    rec["c2c"] = c2c_global
+    return rec.view(np.float64).reshape(-1)
+
+
+@dataclass
+class Template:
+    """Host-side arrays of one instance, prepared once and uploaded per admission."""
+    inst: dict
+    n_cells: int
+    n_nodes: int
+    n_part: int
+    csr_start: np.ndarray
+    csr_ref: np.ndarray
+    overlay_words: int
+
+    @staticmethod
+    def of(inst: dict) -> "Template":
+        m = inst["mesh"]
+        nc, nn = int(m["n_cells"]), int(m["n_nodes"])
+        if not ((m["c2c"] >= -1) & (m["c2c"] < nc)).all() or not ((m["c2n"] >= 0) & (m["c2n"] < nn)).all():
+            raise ValueError("mesh maps out of range")
+        if len(inst["p2c"]) and not ((inst["p2c"] >= 0) & (inst["p2c"] < nc)).all():
+            raise ValueError("p2c out of range")
+        ov = inst.get("overlay")
+        if ov is not None and not (((ov >= 0) & (ov < nc)) | (ov == MAX_CELL_INDEX)).all():
+            raise ValueError("overlay entries out of range")
+        st, ref = oppx.node_cell_csr(m)
+        return Template(inst, nc, nn, len(inst["p2c"]), st, ref, 0 if ov is None else int(ov.size))
+
+    def device_bytes(self) -> int:
+        """HBM a resident request holds: mesh records, node rows, overlay, particles (+ sort buffers)."""
+        per_cell = 128 + 32 + 16 + 24 + 4 + 16 + 16  # det, aux, c2n, ef, cell_inst, node_ref, partials
+        per_node = 8 + 4 + 8 + 4                     # rows, charges
+        per_part = 2 * 84 + 8 + 8 + 4                # state + swap, sort keys, worklist, hops
+        return per_cell * self.n_cells + per_node * self.n_nodes + 4 * self.overlay_words + per_part * self.n_part
+
+
+class FempicArena:
+    """Resident state of the serving loop (see the module docstring)."""
+
+    def __init__(self, max_cells: int, max_nodes: int, max_particles: int, max_overlay: int, max_insts: int = 4096,
+                 device="cuda"):
+        import torch
+        self.torch = torch
+        self.dev = torch.device(device)
+        L = lib()
+        check(L.oppx_init(self.dev.index or 0), "oppx_init")
+        z = lambda n, dt: torch.zeros(int(n), dtype=dt, device=self.dev)
+        C, N, P = int(max_cells), int(max_nodes), int(max_particles)
+        self.det, self.aux, self.c2n, self.ef = z(16 * C, torch.float64), z(4 * C, torch.float64), \
+            z(4 * C, torch.int32), z(3 * C, torch.float64)
+        self.cell_inst = torch.full((C,), -1, dtype=torch.int32, device=self.dev)
+        self.overlay = z(max(1, max_overlay), torch.int32)
+        self.node_start, self.node_end, self.node_ref = z(N, torch.int32), z(N, torch.int32), z(4 * C, torch.int32)
+        self.charge_f32, self.cell_partials = z(N, torch.float32), z(4 * C, torch.float32)
+        self.inst_table = z(max_insts * ctypes.sizeof(BatchInst), torch.uint8)
+        self.pos, self.vel, self.lc = (torch.zeros((d, P), dtype=torch.float64, device=self.dev) for d in (3, 3, 4))
+        self.p2c = torch.full((P,), MAX_CELL_INDEX, dtype=torch.int32, device=self.dev)
+        self.swap = [torch.empty_like(t) for t in (self.pos, self.vel, self.lc, self.p2c)]
+        self.counters = z(oppx.COUNTER_WORDS, torch.int32)
+        self.hole_bits = z(int(L.oppx_hole_bits_words(P)), torch.int64)
+        self.work_bytes = int(L.oppx_fempic_work_bytes(P))
+        self.work = z(max(self.work_bytes, 8), torch.uint8)
+        self.sort_bytes = int(L.oppx_sort_scratch_bytes(P))
+        self.sort_scratch = z(self.sort_bytes, torch.uint8)
+        self.cells, self.nodes, self.ovl = RangeAlloc(C), RangeAlloc(N), RangeAlloc(max(1, max_overlay))
+        self.slots_free = list(range(max_insts - 1, -1, -1))
+        self.res = {}  # slot -> (template, cb, nb, ob)
+        self.cap, self.size = P, 0
+
+    def stream(self):
+        return self.torch.cuda.current_stream(self.dev).cuda_stream
+
+    def _t(self, a, dt):
+        return self.torch.from_numpy(np.ascontiguousarray(a)).to(self.dev, dtype=dt)
+
+    def can_admit(self, tpl: Template) -> bool:
+        return (bool(self.slots_free) and self.size + tpl.n_part <= self.cap and
+                any(ln >= tpl.n_cells for _, ln in self.cells.free) and
+                any(ln >= tpl.n_nodes for _, ln in self.nodes.free) and
+                (tpl.overlay_words == 0 or any(ln >= tpl.overlay_words for _, ln in self.ovl.free)))
+
+    def admit(self, tpl: Template):
+        """Upload a request's mesh and particles into free ranges; returns its slot or None."""
+        if not self.can_admit(tpl):
+            return None
+        torch = self.torch
+        d, m = tpl.inst, tpl.inst["mesh"]
+        nc, nn = tpl.n_cells, tpl.n_nodes
+        cb, nb = self.cells.alloc(nc), self.nodes.alloc(nn)
+        ob = self.ovl.alloc(tpl.overlay_words) if tpl.overlay_words else -1
+        slot = self.slots_free.pop()
+        c2c = np.where(m["c2c"] >= 0, m["c2c"] + cb, m["c2c"]).astype(np.int32)
+        self.det[16 * cb:16 * (cb + nc)] = self._t(m["c_det"].reshape(-1), torch.float64)
+        self.aux[4 * cb:4 * (cb + nc)] = self._t(_aux_bytes(m["c_vol"], c2c), torch.float64)
+        self.c2n[4 * cb:4 * (cb + nc)] = self._t((m["c2n"] + nb).reshape(-1), torch.int32)
+        self.ef[3 * cb:3 * (cb + nc)] = self._t(d["ef"].reshape(-1), torch.float64)
+        self.cell_inst[cb:cb + nc] = slot
+        self.node_start[nb:nb + nn] = self._t(tpl.csr_start[:-1] + 4 * cb, torch.int32)
+        self.node_end[nb:nb + nn] = self._t(tpl.csr_start[1:] + 4 * cb, torch.int32)
+        self.node_ref[4 * cb:4 * (cb + nc)] = self._t(tpl.csr_ref + 4 * cb, torch.int32)
+        rec = BatchInst()
+        rec.cell_base, rec.n_cells, rec.node_base, rec.n_nodes = cb, nc, nb, nn
+        if ob >= 0:
+            ov = d["overlay"]
+            self.overlay[ob:ob + ov.size] = self._t(np.where(ov == MAX_CELL_INDEX, ov, ov + cb), torch.int32)
+            dims = oppx.dh_dims(d["bmin"], d["bmax"], d["gs"])
+            rec.dh_cell = self.overlay.data_ptr() + 4 * ob
+            for a in range(3):
+                rec.dims[a], rec.min[a] = int(dims[a]), float(d["bmin"][a])
+            rec.inv_gs = 1.0 / float(d["gs"])
+        rec.dt = float(d["dt"])
+        rec.k = float(d["charge"]) / float(d["mass"]) * float(d["dt"])
+        sz = ctypes.sizeof(BatchInst)
+        self.inst_table[slot * sz:(slot + 1) * sz] = self._t(np.frombuffer(bytes(rec), np.uint8), torch.uint8)
+        n0, n = self.size, tpl.n_part
+        if n:
+            self.pos[:, n0:n0 + n] = self._t(d["pos"].T, torch.float64)
+            self.vel[:, n0:n0 + n] = self._t(d["vel"].T, torch.float64)
+            self.lc[:, n0:n0 + n] = 0.0
+            self.p2c[n0:n0 + n] = self._t(d["p2c"] + cb, torch.int32)
+        self.size = n0 + n
+        self.counters[0] = self.size
+        self.res[slot] = (tpl, cb, nb, ob)
+        return slot
+
+    def step(self):
+        """One batched step of every resident request: push/move, Sort_All finalize, FP32 deposit."""
+        L, s = lib(), self.stream()
+        n_cells, n_nodes = max(1, self.cells.extent()), max(1, self.nodes.extent())
+        mesh = TetMesh(n_cells, n_nodes, self.det.data_ptr(), self.aux.data_ptr(), self.c2n.data_ptr(),
+                       self.ef.data_ptr())
+        parts = FempicParts(self.cap, self.pos.data_ptr(), self.vel.data_ptr(), self.lc.data_ptr(),
+                            self.p2c.data_ptr())
+        n_insts = max(self.res) + 1 if self.res else 1
+        check(L.oppx_fempic_step_begin(self.counters.data_ptr(), None, 0, s), "step_begin")
+        if self.size:
+            check(L.oppx_fempic_batch_step_fused(self.size, ctypes.byref(parts), ctypes.byref(mesh),
+                                                 self.cell_inst.data_ptr(), self.inst_table.data_ptr(), n_insts,
+                                                 self.counters.data_ptr(), self.hole_bits.data_ptr(),
+                                                 self.work.data_ptr(), self.work_bytes, None, None, s),
+                  "batch_step")
+        cur = [self.pos, self.vel, self.lc, self.p2c]
+        dims = [(3, 8), (3, 8), (4, 8), (1, 4)]
+        ins = (DatDesc * 4)(*[DatDesc(t.data_ptr(), dd, e) for t, (dd, e) in zip(cur, dims)])
+        outs = (DatDesc * 4)(*[DatDesc(t.data_ptr(), dd, e) for t, (dd, e) in zip(self.swap, dims)])
+        check(L.oppx_sort_by_cell(self.size, self.cap, self.counters.data_ptr(), n_cells, self.p2c.data_ptr(), ins,
+                                  outs, 4, None, 1, self.sort_scratch.data_ptr(), self.sort_bytes, s), "sort")
+        self.pos, self.vel, self.lc, self.p2c = self.swap
+        self.swap = cur
+        check(L.oppx_fempic_deposit_sorted_f32(self.size, self.cap, self.counters.data_ptr(), self.p2c.data_ptr(),
+                                               self.lc.data_ptr(), n_cells, self.node_start.data_ptr(),
+                                               self.node_end.data_ptr(), self.node_ref.data_ptr(), n_nodes,
+                                               self.cell_partials.data_ptr(), None, self.charge_f32.data_ptr(), s),
+              "deposit_f32")
+        self.size = int(self.counters[0].item())  # the live count after the finalize (host sync)
+
+    def retire(self, slots) -> list:
+        """Hand back requests' states (each contiguous after the sort) and free their ranges. All range
+        lookups happen before any particle is marked removed, so the set is still sorted for them."""
+        L, s = lib(), self.stream()
+        torch = self.torch
+        slots = list(slots)
+        if not slots:
+            return []
+        keys = []
+        for slot in slots:
+            tpl, cb, nb, ob = self.res[slot]
+            keys += [cb, cb + tpl.n_cells]
+        k = torch.tensor(keys, dtype=torch.int32, device=self.dev)
+        bounds = torch.zeros(len(keys), dtype=torch.int64, device=self.dev)
+        check(L.oppx_lower_bound(self.size, self.counters.data_ptr(), self.p2c.data_ptr(), k.data_ptr(), len(keys),
+                                 bounds.data_ptr(), s), "lower_bound")
+        b = bounds.cpu().numpy()
+        outs = []
+        pat = np.array([MAX_CELL_INDEX], np.int32)
+        for j, slot in enumerate(slots):
+            tpl, cb, nb, ob = self.res.pop(slot)
+            lo, hi = int(b[2 * j]), int(b[2 * j + 1])
+            outs.append(dict(n=hi - lo, pos=self.pos[:, lo:hi].T.cpu().numpy(), vel=self.vel[:, lo:hi].T.cpu().numpy(),
+                             lc=self.lc[:, lo:hi].T.cpu().numpy(), p2c=self.p2c[lo:hi].cpu().numpy() - cb,
+                             charge_f32=self.charge_f32[nb:nb + tpl.n_nodes].cpu().numpy()))
+            if hi > lo:  # leaves the particle set at the next sort
+                check(L.oppx_fill_pattern(self.p2c.data_ptr() + 4 * lo, hi - lo, 1, 4, pat.ctypes.data, 0, s),
+                      "fill_pattern")
+            self.node_start[nb:nb + tpl.n_nodes] = 0  # empty node rows
+            self.node_end[nb:nb + tpl.n_nodes] = 0
+            self.cells.release(cb)
+            self.nodes.release(nb)
+            if ob >= 0:
+                self.ovl.release(ob)
+            self.slots_free.append(slot)
+        return outs
+
+    @staticmethod
+    @staticmethod
+    def for_budget(arena_bytes: float, device="cuda", max_insts: int = 4096) -> "FempicArena":
+        """An arena of about `arena_bytes`: a quarter for mesh records (about 300 B per tet with its
+        nodes, rows and overlay words), the rest for particles (about 210 B each with the sort and
+        walk workspace)."""
+        cells = int(0.25 * arena_bytes / 300)
+        return FempicArena(max_cells=cells, max_nodes=cells // 4, max_particles=int(0.75 * arena_bytes / 210),
+                           max_overlay=2 * cells, max_insts=max_insts, device=device)
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float
+    tpl: Template
+    steps: int
+    slot: int = -1
+    left: int = 0
+    t_admit: float = 0.0
+    t_done: float = 0.0
+    result: dict = field(default_factory=dict)
+
+
+def serve(arena: FempicArena, requests, budget_bytes: float, keep_results: bool = False, log=None):
+    """Continuous batching over `requests` (sorted by arrival, seconds from start). Returns
+    (requests, stats). Admission is FIFO: a request is admitted once it has arrived and fits."""
+    clock = time.perf_counter
+    t0 = clock()
+    pending, active, done = deque(requests), {}, []
+    resident = 0.0
+    peak_resident, iters, part_steps = 0.0, 0, 0
+    while pending or active:
+        now = clock() - t0
+        while pending and pending[0].arrival <= now:
+            r = pending[0]
+            need = r.tpl.device_bytes()
+            if resident + need > budget_bytes or not arena.can_admit(r.tpl):
+                break
+            r.slot = arena.admit(r.tpl)
+            r.left, r.t_admit = r.steps, clock() - t0
+            resident += need
+            peak_resident = max(peak_resident, resident)
+            active[r.slot] = r
+            pending.popleft()
+        if not active:
+            if pending:
+                time.sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.01)))
+            continue
+        part_steps += arena.size
+        arena.step()
+        iters += 1
+        finished = [k for k, r in active.items() if r.left == 1]
+        for slot, res in zip(finished, arena.retire(finished)):
+            r = active.pop(slot)
+            r.t_done = clock() - t0
+            if keep_results:
+                r.result = res
+            resident -= r.tpl.device_bytes()
+            done.append(r)
+        for r in active.values():
+            r.left -= 1
+    wall = clock() - t0
+    lat = np.array([r.t_done - r.arrival for r in done]) * 1e3
+    return done, dict(wall_s=wall, iterations=iters, particle_steps=part_steps, value=part_steps / wall,
+                      p50_latency_ms=float(np.percentile(lat, 50)), p99_latency_ms=float(np.percentile(lat, 99)),
+                      peak_resident_bytes=peak_resident, requests=len(done))
+
+
+def serve_plain(requests, keep_results: bool = False):
+    """The plain per-request launch path: requests one at a time in arrival order, each on its own
+    device state with its own launches."""
+    import torch
+    clock = time.perf_counter
+    t0 = clock()
+    part_steps = 0
+    for r in requests:
+        while clock() - t0 < r.arrival:
+            time.sleep(max(0.0, min(r.arrival - (clock() - t0), 0.01)))
+        d = r.tpl.inst
+        g = oppx.FempicDevice(d["mesh"], d["ef"], d["pos"], d["vel"], d["p2c"], overlay=d.get("overlay"),
+                              bmin=d.get("bmin"), bmax=d.get("bmax"), gs=d.get("gs"))
+        for _ in range(r.steps):
+            part_steps += g.n_max if _ == 0 else g.sync_size()
+            g.step_begin()
+            g.pass_fused(d["charge"], d["mass"], d["dt"], deposit=False)
+            g.sort(drop_removed=True)
+            g.deposit_sorted_f32()
+        out = g.download()
+        if keep_results:
+            out["charge_f32"] = g.charge_f32.cpu().numpy()
+            r.result = out
+        torch.cuda.synchronize()
+        r.t_done = clock() - t0
+        del g
+    wall = clock() - t0
+    lat = np.array([r.t_done - r.arrival for r in requests]) * 1e3
+    return requests, dict(wall_s=wall, particle_steps=part_steps, value=part_steps / wall,
+                          p50_latency_ms=float(np.percentile(lat, 50)), p99_latency_ms=float(np.percentile(lat, 99)),
+                          requests=len(requests))
