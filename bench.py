@@ -34,11 +34,11 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_inputs(nx, n_part, seed, sigma, order):
+def build_inputs(nx, n_part, seed, sigma, order, mesh_order="x"):
     import oppx
 
     t0 = time.time()
-    m = oppx.box_mesh(nx, nx, nx)
+    m = oppx.box_mesh(nx, nx, nx, order=mesh_order)
     ef = oppx.box_field_sines(m)
     pos, vel, p2c = oppx.box_particles(n_part, seed, sigma, m)
     if order == "cell":
@@ -368,6 +368,8 @@ def main():
                     help="move finalize: hole fill, or the Sort_Periodic policy (sort by cell every --sort-every "
                          "steps, hole fill in between)")
     ap.add_argument("--sort-every", type=int, default=2)
+    ap.add_argument("--mesh-order", choices=["x", "morton"], default="x",
+                    help="numbering of the box mesh's cubes and nodes (x fastest, or Morton order)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tag", default="c2", help="workload tag of the profile summaries")
@@ -411,7 +413,8 @@ def main():
     sigma = 0.01
     q, ms = 1e-3, 1.0
     dt = 0.5 * (1.0 / args.nx) / (sigma * math.sqrt(3.0))
-    m, ef, pos, vel, p2c, ov, bmin, bmax, gs = build_inputs(args.nx, args.particles, args.seed, sigma, args.order)
+    m, ef, pos, vel, p2c, ov, bmin, bmax, gs = build_inputs(args.nx, args.particles, args.seed, sigma, args.order,
+                                                            args.mesh_order)
 
     t0 = time.time()
     gpu = oppx.FempicDevice(m, ef, pos, vel, p2c, overlay=ov, bmin=bmin, bmax=bmax, gs=gs)
@@ -514,7 +517,8 @@ def main():
                         f"{args.particles} particles seed {args.seed}, {int(np.prod(oppx.dh_dims(bmin, bmax, gs)))}-box "
                         f"direct-hop overlay, FP64 push+move+atomic deposit, {fill}",
             "tets": nc, "nodes": nn, "particles": args.particles, "overlay_spacing": gs, "particle_order": args.order,
-            "steps_total": W + K, "deposit": "fp64-atomic", "fill": fill, "parallelism": "single-gpu",
+            "steps_total": W + K, "deposit": "fp64-atomic", "fill": fill, "mesh_order": args.mesh_order,
+            "parallelism": "single-gpu",
         },
         "p50_step_ms": float(np.percentile(step_ms, 50)),
         "p99_step_ms": float(np.percentile(step_ms, 99)),

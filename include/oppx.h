@@ -169,16 +169,18 @@ int oppx_soa_to_aos(const void* soa, void* aos, int64_t n, int dim, int64_t stri
  * Host builders (CPU only; no device needed). Inputs of the hot path.
  * ------------------------------------------------------------------------------------- */
 
-/* Box mesh of nx*ny*nz cubes x 6 tets; geometry per apps/app_fempic/minifempic_funcs.h:284-443. */
+/* Box mesh of nx*ny*nz cubes x 6 tets; geometry per apps/app_fempic/minifempic_funcs.h:284-443.
+ * order 0 numbers cubes and nodes x fastest; order 1 in Morton order (spatial neighbours close in
+ * index, so cell records of neighbouring tets share cache and HBM pages). Cube r owns tets 6r..6r+5. */
 int oppx_box_mesh_sizes(int nx, int ny, int nz, int64_t* n_nodes, int64_t* n_cells);
-int oppx_box_mesh_build(int nx, int ny, int nz, double lx, double ly, double lz,
+int oppx_box_mesh_build(int nx, int ny, int nz, double lx, double ly, double lz, int order,
                         double* node_pos, int* c2n, int* c2c, double* c_det, double* c_vol);
 /* c_ef = -grad of the P1 interpolant of sin(pi x) sin(pi y) sin(pi z). */
 int oppx_box_field_sines(int64_t n_nodes, int64_t n_cells, const double* node_pos, const int* c2n,
                          const double* c_det, const double* c_vol, double* c_ef);
 /* Seeded uniform positions, N(0,sigma^2) velocities, containing tet (AoS host arrays). */
 int oppx_box_particles_uniform(int64_t n_part, uint64_t seed, double sigma, int nx, int ny, int nz,
-                               double lx, double ly, double lz, double* pos, double* vel, int* p2c);
+                               double lx, double ly, double lz, int order, double* pos, double* vel, int* p2c);
 /* node -> (cell, vertex) incidence: node_start [n_nodes+1], node_ref [4*n_cells] = c*4+k ascending per node. */
 int oppx_node_cell_csr(int64_t n_cells, int64_t n_nodes, const int* c2n, int* node_start, int* node_ref);
 /* Overlay dims of CellMapper::init_host (opp_lib/include/opp_direct_hop_core.h:317-336). */

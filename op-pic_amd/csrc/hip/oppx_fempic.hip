@@ -807,35 +807,11 @@ int fill_args(StepArgs& P, const oppx_fempic_parts* parts, const oppx_tet_mesh* 
     return OPPX_OK;
 }
 
-// Blocks of `kernel` resident on the whole device (occupancy query cached per kernel).
-unsigned resident_blocks(const void* kernel) {
-    static int cus = 0, per_cu_cache[64] = {0};
-    static const void* kern_cache[64] = {nullptr};
-    if (cus == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-    }
-    int per_cu = 0;
-    for (int i = 0; i < 64; i++) {
-        if (kern_cache[i] == kernel) { per_cu = per_cu_cache[i]; break; }
-        if (kern_cache[i] == nullptr) {
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu <= 0)
-                per_cu = 1;
-            kern_cache[i] = kernel;
-            per_cu_cache[i] = per_cu;
-            break;
-        }
-    }
-    return (unsigned)cus * (unsigned)(per_cu > 0 ? per_cu : 1);
-}
-
 // Grid of a tile-kernel launch: one block per block tile, or (persistent) the resident grid,
 // a multiple of the XCD count so every XCD's queue has blocks draining it.
 unsigned tile_grid(const void* kernel, unsigned block_tiles, bool persistent) {
     if (!persistent) return block_tiles;
-    unsigned g = resident_blocks(kernel);
+    unsigned g = resident_blocks(kernel, kBlock);
     const unsigned need = ((block_tiles + kXcds - 1) / kXcds) * kXcds;
     if (g > need) g = need;
     g = (g / kXcds) * kXcds;
@@ -891,7 +867,7 @@ bool launch_two_pass(const StepArgs& P, int64_t n_max, void* work, size_t work_b
         fempic_first_kernel<kBlock, PUSH, DH, DEPOSIT, BATCH><<<g, kBlock, 0, s>>>(P, g, g_tuning.inline_hops, W);
     }
     auto walk = fempic_walk_kernel<DEPOSIT>;
-    unsigned wg = resident_blocks(reinterpret_cast<const void*>(walk));
+    unsigned wg = resident_blocks(reinterpret_cast<const void*>(walk), kBlock);
     wg = wg < (unsigned)kSlots ? (unsigned)kSlots : (wg / kSlots) * kSlots;
     walk<<<wg, kBlock, 0, s>>>(P, W);
     return true;

@@ -15,6 +15,10 @@ void set_error(const std::string& msg);
 // Process-wide launch shape (oppx_set_tuning), defined in oppx_runtime.hip.
 extern oppx_tuning g_tuning;
 
+// Blocks of `kernel` (with `block` threads) resident on the whole device; the occupancy query is
+// cached per (kernel, block).
+unsigned resident_blocks(const void* kernel, int block);
+
 // Returns from the enclosing C-ABI function with OPPX_ERR_HIP when a HIP call fails.
 #define OPPX_HIP_TRY(call)                                                                       \
     do {                                                                                        \

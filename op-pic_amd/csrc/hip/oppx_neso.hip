@@ -35,30 +35,30 @@ __device__ __forceinline__ double periodic_update(double p, double v, double dt,
     return fma(-ext, trunc(t / ext), t);
 }
 
-template <bool STATS>
-__global__ void __launch_bounds__(kBlock) neso_step_kernel(NesoArgs P, uint32_t nblocks, int flags) {
-    const uint32_t b = xcd_block(blockIdx.x, nblocks);
-    const int64_t i = (int64_t)b * kBlock + threadIdx.x;
+// Resident grid, grid-stride over the particles: the per-block statistics then reach the counters
+// from a few thousand blocks instead of one block per 256 particles (same-address atomics serialise).
+__global__ void __launch_bounds__(kBlock) neso_step_kernel(NesoArgs P, int flags) {
     int64_t n = P.ctr->size;
     n = n < 0 ? 0 : (n > P.n_max ? P.n_max : n);
-    const bool live = i < n;
     const int64_t S = P.stride;
-    int h = 0;
-    bool fault = false;
-    if (live) {
-        double x = P.pos[i], y = P.pos[S + i];
+    long long hsum = 0;
+    int hmax = 0, nfault = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        int h = 0;
+        bool fault = false;
+        double x = __builtin_nontemporal_load(P.pos + i), y = __builtin_nontemporal_load(P.pos + S + i);
         if (flags & OPPX_NESO_UPDATE) {
-            x = periodic_update(x, P.vel[i], P.dt, P.ext_x);
-            y = periodic_update(y, P.vel[S + i], P.dt, P.ext_y);
-            P.pos[i] = x;
-            P.pos[S + i] = y;
+            x = periodic_update(x, __builtin_nontemporal_load(P.vel + i), P.dt, P.ext_x);
+            y = periodic_update(y, __builtin_nontemporal_load(P.vel + S + i), P.dt, P.ext_y);
+            __builtin_nontemporal_store(x, P.pos + i);
+            __builtin_nontemporal_store(y, P.pos + S + i);
         }
-        int cell = (flags & OPPX_NESO_MOVE) ? P.p2c[i] : kMaxCell;
+        int cell = (flags & OPPX_NESO_MOVE) ? __builtin_nontemporal_load(P.p2c + i) : kMaxCell;
         if (cell != kMaxCell) {
             if (!((unsigned)cell < (unsigned)P.n_cells)) {
                 fault = true;
             } else {
-                int mx = P.mdir[i], my = P.mdir[S + i];
+                int mx = __builtin_nontemporal_load(P.mdir + i), my = __builtin_nontemporal_load(P.mdir + S + i);
                 const double cw = P.cw;
                 for (;;) {
                     const oppx_quad_cell q = P.cells[cell];
@@ -87,31 +87,30 @@ __global__ void __launch_bounds__(kBlock) neso_step_kernel(NesoArgs P, uint32_t 
                     }
                     cell = next;
                 }
-                P.mdir[i] = mx;
-                P.mdir[S + i] = my;
-                if (!fault) P.p2c[i] = cell;
+                __builtin_nontemporal_store(mx, P.mdir + i);
+                __builtin_nontemporal_store(my, P.mdir + S + i);
+                if (!fault) __builtin_nontemporal_store(cell, P.p2c + i);
             }
         }
         if (P.hops && (flags & OPPX_NESO_MOVE)) P.hops[i] = h;
+        hsum += h;
+        hmax = hmax > h ? hmax : h;
+        nfault += fault ? 1 : 0;
     }
-    const int lane = threadIdx.x & (kWave - 1);
-    const unsigned long long fmask = __ballot(fault);
-    if (lane == 0 && fmask) atomicAdd(&P.ctr->faults, (int)__popcll(fmask));
-    if (STATS) {
-        __shared__ long long s_sum[kBlock / kWave];
-        __shared__ int s_max[kBlock / kWave];
-        const long long hs = wave_sum_i64(h);
-        const int hm = wave_max_i32(h);
-        const int w = threadIdx.x / kWave;
-        if (lane == 0) { s_sum[w] = hs; s_max[w] = hm; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            long long t = 0;
-            int m = 0;
-            for (int q = 0; q < kBlock / kWave; q++) { t += s_sum[q]; m = m > s_max[q] ? m : s_max[q]; }
-            if (t) atomicAdd((unsigned long long*)&P.ctr->total_hops, (unsigned long long)t);
-            if (m) atomicMax(&P.ctr->max_hops, m);
-        }
+    __shared__ long long s_sum[kBlock / kWave];
+    __shared__ int s_max[kBlock / kWave], s_flt[kBlock / kWave];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const long long hs = wave_sum_i64(hsum);
+    const int hm = wave_max_i32(hmax), nf = wave_sum_i32(nfault);
+    if (lane == 0) { s_sum[w] = hs; s_max[w] = hm; s_flt[w] = nf; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        int m = 0, f = 0;
+        for (int q = 0; q < kBlock / kWave; q++) { t += s_sum[q]; m = m > s_max[q] ? m : s_max[q]; f += s_flt[q]; }
+        if (t) atomicAdd((unsigned long long*)&P.ctr->total_hops, (unsigned long long)t);
+        if (m) atomicMax(&P.ctr->max_hops, m);
+        if (f) atomicAdd(&P.ctr->faults, f);
     }
 }
 
@@ -180,8 +179,10 @@ int oppx_neso_step(int64_t n_max, const oppx_neso_parts* parts, const oppx_quad_
     P.n_max = n_max;
     P.ctr = counters;
     P.hops = hops;
-    const unsigned g = grid_for(n_max, kBlock);
-    neso_step_kernel<true><<<g, kBlock, 0, s>>>(P, g, flags);
+    unsigned g = resident_blocks(reinterpret_cast<const void*>(neso_step_kernel), kBlock);
+    const unsigned need = grid_for(n_max, kBlock);
+    if (g > need) g = need;
+    neso_step_kernel<<<g, kBlock, 0, s>>>(P, flags);
     OPPX_LAUNCH_CHECK("neso_step");
     return OPPX_OK;
 }

@@ -12,6 +12,31 @@ static thread_local std::string g_last_error;
 
 oppx_tuning g_tuning = {256, 16, 0, 0, 16, 64, 256, 0};
 
+unsigned resident_blocks(const void* kernel, int block) {
+    static int cus = 0;
+    static const void* kern_cache[64] = {nullptr};
+    static int block_cache[64] = {0}, per_cu_cache[64] = {0};
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    int per_cu = 0;
+    for (int i = 0; i < 64; i++) {
+        if (kern_cache[i] == kernel && block_cache[i] == block) { per_cu = per_cu_cache[i]; break; }
+        if (kern_cache[i] == nullptr) {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu <= 0)
+                per_cu = 1;
+            kern_cache[i] = kernel;
+            block_cache[i] = block;
+            per_cu_cache[i] = per_cu;
+            break;
+        }
+    }
+    return (unsigned)cus * (unsigned)(per_cu > 0 ? per_cu : 1);
+}
+
 void set_error(const std::string& msg) { g_last_error = msg; }
 
 namespace {

@@ -87,6 +87,37 @@ inline bool tet_contains(const double* det, double vol, double x, double y, doub
     return true;
 }
 
+// Numbering of a structured n0 x n1 x n2 grid: rank[i + n0*(j + n1*k)]. Order 0 is that linear
+// index itself (x fastest); order 1 is the Morton (bit-interleaved) order of (i, j, k), which keeps
+// spatial neighbours close in index -- and in memory -- in all three directions.
+std::vector<int64_t> grid_rank(int64_t n0, int64_t n1, int64_t n2, int order) {
+    const int64_t n = n0 * n1 * n2;
+    std::vector<int64_t> rank(n);
+    if (order == 0) {
+        for (int64_t q = 0; q < n; q++) rank[q] = q;
+        return rank;
+    }
+    auto spread = [](uint64_t v) {  // 21 bits -> every third bit
+        uint64_t x = v & 0x1FFFFFull;
+        x = (x | x << 32) & 0x1F00000000FFFFull;
+        x = (x | x << 16) & 0x1F0000FF0000FFull;
+        x = (x | x << 8) & 0x100F00F00F00F00Full;
+        x = (x | x << 4) & 0x10C30C30C30C30C3ull;
+        x = (x | x << 2) & 0x1249249249249249ull;
+        return x;
+    };
+    std::vector<std::pair<uint64_t, int64_t>> key(n);
+    for (int64_t k = 0; k < n2; k++)
+        for (int64_t j = 0; j < n1; j++)
+            for (int64_t i = 0; i < n0; i++) {
+                const int64_t q = i + n0 * (j + n1 * k);
+                key[q] = {spread((uint64_t)i) | spread((uint64_t)j) << 1 | spread((uint64_t)k) << 2, q};
+            }
+    std::sort(key.begin(), key.end());
+    for (int64_t r = 0; r < n; r++) rank[key[r].second] = r;
+    return rank;
+}
+
 }  // namespace
 
 extern "C" {
@@ -99,39 +130,43 @@ int oppx_box_mesh_sizes(int nx, int ny, int nz, int64_t* n_nodes, int64_t* n_cel
     return OPPX_OK;
 }
 
-int oppx_box_mesh_build(int nx, int ny, int nz, double lx, double ly, double lz,
+int oppx_box_mesh_build(int nx, int ny, int nz, double lx, double ly, double lz, int order,
                         double* node_pos, int* c2n, int* c2c, double* c_det, double* c_vol) {
     int64_t nn = 0, nc = 0;
     int rc = oppx_box_mesh_sizes(nx, ny, nz, &nn, &nc);
     if (rc != OPPX_OK) return rc;
-    if (!node_pos || !c2n || !c2c || !c_det || !c_vol || !(lx > 0) || !(ly > 0) || !(lz > 0))
+    if (!node_pos || !c2n || !c2c || !c_det || !c_vol || !(lx > 0) || !(ly > 0) || !(lz > 0) ||
+        (order != 0 && order != 1))
         return OPPX_ERR_ARG;
+    const std::vector<int64_t> nrank = grid_rank(nx + 1, ny + 1, nz + 1, order);
+    const std::vector<int64_t> crank = grid_rank(nx, ny, nz, order);
+    auto nid = [&](int i, int j, int k) { return nrank[node_id(i, j, k, nx, ny)]; };
 
     // nodes: x = lx * i / nx (one rounding of the exact quotient, then one product)
 #pragma omp parallel for schedule(static)
     for (int k = 0; k <= nz; k++)
         for (int j = 0; j <= ny; j++)
             for (int i = 0; i <= nx; i++) {
-                const int64_t n = node_id(i, j, k, nx, ny);
+                const int64_t n = nid(i, j, k);
                 node_pos[3 * n + 0] = lx * ((double)i / (double)nx);
                 node_pos[3 * n + 1] = ly * ((double)j / (double)ny);
                 node_pos[3 * n + 2] = lz * ((double)k / (double)nz);
             }
 
-    // cells: cube (ci,cj,ck) owns tets 6*cube .. 6*cube+5
+    // cells: cube (ci,cj,ck) of rank r owns tets 6r .. 6r+5
 #pragma omp parallel for schedule(static)
     for (int ck = 0; ck < nz; ck++)
         for (int cj = 0; cj < ny; cj++)
             for (int ci = 0; ci < nx; ci++) {
-                const int64_t cube = ci + (int64_t)nx * (cj + (int64_t)ny * ck);
+                const int64_t cube = crank[ci + (int64_t)nx * (cj + (int64_t)ny * ck)];
                 for (int t = 0; t < 6; t++) {
                     const int64_t c = 6 * cube + t;
                     int off[3] = {0, 0, 0};
                     int con[4];
-                    con[0] = (int)node_id(ci, cj, ck, nx, ny);
+                    con[0] = (int)nid(ci, cj, ck);
                     for (int s = 0; s < 3; s++) {
                         off[kKuhnPerm[t][s]] = 1;
-                        con[s + 1] = (int)node_id(ci + off[0], cj + off[1], ck + off[2], nx, ny);
+                        con[s + 1] = (int)nid(ci + off[0], cj + off[1], ck + off[2]);
                     }
                     // volume (1/6) det4 and the ABCD -> ADBC flip of minifempic_funcs.h:284-313
                     double M[4][4];
@@ -264,10 +299,12 @@ int oppx_box_field_sines(int64_t n_nodes, int64_t n_cells, const double* node_po
 }
 
 int oppx_box_particles_uniform(int64_t n_part, uint64_t seed, double sigma, int nx, int ny, int nz,
-                               double lx, double ly, double lz, double* pos, double* vel, int* p2c) {
+                               double lx, double ly, double lz, int order, double* pos, double* vel, int* p2c) {
     // Uniform positions in the box, N(0, sigma^2) velocity components, and the containing
     // tet from the closed form of the cube split (ordering of the fractional coordinates).
-    if (n_part < 0 || !pos || !vel || !p2c || nx <= 0 || ny <= 0 || nz <= 0) return OPPX_ERR_ARG;
+    if (n_part < 0 || !pos || !vel || !p2c || nx <= 0 || ny <= 0 || nz <= 0 || (order != 0 && order != 1))
+        return OPPX_ERR_ARG;
+    const std::vector<int64_t> crank = grid_rank(nx, ny, nz, order);
     const int n[3] = {nx, ny, nz};
     const double L[3] = {lx, ly, lz};
 #pragma omp parallel for schedule(static)
@@ -289,7 +326,7 @@ int oppx_box_particles_uniform(int64_t n_part, uint64_t seed, double sigma, int 
             const int a = kKuhnPerm[q][0], b = kKuhnPerm[q][1], cc = kKuhnPerm[q][2];
             if (f[a] >= f[b] && f[b] >= f[cc]) { t = q; break; }
         }
-        const int64_t cube = ci[0] + (int64_t)nx * (ci[1] + (int64_t)ny * ci[2]);
+        const int64_t cube = crank[ci[0] + (int64_t)nx * (ci[1] + (int64_t)ny * ci[2])];
         p2c[p] = (int)(6 * cube + t);
     }
     return OPPX_OK;

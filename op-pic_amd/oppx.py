@@ -86,11 +86,11 @@ _SIGS = {
     "oppx_aos_to_soa": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int64, c_int, c_void_p]),
     "oppx_soa_to_aos": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int64, c_int, c_void_p]),
     "oppx_box_mesh_sizes": (c_int, [c_int, c_int, c_int, POINTER(c_int64), POINTER(c_int64)]),
-    "oppx_box_mesh_build": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double,
+    "oppx_box_mesh_build": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_int,
                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "oppx_box_field_sines": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "oppx_box_particles_uniform": (c_int, [c_int64, c_uint64, c_double, c_int, c_int, c_int,
-                                           c_double, c_double, c_double, c_void_p, c_void_p, c_void_p]),
+                                           c_double, c_double, c_double, c_int, c_void_p, c_void_p, c_void_p]),
     "oppx_dh_grid_dims": (c_int, [c_void_p, c_void_p, c_double, c_void_p]),
     "oppx_dh_build_overlay_nodes": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                             c_void_p, c_void_p, c_double, c_void_p]),
@@ -178,16 +178,21 @@ def _ptr(a: np.ndarray):
 # host builders
 # ---------------------------------------------------------------------------------------------
 
-def box_mesh(nx: int, ny: int, nz: int, lx=1.0, ly=1.0, lz=1.0) -> dict:
-    """Box of nx*ny*nz cubes x 6 tets with the FEM geometry the mover consumes."""
+ORDERS = {"x": 0, "morton": 1}
+
+
+def box_mesh(nx: int, ny: int, nz: int, lx=1.0, ly=1.0, lz=1.0, order="x") -> dict:
+    """Box of nx*ny*nz cubes x 6 tets with the FEM geometry the mover consumes; cubes and nodes
+    numbered x fastest ("x") or in Morton order ("morton")."""
     L = lib()
     nn, nc = c_int64(), c_int64()
     check(L.oppx_box_mesh_sizes(nx, ny, nz, ctypes.byref(nn), ctypes.byref(nc)), "box_mesh_sizes")
     nn, nc = nn.value, nc.value
-    m = dict(nx=nx, ny=ny, nz=nz, lx=lx, ly=ly, lz=lz,
+    m = dict(nx=nx, ny=ny, nz=nz, lx=lx, ly=ly, lz=lz, order=order,
              node_pos=np.zeros((nn, 3)), c2n=np.zeros((nc, 4), np.int32), c2c=np.zeros((nc, 4), np.int32),
              c_det=np.zeros((nc, 16)), c_vol=np.zeros(nc))
-    check(L.oppx_box_mesh_build(nx, ny, nz, lx, ly, lz, _ptr(m["node_pos"]), _ptr(m["c2n"]), _ptr(m["c2c"]),
+    check(L.oppx_box_mesh_build(nx, ny, nz, lx, ly, lz, ORDERS[order], _ptr(m["node_pos"]), _ptr(m["c2n"]),
+                                _ptr(m["c2c"]),
                                 _ptr(m["c_det"]), _ptr(m["c_vol"])), "box_mesh_build")
     m["n_nodes"], m["n_cells"] = nn, nc
     return m
@@ -204,7 +209,8 @@ def box_particles(n: int, seed: int, sigma: float, m: dict):
     pos, vel = np.zeros((n, 3)), np.zeros((n, 3))
     p2c = np.zeros(n, np.int32)
     check(lib().oppx_box_particles_uniform(n, seed, sigma, m["nx"], m["ny"], m["nz"], m["lx"], m["ly"], m["lz"],
-                                           _ptr(pos), _ptr(vel), _ptr(p2c)), "box_particles_uniform")
+                                           ORDERS[m.get("order", "x")], _ptr(pos), _ptr(vel), _ptr(p2c)),
+          "box_particles_uniform")
     return pos, vel, p2c
 
 

@@ -222,3 +222,31 @@ def test_sort_by_cell_contract():
     assert np.array_equal(perm, np.argsort(p2c, kind="stable"))
     assert np.all(np.diff(keys) >= 0)
     assert np.array_equal(ids[:, 0].astype(np.int64), perm)
+
+
+def test_morton_numbered_mesh_is_the_same_geometry():
+    # Morton numbering permutes cubes and nodes only: the walk over it gives the same positions,
+    # lc and hop counts, and cells that map to the x-ordered ones through the cube permutation
+    nx = 5
+    a, b = oppx.box_mesh(nx, nx, nx), oppx.box_mesh(nx, nx, nx, order="morton")
+    cent = lambda m: np.round(m["node_pos"][m["c2n"]].mean(axis=1) * 4 * nx).astype(np.int64)
+    ka = {tuple(r): i for i, r in enumerate(cent(a))}
+    to_a = np.array([ka[tuple(r)] for r in cent(b)])  # morton cell -> x-ordered cell
+    assert np.array_equal(np.sort(to_a), np.arange(a["n_cells"]))
+    assert np.array_equal(a["c_vol"][to_a], b["c_vol"])
+    c2c_b = np.where(b["c2c"] >= 0, to_a[b["c2c"]], -1)
+    assert np.array_equal(np.sort(c2c_b, axis=1), np.sort(a["c2c"][to_a], axis=1))
+    ef_a, ef_b = oppx.box_field_sines(a), oppx.box_field_sines(b)
+    pa, va, ca = oppx.box_particles(3000, 9, 0.02, a)
+    pb, vb, cb = oppx.box_particles(3000, 9, 0.02, b)
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb) and np.array_equal(to_a[cb], ca)
+    sa = oracle.FempicState(a, ef_a, pa, va, ca)
+    sb = oracle.FempicState(b, ef_b, pb, vb, cb)
+    for _ in range(4):
+        sa.step(1e-3, 1.0, 0.02)
+        sb.step(1e-3, 1.0, 0.02)
+    ea, eb = sa.live(), sb.live()
+    assert ea["n"] == eb["n"]
+    for k in ("pos", "vel", "lc"):
+        assert np.array_equal(ea[k], eb[k]), k
+    assert np.array_equal(to_a[eb["p2c"]], ea["p2c"])
