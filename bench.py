@@ -34,7 +34,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_inputs(nx, n_part, seed, sigma, order, mesh_order="x"):
+def build_inputs(nx, n_part, seed, sigma, order, mesh_order="x", overlay_cells=128):
     import oppx
 
     t0 = time.time()
@@ -43,7 +43,7 @@ def build_inputs(nx, n_part, seed, sigma, order, mesh_order="x"):
     pos, vel, p2c = oppx.box_particles(n_part, seed, sigma, m)
     if order == "cell":
         pos, vel, p2c = oppx.sort_particles_by_cell(pos, vel, p2c)
-    bmin, bmax, gs = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 1.0 / 128
+    bmin, bmax, gs = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 1.0 / overlay_cells
     ov = oppx.dh_overlay(m, bmin, bmax, gs)
     log(f"inputs built in {time.time() - t0:.1f}s: {m['n_cells']} tets, {m['n_nodes']} nodes, {n_part} particles")
     return m, ef, pos, vel, p2c, ov, bmin, bmax, gs
@@ -74,7 +74,8 @@ def cpu_baseline(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, budget_s):
         steps += 1
     return dict(value=tot_parts / t_tot, unit="particles/s", cores=threads, kind="port",
                 sample=f"oracle (C++/OpenMP restatement, {threads} threads) on every {stride}th particle of the same "
-                       f"input ({n0} particles, full 6M-tet mesh and 129^3 overlay), {steps} steps, {t_tot:.1f}s")
+                       f"input ({n0} particles, the full {m['n_cells']}-tet mesh and overlay), {steps} steps, "
+                       f"{t_tot:.1f}s")
 
 
 def load_traffic(workload_tag):
@@ -368,6 +369,8 @@ def main():
                     help="move finalize: hole fill, or the Sort_Periodic policy (sort by cell every --sort-every "
                          "steps, hole fill in between)")
     ap.add_argument("--sort-every", type=int, default=2)
+    ap.add_argument("--overlay-cells", type=int, default=128,
+                    help="direct-hop overlay spacing 1/overlay-cells (dims overlay-cells + 1 per axis)")
     ap.add_argument("--mesh-order", choices=["x", "morton"], default="x",
                     help="numbering of the box mesh's cubes and nodes (x fastest, or Morton order)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle time for the CPU baseline")
@@ -414,7 +417,7 @@ def main():
     q, ms = 1e-3, 1.0
     dt = 0.5 * (1.0 / args.nx) / (sigma * math.sqrt(3.0))
     m, ef, pos, vel, p2c, ov, bmin, bmax, gs = build_inputs(args.nx, args.particles, args.seed, sigma, args.order,
-                                                            args.mesh_order)
+                                                            args.mesh_order, args.overlay_cells)
 
     t0 = time.time()
     gpu = oppx.FempicDevice(m, ef, pos, vel, p2c, overlay=ov, bmin=bmin, bmax=bmax, gs=gs)

@@ -194,6 +194,14 @@ int oppx_dh_build_overlay_nodes(int64_t n_cells, const double* c_det, const doub
 int oppx_dh_build_overlay_geom(int64_t n_cells, const double* c_det, const double* c_vol, const int* c_gbl_id,
                                const double bmin[3], const double bmax[3], double gs, int* overlay);
 
+/* Overlay files in the reference's format: the zlib stream of the int32 entries
+ * (opp_lib/src/core/opp_util.cpp:442-496), named as opp_lib/src/seq/opp_direct_hop_seq.cpp:107-119 and
+ * :288-298 name them (dh_c<cells>_gs<spacing>_ex<expansion x>_<y>_<z>.bin). load fails unless the file
+ * holds exactly n entries. */
+int oppx_dh_overlay_filename(int64_t n_cells, double grid_spacing, const double expansion[3], char* out, size_t cap);
+int oppx_dh_overlay_save(const char* path, const int32_t* overlay, int64_t n);
+int oppx_dh_overlay_load(const char* path, int32_t* overlay, int64_t n);
+
 /* NESO-advection 2-D periodic quad mesh (apps/app_neso_advection/advec_misc.h:115-143):
  * c_pos_ll [2*nx*ny] lower-left corners, c2c [4*nx*ny] = {x-1, x+1, y-1, y+1} wrapped. */
 int oppx_neso_mesh(int nx, int ny, double cell_width, double* c_pos_ll, int* c2c);

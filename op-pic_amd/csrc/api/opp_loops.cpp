@@ -220,19 +220,33 @@ void opp_init_direct_hop_cg(double grid_spacing, const opp_dat c_gbl_id, const o
     if (useGlobalMove) {
         boundingBox = std::make_shared<opp::BoundingBox>(b_box);
         cellMapper = std::make_shared<opp::CellMapper>(boundingBox, grid_spacing);
-        if (!opp_params->get<OPP_BOOL>("opp_dh_data_generate", false))
-            opp_abort("opp_init_direct_hop: reading a stored overlay is not supported; set opp_dh_data_generate = true");
-        const opp_point& mn = boundingBox->getGlobalMin();
-        const opp_point& mx = boundingBox->getGlobalMax();
-        const double lo[3] = {mn.x, mn.y, mn.z}, hi[3] = {mx.x, mx.y, mx.z};
-        opp_dat c_vol = arg2.dat, c_det = arg3.dat;
-        if (c_vol->dirty_hd == Dirty::Host) opp_download_dat(c_vol);
-        if (c_det->dirty_hd == Dirty::Host) opp_download_dat(c_det);
-        if (c_gbl_id->dirty_hd == Dirty::Host) opp_download_dat(c_gbl_id);
-        check(oppx_dh_build_overlay_geom(c_det->set->size, (const double*)c_det->data, (const double*)c_vol->data,
-                                         (const int*)c_gbl_id->data, lo, hi, grid_spacing,
-                                         cellMapper->structMeshToCellMapping),
-              "direct-hop overlay");
+        const int64_t grid_n = (int64_t)cellMapper->globalGridDimsX * (int64_t)cellMapper->globalGridDimsY *
+                               (int64_t)cellMapper->globalGridDimsZ;
+        char name[256];
+        const double ex[3] = {b_box.domain_expansion.x, b_box.domain_expansion.y, b_box.domain_expansion.z};
+        check(oppx_dh_overlay_filename(c_gbl_id->set->size, grid_spacing, ex, name, sizeof(name)), "overlay name");
+        if (!opp_params->get<OPP_BOOL>("opp_dh_data_generate", false)) {
+            // generateStructuredMeshFromFile (opp_lib/src/seq/opp_direct_hop_seq.cpp:89-130)
+            std::string path = name;
+            const std::string folder = opp_params->get<OPP_STRING>("opp_dh_mesh_folder", false);
+            if (folder != "NOT FOUND") path = folder + path;
+            check(oppx_dh_overlay_load(path.c_str(), cellMapper->structMeshToCellMapping, grid_n),
+                  ("reading direct-hop overlay " + path).c_str());
+        } else {
+            const opp_point& mn = boundingBox->getGlobalMin();
+            const opp_point& mx = boundingBox->getGlobalMax();
+            const double lo[3] = {mn.x, mn.y, mn.z}, hi[3] = {mx.x, mx.y, mx.z};
+            opp_dat c_vol = arg2.dat, c_det = arg3.dat;
+            if (c_vol->dirty_hd == Dirty::Host) opp_download_dat(c_vol);
+            if (c_det->dirty_hd == Dirty::Host) opp_download_dat(c_det);
+            if (c_gbl_id->dirty_hd == Dirty::Host) opp_download_dat(c_gbl_id);
+            check(oppx_dh_build_overlay_geom(c_det->set->size, (const double*)c_det->data, (const double*)c_vol->data,
+                                             (const int*)c_gbl_id->data, lo, hi, grid_spacing,
+                                             cellMapper->structMeshToCellMapping),
+                  "direct-hop overlay");
+            if (OPP_dh_data_dump)  // opp_lib/src/seq/opp_direct_hop_seq.cpp:277-298, to the working directory
+                check(oppx_dh_overlay_save(name, cellMapper->structMeshToCellMapping, grid_n), "writing overlay");
+        }
         cellMapper->hostToDeviceTransferMappings();
     }
     opp_profiler->end("Setup_Mover");

@@ -85,42 +85,88 @@ __global__ void __launch_bounds__(kB) sort_hist(const uint32_t* __restrict__ key
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = s_h[threadIdx.x];
 }
 
-template <int ITEMS>
+// Scatter of one digit pass. Every element's stable rank among the block's elements of its digit
+// comes from wave digit matching, round by round (rounds in index order, waves in order, lanes in
+// order). The block then lays its elements out in LDS in digit order and writes each digit's run to
+// its global offset with consecutive threads on consecutive addresses, so the global writes are
+// whole runs instead of one scattered word per lane.
+template <int ITEMS, bool LAST>
 __global__ void __launch_bounds__(kB) sort_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                    const oppx_move_counters* ctr, int64_t n_max, int shift,
                                                    const uint32_t* __restrict__ offs, int64_t nblocks,
                                                    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
-    __shared__ uint32_t s_run[kBins];                     // elements of each digit already placed by this block
-    __shared__ uint32_t s_wave[kB / kWave][kBins];        // per-wave digit counts of the current round
+    constexpr int E = kB * ITEMS;
+    __shared__ uint32_t s_run[kBins];               // elements of each digit ranked so far (then: digit start)
+    __shared__ uint32_t s_wave[kB / kWave][kBins];  // per-wave digit counts of the current round
+    __shared__ uint32_t s_key[E], s_val[E];
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
     const unsigned long long below = (1ull << lane) - 1ull;
-    s_run[threadIdx.x] = 0;
+    const int64_t base = (int64_t)blockIdx.x * E;
     const int64_t n = live_n(ctr, n_max);
+    const int m = (int)((n - base) < E ? (n - base) : E);  // elements of this block (> 0 for every launched block)
+    s_run[threadIdx.x] = 0;
+    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const int li = r * kB + threadIdx.x;
+        const bool ok = li < m;
+        key[r] = ok ? kin[base + li] : 0u;
+        val[r] = ok ? vin[base + li] : 0u;
+    }
+#pragma unroll
     for (int r = 0; r < ITEMS; r++) {
         for (int q = 0; q < kB / kWave; q++) s_wave[q][threadIdx.x] = 0;
         __syncthreads();
-        const int64_t i = (int64_t)blockIdx.x * (kB * ITEMS) + r * kB + threadIdx.x;
-        const bool ok = i < n;
-        const uint32_t key = ok ? kin[i] : 0u;
-        const uint32_t val = ok ? vin[i] : 0u;
-        const uint32_t d = (key >> shift) & 0xFFu;
-        const unsigned long long act = __ballot(ok);
-        const unsigned long long peers = digit_peers(d, act);
-        const uint32_t rank_w = (uint32_t)__popcll(peers & below);
+        const bool ok = r * kB + (int)threadIdx.x < m;
+        const uint32_t d = (key[r] >> shift) & 0xFFu;
+        const unsigned long long peers = digit_peers(d, __ballot(ok));
         if (ok && lane == __ffsll((long long)peers) - 1) s_wave[w][d] = (uint32_t)__popcll(peers);
         __syncthreads();
-        if (ok) {
-            uint32_t before = s_run[d];
-            for (int q = 0; q < w; q++) before += s_wave[q][d];
-            const uint64_t dst = (uint64_t)offs[(int64_t)d * nblocks + blockIdx.x] + before + rank_w;
-            kout[dst] = key;
-            vout[dst] = val;
-        }
+        uint32_t before = s_run[d];
+        for (int q = 0; q < w; q++) before += s_wave[q][d];
+        rank[r] = before + (uint32_t)__popcll(peers & below);
         __syncthreads();
         uint32_t add = 0;
         for (int q = 0; q < kB / kWave; q++) add += s_wave[q][threadIdx.x];
         s_run[threadIdx.x] += add;
+    }
+    __syncthreads();
+    // digit starts inside the block: exclusive scan of the counts (one digit per thread)
+    {
+        const uint32_t c = s_run[threadIdx.x];
+        uint32_t inc = c;
+#pragma unroll
+        for (int dd = 1; dd < kWave; dd <<= 1) {
+            const uint32_t t = __shfl_up(inc, dd);
+            if (lane >= dd) inc += t;
+        }
+        if (lane == kWave - 1) s_wave[0][w] = inc;
         __syncthreads();
+        uint32_t off = 0;
+        for (int q = 0; q < w; q++) off += s_wave[0][q];
+        __syncthreads();
+        s_run[threadIdx.x] = off + inc - c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        if (r * kB + (int)threadIdx.x < m) {
+            const uint32_t pos = s_run[(key[r] >> shift) & 0xFFu] + rank[r];
+            s_key[pos] = key[r];
+            s_val[pos] = val[r];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const int p = r * kB + threadIdx.x;
+        if (p < m) {
+            const uint32_t k = s_key[p];
+            const uint32_t d = (k >> shift) & 0xFFu;
+            const uint64_t dst = (uint64_t)offs[(int64_t)d * nblocks + blockIdx.x] + (uint32_t)(p - (int)s_run[d]);
+            if (!LAST) kout[dst] = k;
+            vout[dst] = s_val[p];
+        }
     }
 }
 
@@ -319,11 +365,12 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
             scan_sums<<<1, kB, 0, s>>>(sc.sums, sc.scan_blocks);
             scan_apply<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
             OPPX_LAUNCH_CHECK("sort_scan");
-#define OPPX_SCATTER(I) sort_scatter<I><<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, vin, counters, n_max, shift, sc.hist, \
-                                                                              sc.nblocks, kout, vout)
-            if (items == 4) OPPX_SCATTER(4);
-            else if (items == 8) OPPX_SCATTER(8);
-            else OPPX_SCATTER(16);
+            const bool last = shift + 8 >= bits;  // the keys of the last pass are not needed
+#define OPPX_SCATTER(I, LST) sort_scatter<I, LST><<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, vin, counters, n_max, \
+                                                                                       shift, sc.hist, sc.nblocks, kout, vout)
+            if (items == 4) { if (last) OPPX_SCATTER(4, true); else OPPX_SCATTER(4, false); }
+            else if (items == 8) { if (last) OPPX_SCATTER(8, true); else OPPX_SCATTER(8, false); }
+            else { if (last) OPPX_SCATTER(16, true); else OPPX_SCATTER(16, false); }
 #undef OPPX_SCATTER
             OPPX_LAUNCH_CHECK("sort_scatter");
             uint32_t* t = kin; kin = kout; kout = t;

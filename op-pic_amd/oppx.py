@@ -89,6 +89,9 @@ _SIGS = {
     "oppx_box_mesh_build": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_int,
                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "oppx_box_field_sines": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "oppx_dh_overlay_filename": (c_int, [c_int64, c_double, c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
+    "oppx_dh_overlay_save": (c_int, [ctypes.c_char_p, c_void_p, c_int64]),
+    "oppx_dh_overlay_load": (c_int, [ctypes.c_char_p, c_void_p, c_int64]),
     "oppx_box_particles_uniform": (c_int, [c_int64, c_uint64, c_double, c_int, c_int, c_int,
                                            c_double, c_double, c_double, c_int, c_void_p, c_void_p, c_void_p]),
     "oppx_dh_grid_dims": (c_int, [c_void_p, c_void_p, c_double, c_void_p]),
@@ -254,7 +257,7 @@ def neso_particles(n: int, seed: int, m: dict, vmax: float):
 TUNING_FIELDS = ("tile", "refill_min", "split", "persistent", "sort_items", "inline_hops", "first_block")
 
 
-def set_tuning(tile=256, refill_min=16, split=0, persistent=0, sort_items=16, inline_hops=64, first_block=256):
+def set_tuning(tile=256, refill_min=16, split=0, persistent=0, sort_items=8, inline_hops=64, first_block=256):
     """Process-wide launch shape of the walk and the sort (results do not depend on it)."""
     t = Tuning(tile, refill_min, split, persistent, sort_items, inline_hops, first_block)
     check(lib().oppx_set_tuning(ctypes.byref(t)), "set_tuning")
@@ -281,6 +284,24 @@ def dh_overlay_geom(m: dict, bmin, bmax, gs) -> np.ndarray:
     ov = np.zeros(int(np.prod(dh_dims(bmin, bmax, gs))), np.int32)
     check(lib().oppx_dh_build_overlay_geom(m["n_cells"], _ptr(m["c_det"]), _ptr(m["c_vol"]), None, _ptr(bmin),
                                            _ptr(bmax), gs, _ptr(ov)), "dh_build_overlay_geom")
+    return ov
+
+
+def dh_overlay_filename(n_cells: int, gs: float, expansion=(0.0, 0.0, 0.0)) -> str:
+    buf = ctypes.create_string_buffer(256)
+    ex = np.ascontiguousarray(expansion, np.float64)
+    check(lib().oppx_dh_overlay_filename(n_cells, gs, _ptr(ex), buf, 256), "dh_overlay_filename")
+    return buf.value.decode()
+
+
+def dh_overlay_save(path: str, overlay: np.ndarray):
+    ov = np.ascontiguousarray(overlay, np.int32)
+    check(lib().oppx_dh_overlay_save(path.encode(), _ptr(ov), ov.size), "dh_overlay_save")
+
+
+def dh_overlay_load(path: str, n: int) -> np.ndarray:
+    ov = np.zeros(n, np.int32)
+    check(lib().oppx_dh_overlay_load(path.encode(), _ptr(ov), n), "dh_overlay_load")
     return ov
 
 

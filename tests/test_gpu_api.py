@@ -35,7 +35,7 @@ def run_app(binary, params, ind, outd):
     return r
 
 
-def fempic_params(path, g, m, n_part, fill):
+def fempic_params(path, g, m, n_part, fill, generate=True):
     dh = bool(int(g["meta_dh"]))
     b0, b1 = g["bmin"], g["bmax"]
     lines = [
@@ -45,7 +45,9 @@ def fempic_params(path, g, m, n_part, fill):
         f"REAL grid_spacing = {float(g['meta_gs'])!r}",
         f"REAL bminx = {float(b0[0])!r}", f"REAL bminy = {float(b0[1])!r}", f"REAL bminz = {float(b0[2])!r}",
         f"REAL bmaxx = {float(b1[0])!r}", f"REAL bmaxy = {float(b1[1])!r}", f"REAL bmaxz = {float(b1[2])!r}",
-        f"BOOL opp_global_move = {'true' if dh else 'false'}", "BOOL opp_dh_data_generate = true",
+        f"BOOL opp_global_move = {'true' if dh else 'false'}",
+        f"BOOL opp_dh_data_generate = {'true' if generate else 'false'}",
+        f"STRING opp_dh_mesh_folder = {os.path.dirname(golden_path(name_of(g)))}/",
         "REAL opp_allocation_multiple = 1.1", "BOOL opp_profile_all = false",
         f"STRING opp_fill = {fill}", "INT opp_fill_period = 4",
     ]
@@ -57,8 +59,14 @@ def charge_close(a, b):
     return np.abs(a - b).max() <= DEPOSIT_RTOL * max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("name", ["fempic_tiny_seq.npz", "fempic_small_dh_omp1.npz"])
-def test_fempic_api_app_matches_reference(name):
+def name_of(g):
+    return "fempic_" + str(g["meta_case"]) + ".npz"
+
+
+@pytest.mark.parametrize("name,generate", [("fempic_tiny_seq.npz", True), ("fempic_small_dh_omp1.npz", True),
+                                           ("fempic_small_dh_omp1.npz", False)])
+def test_fempic_api_app_matches_reference(name, generate):
+    """generate=False: the overlay is read from the file the reference build dumped for this grid."""
     g = np.load(golden_path(name), allow_pickle=False)
     n = int(g["meta_nx"])
     m = oppx.box_mesh(n, n, n)
@@ -74,7 +82,7 @@ def test_fempic_api_app_matches_reference(name):
         vel.tofile(os.path.join(ind, "p_vel.bin"))
         p2c.astype(np.int32).tofile(os.path.join(ind, "p2c.bin"))
         par = os.path.join(td, "case.param")
-        fempic_params(par, g, m, pos.shape[0], "HoleFill_All")
+        fempic_params(par, g, m, pos.shape[0], "HoleFill_All", generate=generate)
         run_app(app("fempic_api_app"), par, ind, outd)
         rd = lambda k, t: np.fromfile(os.path.join(outd, k + ".bin"), dtype=t)
         st = np.loadtxt(os.path.join(outd, "steps.txt"), ndmin=2).astype(np.int64)
