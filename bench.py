@@ -1,11 +1,13 @@
 """Benchmark of the FEM-PIC particle hot path on one MI355X.
 
 Workload (the metric's large single-instance configuration): box [0,1]^3 split into 100^3 cubes x 6
-tets (6,000,000 tets, 1,030,301 nodes), 50,000,000 particles (seed 2) uniform in the box with
-N(0, 0.01^2) velocity components, field E = -grad(P1 interpolant of sin(pi x)sin(pi y)sin(pi z)),
-direct-hop overlay of spacing 1/128 (129^3 boxes), dt = 0.5*h/(sigma*sqrt(3)), q/m = 1e-3, FP64.
-One step = push + direct-hop + multi-hop walk + FP64 atomic charge deposit (one fused kernel)
-+ hole-fill compaction of removed particles; state stays resident in HBM.
+tets (6,000,000 tets, 1,030,301 nodes; cubes and nodes numbered in Morton order), 50,000,000
+particles (seed 2) uniform in the box with N(0, 0.01^2) velocity components, field
+E = -grad(P1 interpolant of sin(pi x)sin(pi y)sin(pi z)), direct-hop overlay of spacing 1/128
+(129^3 boxes), dt = 0.5*h/(sigma*sqrt(3)), q/m = 1e-3, FP64. One step = push + direct hop +
+multi-hop walk + FP64 atomic charge deposit (one fused move) + the move's finalize: the
+Sort_Periodic policy, a sort by cell every 2 steps and hole fill otherwise, with the sort's record
+gather applied by the next step's move. State stays resident in HBM.
 
 Prints one JSON line: particles/s = sum over timed steps of live particles / wall time, with the
 dominant kernel's roofline (algorithmic bytes / HIP-event duration vs 8 TB/s), p50/p99 step
@@ -165,16 +167,25 @@ def bench_ensemble(args):
     # eager steps: per-step latency and the live counts of the trajectory (deterministic)
     snap.restore()
     torch.cuda.synchronize()
-    lat, live = [], []
+    lat, live, phases = [], [], []
     for _ in range(S):
         live.append(b.read_counters()["size"])
-        a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
-        b.step_batch("f32")
-        e.record()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record()
+        b.step_begin()
+        b.pass_batch(deposit=False)
+        ev[1].record()
+        b.sort(drop_removed=True)
+        ev[2].record()
+        b.deposit_sorted_f32()
+        ev[3].record()
         torch.cuda.synchronize()
-        lat.append(a.elapsed_time(e))
+        lat.append(ev[0].elapsed_time(ev[3]))
+        phases.append([ev[k].elapsed_time(ev[k + 1]) for k in range(3)])
     final_eager = b.split()
+    ph = np.array(phases)
+    log(f"ensemble step phases (move, sort, fp32 deposit) ms: first {ph[0].round(2).tolist()}, "
+        f"median {np.median(ph, axis=0).round(2).tolist()}")
 
     # the S steps as one graph launch (S even: the sort's buffer swap returns to the start buffers)
     snap.restore()
@@ -247,6 +258,8 @@ def bench_ensemble(args):
         "p50_step_ms": float(np.percentile(lat, 50)),
         "p99_step_ms": float(np.percentile(lat, 99)),
         "graph_equals_eager": bool(same),
+        "phase_ms_median": {"move": float(np.median(ph[:, 0])), "sort": float(np.median(ph[:, 1])),
+                            "fp32_deposit": float(np.median(ph[:, 2]))},
         "fp32_deposit_err_vs_fp64": f32_err,
         "autotune_table": [[c, round(ms, 4)] for c, ms in table],
     }
@@ -351,6 +364,7 @@ def bench_serve(args):
         "plain_value": sp["value"],
         "plain_p50_latency_ms": sp["p50_latency_ms"],
         "plain_p99_latency_ms": sp["p99_latency_ms"],
+        "host_s": {k: st[k] for k in ("admit_s", "step_s", "retire_s")},
         "speedup_vs_plain": st["value"] / sp["value"],
     }
 
@@ -369,9 +383,11 @@ def main():
                     help="move finalize: hole fill, or the Sort_Periodic policy (sort by cell every --sort-every "
                          "steps, hole fill in between)")
     ap.add_argument("--sort-every", type=int, default=2)
+    ap.add_argument("--eager-sort", action="store_true",
+                    help="gather the records inside the sort instead of in the next step's move")
     ap.add_argument("--overlay-cells", type=int, default=128,
                     help="direct-hop overlay spacing 1/overlay-cells (dims overlay-cells + 1 per axis)")
-    ap.add_argument("--mesh-order", choices=["x", "morton"], default="x",
+    ap.add_argument("--mesh-order", choices=["x", "morton"], default="morton",
                     help="numbering of the box mesh's cubes and nodes (x fastest, or Morton order)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
@@ -443,7 +459,7 @@ def main():
             ev[2].record()
         hops_log[s] = gpu.counters[4:6].view(torch.int64)[0]
         if args.fill == "sort" and args.sort_every > 0 and (s + 1) % args.sort_every == 0:
-            gpu.sort(drop_removed=True)
+            gpu.sort(drop_removed=True, lazy=not args.eager_sort)  # the next step's move applies it
         else:
             gpu.holefill()
         if ev is not None:

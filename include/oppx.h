@@ -160,6 +160,12 @@ int oppx_stream_sync(void* stream);
  * graph-captured. Serves opp_reset_dat (opp_lib/include/opp_lib.h:98-102). */
 int oppx_fill_pattern(void* dst, int64_t n, int dim, int elem_bytes, const void* pattern, int64_t stride, void* stream);
 
+/* Adds `offset` to the index fields [first, first+count) of n_records records of rec_ints int32
+ * each (device), leaving negative entries and MAX_CELL_INDEX alone: rebasing the maps of an instance
+ * placed into a concatenated mesh (no reference counterpart). */
+int oppx_offset_indices(int32_t* base, int64_t n_records, int rec_ints, int first, int count, int32_t offset,
+                        void* stream);
+
 /* AoS (host layout, doc/api.rst:115-118) <-> SoA (device layout) transposes on the device,
  * replacing the host transposes of opp_lib/src/hip/opp_hip_utils.cpp:35-153. */
 int oppx_aos_to_soa(const void* aos, void* soa, int64_t n, int dim, int64_t stride, int elem_bytes, void* stream);
@@ -260,6 +266,17 @@ int oppx_fempic_step_fused(int64_t n_max, const oppx_fempic_parts* parts, const 
                            uint64_t* hole_bits, void* work, size_t work_bytes, int32_t* hops, double* node_charge,
                            void* stream);
 
+/* The fused step with a pending sort applied on the fly: particle i reads its pos/vel/p2c (and, for
+ * fields the step does not produce, lc) at perm[i] of `src` and writes every field at i of `parts`
+ * (separate buffers, same stride). A Sort_All / Sort_Periodic finalize can then compute only the
+ * permutation (oppx_sort_by_cell with n_dats = 0 and perm_out) and leave the record gather to the
+ * next step, which reads the records anyway (no reference counterpart: the reference gathers every
+ * dat in the finalize). Needs the two-pass move (tuning.inline_hops >= 1 and its workspace). */
+int oppx_fempic_step_fused_gather(int64_t n_max, const int32_t* perm, const oppx_fempic_parts* src,
+                                  const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh, const oppx_dh_grid* dh,
+                                  double k, double dt, oppx_move_counters* counters, uint64_t* hole_bits, void* work,
+                                  size_t work_bytes, int32_t* hops, double* node_charge, void* stream);
+
 /* Batched fused step over an ensemble of independent instances concatenated as described at
  * oppx_batch_inst: one launch walks every instance's particles (no reference counterpart: OP-PIC
  * runs one instance per process). cell_inst (device [mesh->n_cells]) is the instance of every cell,
@@ -335,11 +352,12 @@ int oppx_holefill(int64_t n_max, int64_t stride, oppx_move_counters* counters, c
  * node_charge (double, nullable) and/or node_charge_f32 (nullable). The OPP_INC deposit of
  * apps/app_fempic/kernels.h:164-175 in the segmented-reduction form of
  * apps_codegen/app_fempic_cg/hip/deposit_charge_on_nodes_kernel_loop.hpp:98-158; every addition order
- * is fixed by the data (deterministic). */
+ * is fixed by the data (deterministic). scratch: oppx_fempic_deposit_scratch_bytes(n_max) bytes. */
+size_t oppx_fempic_deposit_scratch_bytes(int64_t n_max);
 int oppx_fempic_deposit_sorted_f32(int64_t n_max, int64_t stride, oppx_move_counters* counters, const int32_t* p2c,
                                    const double* lc, int64_t n_cells, const int32_t* node_start, const int32_t* node_end,
                                    const int32_t* node_ref, int64_t n_nodes, float* cell_partials, double* node_charge,
-                                   float* node_charge_f32, void* stream);
+                                   float* node_charge_f32, void* scratch, size_t scratch_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------------
  * Sort
@@ -357,6 +375,11 @@ size_t oppx_sort_scratch_bytes(int64_t n_max);
 int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counters, int64_t n_cells,
                       const int32_t* p2c, const oppx_dat_desc* dats, const oppx_dat_desc* out_dats, int n_dats,
                       int32_t* perm_out, int drop_removed, void* scratch, size_t scratch_bytes, void* stream);
+
+/* out_dats[q][i] = dats[q][perm[i]] for i < counters->size: materialises a permutation left pending
+ * by oppx_fempic_step_fused_gather's caller (e.g. before the state is read back). */
+int oppx_gather_dats(int64_t n_max, int64_t stride, const oppx_move_counters* counters, const int32_t* perm,
+                     const oppx_dat_desc* dats, const oppx_dat_desc* out_dats, int n_dats, void* stream);
 
 /* Range lookup in cell-sorted particles: out[j] = the first index in [0, counters->size) whose p2c is
  * >= keys[j] (device arrays, k keys). With keys = {first cell, end cell} of a mesh region this gives

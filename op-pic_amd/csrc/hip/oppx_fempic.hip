@@ -37,6 +37,13 @@ struct StepArgs {
     double* __restrict__ lc;
     int32_t* __restrict__ p2c;
     DhView dh;
+    // gather mode: particle i of the pass reads its state at perm[i] of the src arrays (a pending
+    // sort permutation applied on the fly) and writes every field at i of pos/vel/lc/p2c
+    const int32_t* __restrict__ perm;
+    const double* __restrict__ src_pos;
+    const double* __restrict__ src_vel;
+    const double* __restrict__ src_lc;
+    const int32_t* __restrict__ src_p2c;
     double k, dt;
     int64_t n_max;
     int tile;        // particles per wave tile (multiple of 64, <= kMaxTile)
@@ -348,7 +355,7 @@ struct WorkList {
     int64_t cap;
 };
 
-template <int BS, bool PUSH, bool DH, bool DEPOSIT, bool BATCH>
+template <int BS, bool PUSH, bool DH, bool DEPOSIT, bool BATCH, bool GATHER>
 __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t nblocks, int inline_hops, WorkList W) {
     const uint32_t b = xcd_block(blockIdx.x, nblocks);
     const int64_t i = (int64_t)b * BS + threadIdx.x;
@@ -360,13 +367,25 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
     const bool live = i < n;
     const int64_t S = P.stride;
     const int lane = threadIdx.x & (kWave - 1);
+    const double* __restrict__ spos = GATHER ? P.src_pos : P.pos;
+    const double* __restrict__ svel = GATHER ? P.src_vel : P.vel;
+    const int32_t* __restrict__ sp2c = GATHER ? P.src_p2c : P.p2c;
+    int64_t src = i;
+    bool fault = false;
+    if (GATHER && live) {
+        src = P.perm[i];
+        if (!((uint64_t)src < (uint64_t)S)) {  // a corrupt permutation entry: report, touch nothing
+            fault = true;
+            src = i;
+        }
+    }
 
-    int cell = live ? ld_stream(P.p2c + i) : kMaxCell;
+    int cell = (live && !fault) ? ld_stream(sp2c + src) : kMaxCell;
     // a particle already holding MAX_CELL_INDEX is skipped, as the reference loops do
     // (apps_codegen/app_fempic_cg/seq/move_kernel_loop.hpp:92-94)
-    const bool skip = live && cell == kMaxCell;
-    bool fault = live && !skip && !((unsigned)cell < (unsigned)P.n_cells);
-    bool removed = false, pending = false;
+    const bool skip = live && !fault && cell == kMaxCell;
+    fault = fault || (live && !skip && !((unsigned)cell < (unsigned)P.n_cells));
+    bool removed = false, pending = false, pushed = false, p2c_done = false, lc_done = false;
     int h = 0;
     double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
     __shared__ DepositTable s_table;
@@ -390,12 +409,12 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
             }
         }
         if (!fault) {
-            double x = ld_stream(P.pos + i), y = ld_stream(P.pos + S + i), z = ld_stream(P.pos + 2 * S + i);
+            double x = ld_stream(spos + src), y = ld_stream(spos + S + src), z = ld_stream(spos + 2 * S + src);
             if (PUSH) {
                 // calculate_new_pos_vel_kernel: v = fma(k, E, v); x = fma(v, dt, x)
                 const double* e = P.ef + 3 * (int64_t)cell;
                 const double ex = e[0], ey = e[1], ez = e[2];
-                double vx = ld_stream(P.vel + i), vy = ld_stream(P.vel + S + i), vz = ld_stream(P.vel + 2 * S + i);
+                double vx = ld_stream(svel + src), vy = ld_stream(svel + S + src), vz = ld_stream(svel + 2 * S + src);
                 vx = fma(kk, ex, vx);
                 x = fma(vx, dt, x);
                 vy = fma(kk, ey, vy);
@@ -404,6 +423,7 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
                 z = fma(vz, dt, z);
                 st_stream(P.vel + i, vx); st_stream(P.vel + S + i, vy); st_stream(P.vel + 2 * S + i, vz);
                 st_stream(P.pos + i, x); st_stream(P.pos + S + i, y); st_stream(P.pos + 2 * S + i, z);
+                pushed = true;
             }
             if (DH && (!BATCH || D.cell != nullptr)) {
                 cell = dh_lookup(D, x, y, z);
@@ -423,10 +443,26 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
             if (!pending && !removed && h > 0) {
                 st_stream(P.lc + i, l0); st_stream(P.lc + S + i, l1);
                 st_stream(P.lc + 2 * S + i, l2); st_stream(P.lc + 3 * S + i, l3);
+                lc_done = true;
             }
             st_stream(P.p2c + i, removed ? kMaxCell : cell);  // a pending particle resumes from `cell`
+            p2c_done = true;
         }
         if (P.hops) st_stream(P.hops + i, h);
+    }
+    if (GATHER && live) {
+        // every field of the output slot is written: what the pass did not produce is carried over
+        // (a pending particle's lc is final only after the worklist walk, which overwrites this)
+        if (!pushed) {
+            for (int d = 0; d < 3; d++) {
+                st_stream(P.pos + d * S + i, ld_stream(spos + d * S + src));
+                st_stream(P.vel + d * S + i, ld_stream(svel + d * S + src));
+            }
+        }
+        if (!p2c_done) st_stream(P.p2c + i, ld_stream(sp2c + src));
+        if (!lc_done)
+            for (int d = 0; d < 4; d++) st_stream(P.lc + d * S + i, ld_stream(P.src_lc + d * S + src));
+        if (P.hops && !(live && !skip && !fault)) P.hops[i] = h;
     }
 
     // hand the still-moving particles to worklist (blockIdx mod kSlots): same XCD, spread counters
@@ -855,16 +891,16 @@ void fold_slots(const StepArgs& P, hipStream_t s) {
 
 // Two-pass move (first-hop pass + worklist walk) when the work buffer is large enough and the
 // tuning asks for it; returns false when the caller should use the cooperative tile walk instead.
-template <bool PUSH, bool DH, bool DEPOSIT, bool BATCH>
+template <bool PUSH, bool DH, bool DEPOSIT, bool BATCH, bool GATHER = false>
 bool launch_two_pass(const StepArgs& P, int64_t n_max, void* work, size_t work_bytes, hipStream_t s) {
     if (!P.slots || g_tuning.inline_hops < 1 || work_bytes < work_need(n_max)) return false;
     WorkList W{reinterpret_cast<int2*>(reinterpret_cast<char*>(work) + kWorkHeader), work_cap(n_max)};
     if (g_tuning.first_block == kMaxBlock) {
         const unsigned g = grid_for(n_max, kMaxBlock);
-        fempic_first_kernel<kMaxBlock, PUSH, DH, DEPOSIT, BATCH><<<g, kMaxBlock, 0, s>>>(P, g, g_tuning.inline_hops, W);
+        fempic_first_kernel<kMaxBlock, PUSH, DH, DEPOSIT, BATCH, GATHER><<<g, kMaxBlock, 0, s>>>(P, g, g_tuning.inline_hops, W);
     } else {
         const unsigned g = grid_for(n_max, kBlock);
-        fempic_first_kernel<kBlock, PUSH, DH, DEPOSIT, BATCH><<<g, kBlock, 0, s>>>(P, g, g_tuning.inline_hops, W);
+        fempic_first_kernel<kBlock, PUSH, DH, DEPOSIT, BATCH, GATHER><<<g, kBlock, 0, s>>>(P, g, g_tuning.inline_hops, W);
     }
     auto walk = fempic_walk_kernel<DEPOSIT>;
     unsigned wg = resident_blocks(reinterpret_cast<const void*>(walk), kBlock);
@@ -1001,6 +1037,48 @@ int oppx_fempic_step_fused(int64_t n_max, const oppx_fempic_parts* parts, const 
     }
     fold_slots(P, s);
     OPPX_LAUNCH_CHECK("fempic_step_fused");
+    return OPPX_OK;
+}
+
+int oppx_fempic_step_fused_gather(int64_t n_max, const int32_t* perm, const oppx_fempic_parts* src,
+                                  const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh, const oppx_dh_grid* dh,
+                                  double k, double dt, oppx_move_counters* counters, uint64_t* hole_bits, void* work,
+                                  size_t work_bytes, int32_t* hops, double* node_charge, void* stream) {
+    StepArgs P;
+    int rc = fill_args(P, parts, mesh, dh, counters);
+    if (rc) return rc;
+    OPPX_ARG_CHECK(perm && src && src->pos && src->vel && src->lc && src->p2c,
+                   "oppx_fempic_step_fused_gather: permutation and source state required");
+    OPPX_ARG_CHECK(src->stride == parts->stride, "oppx_fempic_step_fused_gather: source and target strides differ");
+    OPPX_ARG_CHECK(src->pos != parts->pos && src->vel != parts->vel && src->lc != parts->lc && src->p2c != parts->p2c,
+                   "oppx_fempic_step_fused_gather: source and target must be separate buffers");
+    OPPX_ARG_CHECK(n_max >= 0 && n_max <= parts->stride, "oppx_fempic_step_fused_gather: n_max beyond capacity");
+    OPPX_ARG_CHECK(hole_bits != nullptr, "oppx_fempic_step_fused_gather: hole_bits required");
+    OPPX_ARG_CHECK(g_tuning.inline_hops >= 1 && work && work_bytes >= work_need(n_max),
+                   "oppx_fempic_step_fused_gather: needs the two-pass move (tuning.inline_hops >= 1 and its workspace)");
+    hipStream_t s = as_stream(stream);
+    if (n_max == 0) return OPPX_OK;
+    P.n_max = n_max;
+    P.k = k;
+    P.dt = dt;
+    P.hole_bits = hole_bits;
+    P.hops = hops;
+    P.charge = node_charge;
+    P.perm = perm;
+    P.src_pos = src->pos;
+    P.src_vel = src->vel;
+    P.src_lc = src->lc;
+    P.src_p2c = src->p2c;
+    attach_slots(P, work, work_bytes, s);
+    if (dh) {
+        if (node_charge) launch_two_pass<true, true, true, false, true>(P, n_max, work, work_bytes, s);
+        else launch_two_pass<true, true, false, false, true>(P, n_max, work, work_bytes, s);
+    } else {
+        if (node_charge) launch_two_pass<true, false, true, false, true>(P, n_max, work, work_bytes, s);
+        else launch_two_pass<true, false, false, false, true>(P, n_max, work, work_bytes, s);
+    }
+    fold_slots(P, s);
+    OPPX_LAUNCH_CHECK("fempic_step_fused_gather");
     return OPPX_OK;
 }
 

@@ -79,6 +79,17 @@ __global__ void fill_pattern_kernel(T* __restrict__ dst, int64_t n, int dim, int
     }
 }
 
+// entries [first, first + count) of every `rec`-int record: v -> v + offset unless v < 0 or v == MAX
+__global__ void offset_indices_kernel(int32_t* __restrict__ base, int64_t n, int rec, int first, int count,
+                                      int32_t offset) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * count; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = t / count, k = t % count;
+        int32_t* p = base + i * rec + first + k;
+        const int32_t v = *p;
+        if (v >= 0 && v != kMaxCell) *p = v + offset;
+    }
+}
+
 }  // namespace
 }  // namespace oppx
 
@@ -193,6 +204,19 @@ int oppx_fill_pattern(void* dst, int64_t n, int dim, int elem_bytes, const void*
     else
         fill_pattern_kernel<int32_t><<<(unsigned)g, 256, 0, as_stream(stream)>>>((int32_t*)dst, n, dim, stride, p);
     OPPX_LAUNCH_CHECK("fill_pattern");
+    return OPPX_OK;
+}
+
+int oppx_offset_indices(int32_t* base, int64_t n_records, int rec_ints, int first, int count, int32_t offset,
+                        void* stream) {
+    OPPX_ARG_CHECK(n_records >= 0 && rec_ints > 0 && first >= 0 && count > 0 && first + count <= rec_ints,
+                   "oppx_offset_indices: bad record layout");
+    if (n_records == 0 || offset == 0) return OPPX_OK;
+    OPPX_ARG_CHECK(base != nullptr, "oppx_offset_indices: null pointer");
+    int64_t g = grid_for(n_records * count, 256);
+    if (g > 8192) g = 8192;
+    offset_indices_kernel<<<(unsigned)g, 256, 0, as_stream(stream)>>>(base, n_records, rec_ints, first, count, offset);
+    OPPX_LAUNCH_CHECK("offset_indices");
     return OPPX_OK;
 }
 

@@ -22,6 +22,7 @@ constexpr int kElems = kB * kItems;
 constexpr int kMinSortItems = 4;     // smallest radix block (elements per thread) the scratch is sized for
 constexpr int kBins = 256;
 constexpr int kMaxDats = 16;
+constexpr int kRemSlots = 256;  // removed-particle counters of sort_prepare
 
 struct GatherTable {
     int n;
@@ -62,7 +63,9 @@ __global__ void __launch_bounds__(kB) sort_prepare(const int32_t* __restrict__ p
         vals[i] = (uint32_t)i;
     }
     const unsigned long long m = __ballot(rem);
-    if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(n_removed, (unsigned long long)__popcll(m));
+    // per-wave counts land on one of kRemSlots counters (block id modulo kRemSlots): one shared
+    // counter would serialise every wave's atomic in one L2 channel
+    if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(n_removed + (blockIdx.x % kRemSlots), (unsigned long long)__popcll(m));
 }
 
 template <int ITEMS>
@@ -245,6 +248,7 @@ __global__ void __launch_bounds__(kB) sort_gather(const uint32_t* __restrict__ p
     const int64_t j = (int64_t)blockIdx.x * kB + threadIdx.x;
     if (j >= n) return;
     const int64_t from = perm[j];
+    if (from >= T.stride) return;  // never read outside the dats
     for (int q = 0; q < T.n; q++) {
         if (T.eb[q] == 8) gather_rec<double>(T.in[q], T.out[q], T.dim[q], T.stride, from, j);
         else gather_rec<int32_t>(T.in[q], T.out[q], T.dim[q], T.stride, from, j);
@@ -252,7 +256,7 @@ __global__ void __launch_bounds__(kB) sort_gather(const uint32_t* __restrict__ p
     if (perm_out) perm_out[j] = (int32_t)from;
 }
 
-__global__ void zero_u64(unsigned long long* p) { *p = 0ull; }
+__global__ void zero_u64(unsigned long long* p) { p[threadIdx.x] = 0ull; }  // kRemSlots threads
 
 __global__ void lower_bound_kernel(const int32_t* __restrict__ p2c, const oppx_move_counters* ctr, int64_t n_max,
                                    const int32_t* __restrict__ keys, int64_t k, int64_t* __restrict__ out) {
@@ -271,7 +275,8 @@ __global__ void lower_bound_kernel(const int32_t* __restrict__ p2c, const oppx_m
 
 __global__ void sort_finalize(oppx_move_counters* ctr, int64_t n_max, const unsigned long long* n_removed, int drop) {
     const int64_t n = live_n(ctr, n_max);
-    const int64_t r = (int64_t)*n_removed;
+    int64_t r = 0;
+    for (int q = 0; q < kRemSlots; q++) r += (int64_t)n_removed[q];
     ctr->removed = (int32_t)r;
     ctr->holes_low = 0;
     if (drop) ctr->size = (int32_t)(n - r);
@@ -310,7 +315,7 @@ size_t scratch_need(int64_t n_max) {
     const int64_t hist_n = nb * kBins;
     const int64_t scan_blocks = (hist_n + kElems - 1) / kElems;
     return 4 * align256((size_t)(n_max > 0 ? n_max : 1) * 4) + align256((size_t)hist_n * 4) +
-           align256((size_t)scan_blocks * 4) + 256;
+           align256((size_t)scan_blocks * 4) + kRemSlots * sizeof(unsigned long long);
 }
 
 }  // namespace
@@ -349,7 +354,7 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
     const int items = g_tuning.sort_items;
     OPPX_ARG_CHECK(items == 4 || items == 8 || items == 16, "oppx_sort_by_cell: tuning.sort_items must be 4, 8 or 16");
     SortScratch sc = carve(scratch, n_max, items);
-    zero_u64<<<1, 1, 0, s>>>(sc.n_removed);  // a kernel, not a memset node, so the sort can be captured
+    zero_u64<<<1, kRemSlots, 0, s>>>(sc.n_removed);  // a kernel, not a memset node, so the sort can be captured
     if (n_max > 0) {
         sort_prepare<<<grid_for(n_max, kB), kB, 0, s>>>(p2c, counters, n_max, n_cells, sc.ka, sc.va, sc.n_removed);
         OPPX_LAUNCH_CHECK("sort_prepare");
@@ -381,6 +386,31 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
     }
     sort_finalize<<<1, 1, 0, s>>>(counters, n_max, sc.n_removed, drop_removed);
     OPPX_LAUNCH_CHECK("sort_finalize");
+    return OPPX_OK;
+}
+
+int oppx_gather_dats(int64_t n_max, int64_t stride, const oppx_move_counters* counters, const int32_t* perm,
+                     const oppx_dat_desc* dats, const oppx_dat_desc* out_dats, int n_dats, void* stream) {
+    OPPX_ARG_CHECK(counters && perm && n_dats > 0 && n_dats <= kMaxDats && dats && out_dats,
+                   "oppx_gather_dats: bad arguments");
+    OPPX_ARG_CHECK(n_max >= 0 && stride >= n_max, "oppx_gather_dats: need 0 <= n_max <= stride");
+    GatherTable T{};
+    T.n = n_dats;
+    T.stride = stride;
+    for (int q = 0; q < n_dats; q++) {
+        OPPX_ARG_CHECK(dats[q].base && out_dats[q].base && dats[q].dim > 0 && dats[q].dim == out_dats[q].dim &&
+                           dats[q].elem_bytes == out_dats[q].elem_bytes &&
+                           (dats[q].elem_bytes == 4 || dats[q].elem_bytes == 8) && dats[q].base != out_dats[q].base,
+                       "oppx_gather_dats: bad dat descriptor");
+        T.in[q] = dats[q].base;
+        T.out[q] = out_dats[q].base;
+        T.dim[q] = dats[q].dim;
+        T.eb[q] = dats[q].elem_bytes;
+    }
+    if (n_max == 0) return OPPX_OK;
+    sort_gather<<<grid_for(n_max, kB), kB, 0, as_stream(stream)>>>(reinterpret_cast<const uint32_t*>(perm),
+                                                                   counters, n_max, T, nullptr);
+    OPPX_LAUNCH_CHECK("gather_dats");
     return OPPX_OK;
 }
 

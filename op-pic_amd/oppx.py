@@ -82,6 +82,7 @@ _SIGS = {
     "oppx_memcpy_dtod": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "oppx_memset_zero": (c_int, [c_void_p, c_size_t, c_void_p]),
     "oppx_stream_sync": (c_int, [c_void_p]),
+    "oppx_offset_indices": (c_int, [c_void_p, c_int64, c_int, c_int, c_int, c_int32, c_void_p]),
     "oppx_fill_pattern": (c_int, [c_void_p, c_int64, c_int, c_int, c_void_p, c_int64, c_void_p]),
     "oppx_aos_to_soa": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int64, c_int, c_void_p]),
     "oppx_soa_to_aos": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_int64, c_int, c_void_p]),
@@ -113,6 +114,11 @@ _SIGS = {
     "oppx_fempic_step_fused": (c_int, [c_int64, POINTER(FempicParts), POINTER(TetMesh), POINTER(DhGrid),
                                        c_double, c_double, c_void_p, c_void_p, c_void_p, ctypes.c_size_t, c_void_p,
                                        c_void_p, c_void_p]),
+    "oppx_fempic_step_fused_gather": (c_int, [c_int64, c_void_p, POINTER(FempicParts), POINTER(FempicParts),
+                                              POINTER(TetMesh), POINTER(DhGrid), c_double, c_double, c_void_p,
+                                              c_void_p, c_void_p, ctypes.c_size_t, c_void_p, c_void_p, c_void_p]),
+    "oppx_gather_dats": (c_int, [c_int64, c_int64, c_void_p, c_void_p, POINTER(DatDesc), POINTER(DatDesc), c_int,
+                                 c_void_p]),
     "oppx_fempic_batch_step_fused": (c_int, [c_int64, POINTER(FempicParts), POINTER(TetMesh), c_void_p, c_void_p,
                                              c_int32, c_void_p, c_void_p, c_void_p, ctypes.c_size_t, c_void_p,
                                              c_void_p, c_void_p]),
@@ -122,8 +128,10 @@ _SIGS = {
     "oppx_fempic_step_begin": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
     "oppx_hole_bits_words": (c_int64, [c_int64]),
     "oppx_node_cell_csr": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
+    "oppx_fempic_deposit_scratch_bytes": (ctypes.c_size_t, [c_int64]),
     "oppx_fempic_deposit_sorted_f32": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
-                                               c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+                                               c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                               ctypes.c_size_t, c_void_p]),
     "oppx_lower_bound": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "oppx_sort_scratch_bytes": (c_size_t, [c_int64]),
     "oppx_sort_by_cell": (c_int, [c_int64, c_int64, c_void_p, c_int64, c_void_p, POINTER(DatDesc), POINTER(DatDesc),
@@ -392,42 +400,73 @@ class FempicDevice:
 
     # -- the reference loops, one C-ABI call each --------------------------------------------
     def push(self, charge, mass, dt):
+        self.materialize()
         k = charge / mass * dt
         check(lib().oppx_fempic_push(self.n_max, ctypes.byref(self.parts), ctypes.byref(self.mesh), k, dt,
                                      self.counters.data_ptr(), self.stream()), "fempic_push")
 
     def move(self, record_hops=True):
+        self.materialize()
         check(lib().oppx_fempic_move(self.n_max, ctypes.byref(self.parts), ctypes.byref(self.mesh), self._dh_ref(),
                                      self.counters.data_ptr(), self.hole_bits.data_ptr(), self.work.data_ptr(),
                                      self.work_bytes, self.hops.data_ptr() if record_hops else None, self.stream()),
               "fempic_move")
 
     def holefill(self):
+        self.materialize()
         check(lib().oppx_holefill(self.n_max, self.cap, self.counters.data_ptr(), self.hole_bits.data_ptr(),
                                   self.dats, 4, self.scratch.data_ptr(), self.scratch_bytes, self.stream()),
               "holefill")
 
-    def sort(self, drop_removed=False, perm=None):
-        """Stable sort of the particle dats by cell (removed last); drop_removed applies the finalize."""
+    def _swap_buffers(self):
         torch = self.torch
         if not hasattr(self, "_swap"):
             self._swap = [torch.empty_like(t) for t in (self.pos, self.vel, self.lc, self.p2c)]
             self.sort_scratch_bytes = int(lib().oppx_sort_scratch_bytes(self.cap))
             self.sort_scratch = torch.zeros(self.sort_scratch_bytes, dtype=torch.uint8, device=self.dev)
-        cur = [self.pos, self.vel, self.lc, self.p2c]
+            self.perm = torch.zeros(self.cap, dtype=torch.int32, device=self.dev)
+        return self._swap
+
+    def _descs(self, tensors):
         dims = [(3, 8), (3, 8), (4, 8), (1, 4)]
-        ins = (DatDesc * 4)(*[DatDesc(t.data_ptr(), d, e) for t, (d, e) in zip(cur, dims)])
-        outs = (DatDesc * 4)(*[DatDesc(t.data_ptr(), d, e) for t, (d, e) in zip(self._swap, dims)])
+        return (DatDesc * 4)(*[DatDesc(t.data_ptr(), d, e) for t, (d, e) in zip(tensors, dims)])
+
+    def sort(self, drop_removed=False, perm=None, lazy=False):
+        """Stable sort of the particle dats by cell (removed last); drop_removed applies the finalize.
+        lazy=True computes only the permutation and leaves the record gather pending: the next fused
+        step reads its inputs through it (pass_fused), or materialize() applies it."""
+        self.materialize()
+        swap = self._swap_buffers()
+        cur = [self.pos, self.vel, self.lc, self.p2c]
+        out_perm = perm if perm is not None else (self.perm if lazy else None)
         check(lib().oppx_sort_by_cell(self.n_max, self.cap, self.counters.data_ptr(), self.nc, self.p2c.data_ptr(),
-                                      ins, outs, 4, perm.data_ptr() if perm is not None else None,
+                                      self._descs(cur) if not lazy else None, self._descs(swap) if not lazy else None,
+                                      0 if lazy else 4, out_perm.data_ptr() if out_perm is not None else None,
                                       1 if drop_removed else 0, self.sort_scratch.data_ptr(), self.sort_scratch_bytes,
                                       self.stream()), "sort_by_cell")
-        self.pos, self.vel, self.lc, self.p2c = self._swap
+        if lazy:
+            self.pending = True
+            return
+        self.pos, self.vel, self.lc, self.p2c = swap
         self._swap = cur
         self._rebind()
 
+    def materialize(self):
+        """Apply a pending permutation to the particle dats."""
+        if not getattr(self, "pending", False):
+            return
+        swap = self._swap_buffers()
+        cur = [self.pos, self.vel, self.lc, self.p2c]
+        check(lib().oppx_gather_dats(self.n_max, self.cap, self.counters.data_ptr(), self.perm.data_ptr(),
+                                     self._descs(cur), self._descs(swap), 4, self.stream()), "gather_dats")
+        self.pos, self.vel, self.lc, self.p2c = swap
+        self._swap = cur
+        self._rebind()
+        self.pending = False
+
     def deposit_sorted_f32(self):
         """FP32 Kahan deposit over the (cell-sorted) particles; returns the FP32 node charge tensor."""
+        self.materialize()
         torch = self.torch
         if not hasattr(self, "node_start"):
             st, ref = node_cell_csr(self._mesh_host)
@@ -435,10 +474,13 @@ class FempicDevice:
             self.node_ref = torch.from_numpy(ref).to(self.dev)
             self.cell_partials = torch.zeros(4 * self.nc, dtype=torch.float32, device=self.dev)
             self.charge_f32 = torch.zeros(self.nn, dtype=torch.float32, device=self.dev)
+            self.dep_bytes = int(lib().oppx_fempic_deposit_scratch_bytes(self.cap))
+            self.dep_scratch = torch.zeros(self.dep_bytes, dtype=torch.uint8, device=self.dev)
         check(lib().oppx_fempic_deposit_sorted_f32(self.n_max, self.cap, self.counters.data_ptr(), self.p2c.data_ptr(),
                                                    self.lc.data_ptr(), self.nc, self.node_start.data_ptr(), None,
                                                    self.node_ref.data_ptr(), self.nn, self.cell_partials.data_ptr(),
-                                                   None, self.charge_f32.data_ptr(), self.stream()),
+                                                   None, self.charge_f32.data_ptr(), self.dep_scratch.data_ptr(),
+                                                   self.dep_bytes, self.stream()),
               "deposit_sorted_f32")
         return self.charge_f32
 
@@ -449,6 +491,7 @@ class FempicDevice:
                                   DatDesc(self.lc.data_ptr(), 4, 8), DatDesc(self.p2c.data_ptr(), 1, 4))
 
     def deposit(self):
+        self.materialize()
         check(lib().oppx_fempic_deposit_atomic(self.n_max, ctypes.byref(self.parts), ctypes.byref(self.mesh),
                                                self.charge.data_ptr(), self.counters.data_ptr(), self.stream()),
               "fempic_deposit")
@@ -459,6 +502,23 @@ class FempicDevice:
 
     def pass_fused(self, charge, mass, dt, record_hops=False, deposit=True):
         k = charge / mass * dt
+        if getattr(self, "pending", False):  # the step reads its inputs through the pending sort
+            swap = self._swap_buffers()
+            src = FempicParts(self.cap, self.pos.data_ptr(), self.vel.data_ptr(), self.lc.data_ptr(),
+                              self.p2c.data_ptr())
+            dst = FempicParts(self.cap, swap[0].data_ptr(), swap[1].data_ptr(), swap[2].data_ptr(),
+                              swap[3].data_ptr())
+            check(lib().oppx_fempic_step_fused_gather(
+                self.n_max, self.perm.data_ptr(), ctypes.byref(src), ctypes.byref(dst), ctypes.byref(self.mesh),
+                self._dh_ref(), k, dt, self.counters.data_ptr(), self.hole_bits.data_ptr(), self.work.data_ptr(),
+                self.work_bytes, self.hops.data_ptr() if record_hops else None,
+                self.charge.data_ptr() if deposit else None, self.stream()), "fempic_step_fused_gather")
+            cur = [self.pos, self.vel, self.lc, self.p2c]
+            self.pos, self.vel, self.lc, self.p2c = swap
+            self._swap = cur
+            self._rebind()
+            self.pending = False
+            return
         check(lib().oppx_fempic_step_fused(self.n_max, ctypes.byref(self.parts), ctypes.byref(self.mesh),
                                            self._dh_ref(), k, dt, self.counters.data_ptr(),
                                            self.hole_bits.data_ptr(), self.work.data_ptr(), self.work_bytes,
@@ -498,6 +558,7 @@ class FempicDevice:
         return self.n_max
 
     def download(self):
+        self.materialize()
         n = self.read_counters()["size"]
         return dict(n=n, pos=self.pos[:, :n].T.cpu().numpy().copy(), vel=self.vel[:, :n].T.cpu().numpy().copy(),
                     lc=self.lc[:, :n].T.cpu().numpy().copy(), p2c=self.p2c[:n].cpu().numpy().copy(),

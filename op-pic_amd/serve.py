@@ -81,17 +81,18 @@ def _aux_bytes(vol: np.ndarray, c2c_global: np.ndarray) -> np.ndarray:
 
 @dataclass
 class Template:
-    """Host-side arrays of one instance, prepared once and uploaded per admission."""
+    """One instance, staged once in pinned host memory in the arena's layout (indices local to the
+    instance) so an admission is a handful of asynchronous copies plus index rebasing on the device."""
     inst: dict
     n_cells: int
     n_nodes: int
     n_part: int
-    csr_start: np.ndarray
-    csr_ref: np.ndarray
     overlay_words: int
+    host: dict = field(default_factory=dict)
 
     @staticmethod
     def of(inst: dict) -> "Template":
+        import torch
         m = inst["mesh"]
         nc, nn = int(m["n_cells"]), int(m["n_nodes"])
         if not ((m["c2c"] >= -1) & (m["c2c"] < nc)).all() or not ((m["c2n"] >= 0) & (m["c2n"] < nn)).all():
@@ -102,7 +103,15 @@ class Template:
         if ov is not None and not (((ov >= 0) & (ov < nc)) | (ov == MAX_CELL_INDEX)).all():
             raise ValueError("overlay entries out of range")
         st, ref = oppx.node_cell_csr(m)
-        return Template(inst, nc, nn, len(inst["p2c"]), st, ref, 0 if ov is None else int(ov.size))
+        pin = lambda a: torch.from_numpy(np.ascontiguousarray(a)).pin_memory()
+        host = dict(det=pin(m["c_det"].reshape(-1)), aux=pin(_aux_bytes(m["c_vol"], m["c2c"].astype(np.int32))),
+                    c2n=pin(m["c2n"].reshape(-1).astype(np.int32)), ef=pin(inst["ef"].reshape(-1)),
+                    nstart=pin(st[:-1].astype(np.int32)), nend=pin(st[1:].astype(np.int32)),
+                    nref=pin(ref.astype(np.int32)), pos=pin(inst["pos"].T), vel=pin(inst["vel"].T),
+                    p2c=pin(inst["p2c"].astype(np.int32)))
+        if ov is not None:
+            host["overlay"] = pin(ov.astype(np.int32))
+        return Template(inst, nc, nn, len(inst["p2c"]), 0 if ov is None else int(ov.size), host)
 
     def device_bytes(self) -> int:
         """HBM a resident request holds: mesh records, node rows, overlay, particles (+ sort buffers)."""
@@ -131,6 +140,10 @@ class FempicArena:
         self.node_start, self.node_end, self.node_ref = z(N, torch.int32), z(N, torch.int32), z(4 * C, torch.int32)
         self.charge_f32, self.cell_partials = z(N, torch.float32), z(4 * C, torch.float32)
         self.inst_table = z(max_insts * ctypes.sizeof(BatchInst), torch.uint8)
+        # pinned host mirrors: records and the live count are copied asynchronously from here (a slot's
+        # record is rewritten only after its request retired, which synchronises)
+        self.h_table = torch.zeros(max_insts * ctypes.sizeof(BatchInst), dtype=torch.uint8, pin_memory=True)
+        self.h_size = torch.zeros(1, dtype=torch.int32, pin_memory=True)
         self.pos, self.vel, self.lc = (torch.zeros((d, P), dtype=torch.float64, device=self.dev) for d in (3, 3, 4))
         self.p2c = torch.full((P,), MAX_CELL_INDEX, dtype=torch.int32, device=self.dev)
         self.swap = [torch.empty_like(t) for t in (self.pos, self.vel, self.lc, self.p2c)]
@@ -140,6 +153,8 @@ class FempicArena:
         self.work = z(max(self.work_bytes, 8), torch.uint8)
         self.sort_bytes = int(L.oppx_sort_scratch_bytes(P))
         self.sort_scratch = z(self.sort_bytes, torch.uint8)
+        self.dep_bytes = int(L.oppx_fempic_deposit_scratch_bytes(P))
+        self.dep_scratch = z(self.dep_bytes, torch.uint8)
         self.cells, self.nodes, self.ovl = RangeAlloc(C), RangeAlloc(N), RangeAlloc(max(1, max_overlay))
         self.slots_free = list(range(max_insts - 1, -1, -1))
         self.res = {}  # slot -> (template, cb, nb, ob)
@@ -158,29 +173,38 @@ class FempicArena:
                 (tpl.overlay_words == 0 or any(ln >= tpl.overlay_words for _, ln in self.ovl.free)))
 
     def admit(self, tpl: Template):
-        """Upload a request's mesh and particles into free ranges; returns its slot or None."""
+        """Copy a request's staged mesh and particles into free ranges (asynchronously, on the
+        arena's stream) and rebase its indices on the device; returns its slot or None."""
         if not self.can_admit(tpl):
             return None
+        L, s = lib(), self.stream()
         torch = self.torch
-        d, m = tpl.inst, tpl.inst["mesh"]
-        nc, nn = tpl.n_cells, tpl.n_nodes
+        h, d = tpl.host, tpl.inst
+        nc, nn, n = tpl.n_cells, tpl.n_nodes, tpl.n_part
         cb, nb = self.cells.alloc(nc), self.nodes.alloc(nn)
         ob = self.ovl.alloc(tpl.overlay_words) if tpl.overlay_words else -1
         slot = self.slots_free.pop()
-        c2c = np.where(m["c2c"] >= 0, m["c2c"] + cb, m["c2c"]).astype(np.int32)
-        self.det[16 * cb:16 * (cb + nc)] = self._t(m["c_det"].reshape(-1), torch.float64)
-        self.aux[4 * cb:4 * (cb + nc)] = self._t(_aux_bytes(m["c_vol"], c2c), torch.float64)
-        self.c2n[4 * cb:4 * (cb + nc)] = self._t((m["c2n"] + nb).reshape(-1), torch.int32)
-        self.ef[3 * cb:3 * (cb + nc)] = self._t(d["ef"].reshape(-1), torch.float64)
-        self.cell_inst[cb:cb + nc] = slot
-        self.node_start[nb:nb + nn] = self._t(tpl.csr_start[:-1] + 4 * cb, torch.int32)
-        self.node_end[nb:nb + nn] = self._t(tpl.csr_start[1:] + 4 * cb, torch.int32)
-        self.node_ref[4 * cb:4 * (cb + nc)] = self._t(tpl.csr_ref + 4 * cb, torch.int32)
+        cp = lambda dst, src: dst.copy_(src, non_blocking=True)
+        cp(self.det[16 * cb:16 * (cb + nc)], h["det"])
+        cp(self.aux[4 * cb:4 * (cb + nc)], h["aux"])
+        cp(self.c2n[4 * cb:4 * (cb + nc)], h["c2n"])
+        cp(self.ef[3 * cb:3 * (cb + nc)], h["ef"])
+        cp(self.node_start[nb:nb + nn], h["nstart"])
+        cp(self.node_end[nb:nb + nn], h["nend"])
+        cp(self.node_ref[4 * cb:4 * (cb + nc)], h["nref"])
+        i4 = lambda t, off: t.data_ptr() + 4 * off
+        check(L.oppx_offset_indices(i4(self.aux, 8 * cb), nc, 8, 2, 4, cb, s), "rebase c2c")  # aux: c2c at ints 2..5
+        check(L.oppx_offset_indices(i4(self.c2n, 4 * cb), 4 * nc, 1, 0, 1, nb, s), "rebase c2n")
+        check(L.oppx_offset_indices(i4(self.node_start, nb), nn, 1, 0, 1, 4 * cb, s), "rebase rows")
+        check(L.oppx_offset_indices(i4(self.node_end, nb), nn, 1, 0, 1, 4 * cb, s), "rebase rows")
+        check(L.oppx_offset_indices(i4(self.node_ref, 4 * cb), 4 * nc, 1, 0, 1, 4 * cb, s), "rebase refs")
+        pat = np.array([slot], np.int32)
+        check(L.oppx_fill_pattern(i4(self.cell_inst, cb), nc, 1, 4, pat.ctypes.data, 0, s), "cell_inst")
         rec = BatchInst()
         rec.cell_base, rec.n_cells, rec.node_base, rec.n_nodes = cb, nc, nb, nn
         if ob >= 0:
-            ov = d["overlay"]
-            self.overlay[ob:ob + ov.size] = self._t(np.where(ov == MAX_CELL_INDEX, ov, ov + cb), torch.int32)
+            cp(self.overlay[ob:ob + tpl.overlay_words], h["overlay"])
+            check(L.oppx_offset_indices(i4(self.overlay, ob), tpl.overlay_words, 1, 0, 1, cb, s), "rebase overlay")
             dims = oppx.dh_dims(d["bmin"], d["bmax"], d["gs"])
             rec.dh_cell = self.overlay.data_ptr() + 4 * ob
             for a in range(3):
@@ -189,17 +213,25 @@ class FempicArena:
         rec.dt = float(d["dt"])
         rec.k = float(d["charge"]) / float(d["mass"]) * float(d["dt"])
         sz = ctypes.sizeof(BatchInst)
-        self.inst_table[slot * sz:(slot + 1) * sz] = self._t(np.frombuffer(bytes(rec), np.uint8), torch.uint8)
-        n0, n = self.size, tpl.n_part
+        self.h_table[slot * sz:(slot + 1) * sz] = torch.frombuffer(bytearray(bytes(rec)), dtype=torch.uint8)
+        self.inst_table[slot * sz:(slot + 1) * sz].copy_(self.h_table[slot * sz:(slot + 1) * sz], non_blocking=True)
+        n0 = self.size
         if n:
-            self.pos[:, n0:n0 + n] = self._t(d["pos"].T, torch.float64)
-            self.vel[:, n0:n0 + n] = self._t(d["vel"].T, torch.float64)
-            self.lc[:, n0:n0 + n] = 0.0
-            self.p2c[n0:n0 + n] = self._t(d["p2c"] + cb, torch.int32)
+            cp(self.pos[:, n0:n0 + n], h["pos"])
+            cp(self.vel[:, n0:n0 + n], h["vel"])
+            cp(self.p2c[n0:n0 + n], h["p2c"])
+            check(L.oppx_offset_indices(i4(self.p2c, n0), n, 1, 0, 1, cb, s), "rebase p2c")
+            zero = np.zeros(1, np.float64)
+            check(L.oppx_fill_pattern(self.lc.data_ptr() + 8 * n0, n, 4, 8, zero.ctypes.data, self.cap, s), "lc")
         self.size = n0 + n
-        self.counters[0] = self.size
         self.res[slot] = (tpl, cb, nb, ob)
         return slot
+
+    def publish_size(self):
+        """Make the device live count match the host's after admissions (one small asynchronous copy;
+        the pinned value is rewritten only after the step's closing synchronisation)."""
+        self.h_size[0] = self.size
+        self.counters[0:1].copy_(self.h_size, non_blocking=True)
 
     def step(self):
         """One batched step of every resident request: push/move, Sort_All finalize, FP32 deposit."""
@@ -210,6 +242,7 @@ class FempicArena:
         parts = FempicParts(self.cap, self.pos.data_ptr(), self.vel.data_ptr(), self.lc.data_ptr(),
                             self.p2c.data_ptr())
         n_insts = max(self.res) + 1 if self.res else 1
+        self.publish_size()
         check(L.oppx_fempic_step_begin(self.counters.data_ptr(), None, 0, s), "step_begin")
         if self.size:
             check(L.oppx_fempic_batch_step_fused(self.size, ctypes.byref(parts), ctypes.byref(mesh),
@@ -228,7 +261,8 @@ class FempicArena:
         check(L.oppx_fempic_deposit_sorted_f32(self.size, self.cap, self.counters.data_ptr(), self.p2c.data_ptr(),
                                                self.lc.data_ptr(), n_cells, self.node_start.data_ptr(),
                                                self.node_end.data_ptr(), self.node_ref.data_ptr(), n_nodes,
-                                               self.cell_partials.data_ptr(), None, self.charge_f32.data_ptr(), s),
+                                               self.cell_partials.data_ptr(), None, self.charge_f32.data_ptr(),
+                                               self.dep_scratch.data_ptr(), self.dep_bytes, s),
               "deposit_f32")
         self.size = int(self.counters[0].item())  # the live count after the finalize (host sync)
 
@@ -249,27 +283,38 @@ class FempicArena:
         check(L.oppx_lower_bound(self.size, self.counters.data_ptr(), self.p2c.data_ptr(), k.data_ptr(), len(keys),
                                  bounds.data_ptr(), s), "lower_bound")
         b = bounds.cpu().numpy()
-        outs = []
+        staged = []
         pat = np.array([MAX_CELL_INDEX], np.int32)
         for j, slot in enumerate(slots):
             tpl, cb, nb, ob = self.res.pop(slot)
             lo, hi = int(b[2 * j]), int(b[2 * j + 1])
-            outs.append(dict(n=hi - lo, pos=self.pos[:, lo:hi].T.cpu().numpy(), vel=self.vel[:, lo:hi].T.cpu().numpy(),
-                             lc=self.lc[:, lo:hi].T.cpu().numpy(), p2c=self.p2c[lo:hi].cpu().numpy() - cb,
-                             charge_f32=self.charge_f32[nb:nb + tpl.n_nodes].cpu().numpy()))
-            if hi > lo:  # leaves the particle set at the next sort
-                check(L.oppx_fill_pattern(self.p2c.data_ptr() + 4 * lo, hi - lo, 1, 4, pat.ctypes.data, 0, s),
+            k = hi - lo
+            pin = lambda shape, dt: torch.empty(shape, dtype=dt, pin_memory=True)
+            out = dict(pos=pin((3, k), torch.float64), vel=pin((3, k), torch.float64), lc=pin((4, k), torch.float64),
+                       p2c=pin((k,), torch.int32), charge_f32=pin((tpl.n_nodes,), torch.float32))
+            out["pos"].copy_(self.pos[:, lo:hi], non_blocking=True)
+            out["vel"].copy_(self.vel[:, lo:hi], non_blocking=True)
+            out["lc"].copy_(self.lc[:, lo:hi], non_blocking=True)
+            out["p2c"].copy_(self.p2c[lo:hi], non_blocking=True)
+            out["charge_f32"].copy_(self.charge_f32[nb:nb + tpl.n_nodes], non_blocking=True)
+            staged.append((out, k, cb))
+            if k:  # leaves the particle set at the next sort
+                check(L.oppx_fill_pattern(self.p2c.data_ptr() + 4 * lo, k, 1, 4, pat.ctypes.data, 0, s),
                       "fill_pattern")
-            self.node_start[nb:nb + tpl.n_nodes] = 0  # empty node rows
-            self.node_end[nb:nb + tpl.n_nodes] = 0
+            zero = np.zeros(1, np.int32)
+            check(L.oppx_fill_pattern(self.node_start.data_ptr() + 4 * nb, tpl.n_nodes, 1, 4, zero.ctypes.data, 0, s),
+                  "rows")  # empty node rows
+            check(L.oppx_fill_pattern(self.node_end.data_ptr() + 4 * nb, tpl.n_nodes, 1, 4, zero.ctypes.data, 0, s),
+                  "rows")
             self.cells.release(cb)
             self.nodes.release(nb)
             if ob >= 0:
                 self.ovl.release(ob)
             self.slots_free.append(slot)
-        return outs
+        torch.cuda.current_stream(self.dev).synchronize()
+        return [dict(n=k, pos=o["pos"].numpy().T, vel=o["vel"].numpy().T, lc=o["lc"].numpy().T,
+                     p2c=o["p2c"].numpy() - cb, charge_f32=o["charge_f32"].numpy()) for o, k, cb in staged]
 
-    @staticmethod
     @staticmethod
     def for_budget(arena_bytes: float, device="cuda", max_insts: int = 4096) -> "FempicArena":
         """An arena of about `arena_bytes`: a quarter for mesh records (about 300 B per tet with its
@@ -301,8 +346,11 @@ def serve(arena: FempicArena, requests, budget_bytes: float, keep_results: bool 
     pending, active, done = deque(requests), {}, []
     resident = 0.0
     peak_resident, iters, part_steps = 0.0, 0, 0
+    dropping = 0  # particles of retired requests still in the set until the next sort
+    spent = dict(admit_s=0.0, step_s=0.0, retire_s=0.0)
     while pending or active:
         now = clock() - t0
+        ta = clock()
         while pending and pending[0].arrival <= now:
             r = pending[0]
             need = r.tpl.device_bytes()
@@ -314,28 +362,35 @@ def serve(arena: FempicArena, requests, budget_bytes: float, keep_results: bool 
             peak_resident = max(peak_resident, resident)
             active[r.slot] = r
             pending.popleft()
+        spent["admit_s"] += clock() - ta
         if not active:
             if pending:
                 time.sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.01)))
             continue
-        part_steps += arena.size
+        part_steps += arena.size - dropping
+        ts = clock()
         arena.step()
+        spent["step_s"] += clock() - ts
+        dropping = 0
         iters += 1
         finished = [k for k, r in active.items() if r.left == 1]
+        tr = clock()
         for slot, res in zip(finished, arena.retire(finished)):
             r = active.pop(slot)
             r.t_done = clock() - t0
             if keep_results:
                 r.result = res
+            dropping += res["n"]
             resident -= r.tpl.device_bytes()
             done.append(r)
+        spent["retire_s"] += clock() - tr
         for r in active.values():
             r.left -= 1
     wall = clock() - t0
     lat = np.array([r.t_done - r.arrival for r in done]) * 1e3
     return done, dict(wall_s=wall, iterations=iters, particle_steps=part_steps, value=part_steps / wall,
                       p50_latency_ms=float(np.percentile(lat, 50)), p99_latency_ms=float(np.percentile(lat, 99)),
-                      peak_resident_bytes=peak_resident, requests=len(done))
+                      peak_resident_bytes=peak_resident, requests=len(done), **spent)
 
 
 def serve_plain(requests, keep_results: bool = False):

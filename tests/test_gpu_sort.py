@@ -117,3 +117,39 @@ def test_gpu_f32_compensated_deposit_vs_fp64_oracle(nx, npart, seed):
         assert np.array_equal(again, f32.astype(np.float32))
         # and the FP64 atomic deposit of the same step agrees with the oracle to 1e-12
         assert np.abs(gpu.charge.cpu().numpy() - rho64).max() <= 1e-12 * np.abs(rho64).max()
+
+
+@pytest.mark.parametrize("every", [1, 2])
+def test_gpu_lazy_sort_steps_equal_eager_sort_and_oracle(every):
+    # Sort_Periodic with the record gather left pending and applied by the next step's move gives the
+    # same state, bit for bit, as the eager sort; both follow the oracle's Sort_All steps
+    nx, npart, seed = 6, 30000, 21
+    m = oppx.box_mesh(nx, nx, nx)
+    ef = oppx.box_field_sines(m)
+    pos, vel, p2c = oppx.box_particles(npart, seed, 0.02, m)
+    gs = 1.0 / 12
+    ov = oppx.dh_overlay(m, (0, 0, 0), (1, 1, 1), gs)
+    kw = dict(overlay=ov, bmin=(0, 0, 0), bmax=(1, 1, 1), gs=gs)
+    q, ms, dt = 1e-3, 1.0, 0.5 / nx / (0.02 * math.sqrt(3.0))
+    eager = oppx.FempicDevice(m, ef, pos, vel, p2c, **kw)
+    lazy = oppx.FempicDevice(m, ef, pos, vel, p2c, **kw)
+    orc = oracle.FempicState(m, ef, pos, vel, p2c, overlay=ov, bmin=(0, 0, 0), bmax=(1, 1, 1), gs=gs)
+    for s in range(6):
+        sort_now = (s + 1) % every == 0
+        for g, lz in ((eager, False), (lazy, True)):
+            g.step_begin()
+            g.pass_fused(q, ms, dt)
+            if sort_now:
+                g.sort(drop_removed=True, lazy=lz)
+            else:
+                g.holefill()
+        if sort_now:
+            orc.step_sorted(q, ms, dt)
+        else:
+            orc.step(q, ms, dt)
+        a, b, e = eager.download(), lazy.download(), orc.live()
+        assert a["n"] == b["n"] == e["n"], f"step {s}"
+        for key in ("p2c", "pos", "vel", "lc"):
+            assert np.array_equal(a[key], b[key]), f"step {s} {key}"
+            assert np.array_equal(a[key], e[key]), f"step {s} {key} vs oracle"
+        assert np.abs(a["charge"] - e["charge"]).max() <= 1e-12 * np.abs(e["charge"]).max()
