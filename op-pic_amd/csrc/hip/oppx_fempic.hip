@@ -370,6 +370,9 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
     const double* __restrict__ spos = GATHER ? P.src_pos : P.pos;
     const double* __restrict__ svel = GATHER ? P.src_vel : P.vel;
     const int32_t* __restrict__ sp2c = GATHER ? P.src_p2c : P.p2c;
+    // streamed particle loads are non-temporal; loads through a permutation are ordinary cached
+    // loads, so the neighbours of a gathered word still find its line in L2
+    auto ld = [](const auto* p) { return GATHER ? *p : ld_stream(p); };
     int64_t src = i;
     bool fault = false;
     if (GATHER && live) {
@@ -380,7 +383,7 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
         }
     }
 
-    int cell = (live && !fault) ? ld_stream(sp2c + src) : kMaxCell;
+    int cell = (live && !fault) ? ld(sp2c + src) : kMaxCell;
     // a particle already holding MAX_CELL_INDEX is skipped, as the reference loops do
     // (apps_codegen/app_fempic_cg/seq/move_kernel_loop.hpp:92-94)
     const bool skip = live && !fault && cell == kMaxCell;
@@ -409,12 +412,12 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
             }
         }
         if (!fault) {
-            double x = ld_stream(spos + src), y = ld_stream(spos + S + src), z = ld_stream(spos + 2 * S + src);
+            double x = ld(spos + src), y = ld(spos + S + src), z = ld(spos + 2 * S + src);
             if (PUSH) {
                 // calculate_new_pos_vel_kernel: v = fma(k, E, v); x = fma(v, dt, x)
                 const double* e = P.ef + 3 * (int64_t)cell;
                 const double ex = e[0], ey = e[1], ez = e[2];
-                double vx = ld_stream(svel + src), vy = ld_stream(svel + S + src), vz = ld_stream(svel + 2 * S + src);
+                double vx = ld(svel + src), vy = ld(svel + S + src), vz = ld(svel + 2 * S + src);
                 vx = fma(kk, ex, vx);
                 x = fma(vx, dt, x);
                 vy = fma(kk, ey, vy);
@@ -455,13 +458,13 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
         // (a pending particle's lc is final only after the worklist walk, which overwrites this)
         if (!pushed) {
             for (int d = 0; d < 3; d++) {
-                st_stream(P.pos + d * S + i, ld_stream(spos + d * S + src));
-                st_stream(P.vel + d * S + i, ld_stream(svel + d * S + src));
+                st_stream(P.pos + d * S + i, spos[d * S + src]);
+                st_stream(P.vel + d * S + i, svel[d * S + src]);
             }
         }
-        if (!p2c_done) st_stream(P.p2c + i, ld_stream(sp2c + src));
+        if (!p2c_done) st_stream(P.p2c + i, sp2c[src]);
         if (!lc_done)
-            for (int d = 0; d < 4; d++) st_stream(P.lc + d * S + i, ld_stream(P.src_lc + d * S + src));
+            for (int d = 0; d < 4; d++) st_stream(P.lc + d * S + i, P.src_lc[d * S + src]);
         if (P.hops && !(live && !skip && !fault)) P.hops[i] = h;
     }
 
