@@ -267,7 +267,9 @@ def bench_ensemble(args):
 
 def bench_neso(args):
     """NESO-advection pure move: 1024^2 periodic quads, 64M particles (seed 3) at uniform positions,
-    velocities uniform in [-v, v]^2 with v = 4 cells per step, FP64 update + multi-hop move."""
+    velocities uniform in [-v, v]^2 with v = 4 cells per step, FP64 update + multi-hop move. The mesh
+    is verified to be the periodic grid, so the walk computes its records; the gathered-record walk is
+    timed beside it."""
     import torch
 
     import oppx
@@ -277,24 +279,39 @@ def bench_neso(args):
     m = oppx.neso_mesh(nx, nx, cw)
     pos, vel, p2c = oppx.neso_particles(args.neso_particles, 3, m, 4.0 * cw / dt)
     log(f"neso inputs in {time.time() - t0:.1f}s")
-    g = oppx.NesoDevice(m, pos, vel, p2c)
-    del pos, vel, p2c
-    for _ in range(args.warmup):
-        g.step(dt)
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    t1 = time.perf_counter()
-    for a, b in evs:
-        a.record()
-        g.step(dt)
-        b.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t1
-    ms = np.array([a.elapsed_time(b) for a, b in evs])
+    every = args.neso_sort_every
     n = args.neso_particles
+
+    def run(grid, steps, warmup):
+        g = oppx.NesoDevice(m, pos, vel, p2c, grid=grid)
+        if every:
+            g.sort()  # input layout: particles ordered by cell, as the FEM-PIC inputs are
+        for _ in range(warmup):
+            g.step(dt)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t1 = time.perf_counter()
+        for s, (a, b) in enumerate(evs):
+            a.record()
+            g.step(dt)
+            if every and (s + 1) % every == 0:
+                g.sort()  # Sort_Periodic: part of the timed step
+            b.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
+        ms = np.array([a.elapsed_time(b) for a, b in evs])
+        c = g.read_counters()
+        log(f"neso {'grid' if g.grid else 'gathered'} records: {1e3 * wall / steps:.3f} ms/step, "
+            f"p50 {np.percentile(ms, 50):.3f} ms, hops last step {c['total_hops'] / n:.2f}/particle")
+        used_grid = g.grid
+        del g
+        torch.cuda.empty_cache()
+        return wall, ms, c, used_grid
+
+    wall, ms, c, used_grid = run(True, args.steps, args.warmup)
+    g_wall, g_ms, _, _ = run(False, min(args.steps, 16), 2)
     alg = 64.0 * n + 32.0 * nx * nx  # R pos 16, vel 16, mdir 8, p2c 4; W pos 16, mdir 8, p2c 4 (B/particle)
     achieved = alg / (ms.mean() * 1e-3) / 1e9
-    c = g.read_counters()
     return {
         "metric": "NESO-advection particle moves/s (update + multi-hop move)",
         "value": n * args.steps / wall,
@@ -304,12 +321,18 @@ def bench_neso(args):
         "warmup": args.warmup,
         "ms_per_step": 1e3 * wall / args.steps,
         "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"NESO advection, {nx}x{nx} periodic quads, {n} particles seed 3, 4 cells/step max"},
+        "config": {"workload": f"NESO advection, {nx}x{nx} periodic quads, {n} particles seed 3, 4 cells/step max",
+                   "sort_every": every, "records": "grid" if used_grid else "gathered"},
         "p50_step_ms": float(np.percentile(ms, 50)),
         "p99_step_ms": float(np.percentile(ms, 99)),
         "max_hops_last_step": c["max_hops"],
+        "hops_per_particle_last_step": c["total_hops"] / n,
+        "gathered_records": {"ms_per_step": 1e3 * g_wall / len(g_ms), "value": n * len(g_ms) / g_wall,
+                             "p50_step_ms": float(np.percentile(g_ms, 50))},
         "roofline": {"kernel": "neso_step_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
     }
@@ -389,6 +412,7 @@ def main():
                     help="direct-hop overlay spacing 1/overlay-cells (dims overlay-cells + 1 per axis)")
     ap.add_argument("--mesh-order", choices=["x", "morton"], default="morton",
                     help="numbering of the box mesh's cubes and nodes (x fastest, or Morton order)")
+    ap.add_argument("--tuning", default="", help="launch-shape overrides, e.g. inline_hops=3,first_block=1024")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tag", default="c2", help="workload tag of the profile summaries")
@@ -397,6 +421,8 @@ def main():
                          "serve: the serving stream; neso: the NESO-advection pure move")
     ap.add_argument("--neso-nx", type=int, default=1024)
     ap.add_argument("--neso-particles", type=int, default=64_000_000)
+    ap.add_argument("--neso-sort-every", type=int, default=0,
+                    help="NESO: sort particles by cell at set-up and every K steps inside the timed loop (0: never)")
     ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-pool", type=int, default=48)
@@ -435,6 +461,9 @@ def main():
     m, ef, pos, vel, p2c, ov, bmin, bmax, gs = build_inputs(args.nx, args.particles, args.seed, sigma, args.order,
                                                             args.mesh_order, args.overlay_cells)
 
+    if args.tuning:
+        over = {k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))}
+        oppx.set_tuning(**dict(oppx.get_tuning(), **over))
     t0 = time.time()
     gpu = oppx.FempicDevice(m, ef, pos, vel, p2c, overlay=ov, bmin=bmin, bmax=bmax, gs=gs)
     torch.cuda.synchronize()

@@ -322,6 +322,20 @@ int oppx_neso_step(int64_t n_max, const oppx_neso_parts* parts, const oppx_quad_
                    double dt, double cell_width, double ext_x, double ext_y, int flags,
                    oppx_move_counters* counters, int32_t* hops, void* stream);
 
+/* *mismatches (device) = the number of quad records that differ, bit for bit, from those of the
+ * nx x ny periodic grid of apps/app_neso_advection/advec_misc.h:115-143 (corner (x, y) * cell_width,
+ * neighbours wrapped on both axes; cell = x + nx * y). Zero lets oppx_neso_step_grid replace the
+ * record gathers. */
+int oppx_quad_grid_check(const oppx_quad_cell* cells, int64_t n_cells, int nx, int ny, double cell_width,
+                         int32_t* mismatches, void* stream);
+
+/* oppx_neso_step on a mesh that oppx_quad_grid_check found to be the nx x ny periodic grid: the walk
+ * visits the same cells with the same hop counts, each visited record computed in registers instead
+ * of gathered. */
+int oppx_neso_step_grid(int64_t n_max, const oppx_neso_parts* parts, int nx, int ny, double dt, double cell_width,
+                        double ext_x, double ext_y, int flags, oppx_move_counters* counters, int32_t* hops,
+                        void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * Compaction
  * ------------------------------------------------------------------------------------- */

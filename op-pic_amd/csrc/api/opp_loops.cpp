@@ -26,6 +26,7 @@ struct QuadCache {
     const void* c2c_d = nullptr;
     int64_t n = 0;
     oppx_quad_cell* cells = nullptr;
+    int gnx = 0, gny = 0;  // > 0: the mesh is, bit for bit, this periodic grid
 };
 static TetAuxCache g_tet;
 static QuadCache g_quad;
@@ -265,6 +266,22 @@ static const oppx_quad_cell* quad_cells(opp_dat c_pos_ll, opp_map c2c) {
         g_quad.ll_d = c_pos_ll->data_d;
         g_quad.c2c_d = c2c->map_d;
         g_quad.n = n;
+        g_quad.gnx = g_quad.gny = 0;
+        // NESO's grid numbers cell (x, y) as x + nx * y, so cell 0's x-1 neighbour is nx - 1
+        // (apps/app_neso_advection/advec_misc.h:128-140); the device check then compares every record
+        const int nx = (c2c->map && n > 0) ? c2c->map[0] + 1 : 0;
+        if (nx > 0 && n % nx == 0) {
+            const double cw = const_real("CONST_cell_width");
+            int32_t* bad_d = (int32_t*)dev_alloc(sizeof(int32_t));
+            int32_t bad = -1;
+            check(oppx_quad_grid_check(g_quad.cells, n, nx, (int)(n / nx), cw, bad_d, nullptr), "quad grid check");
+            check(oppx_memcpy_dtoh(&bad, bad_d, sizeof(int32_t), nullptr), "quad grid check");
+            dev_free(bad_d);
+            if (bad == 0) {
+                g_quad.gnx = nx;
+                g_quad.gny = (int)(n / nx);
+            }
+        }
     }
     return g_quad.cells;
 }
@@ -309,9 +326,15 @@ void opp_particle_move__move_kernel(opp_set set, opp_map c2c_map, opp_map p2c_ma
     neso_consts(dt, cw, ex, ey);
     oppx_neso_parts parts{set->set_capacity, (double*)arg0.dat->data_d, nullptr, (int32_t*)arg1.dat->data_d,
                           (int32_t*)p2c_map->p2c_dat->data_d};
-    check(oppx_neso_step(set->size, &parts, quad_cells(arg2.dat, c2c_map), c2c_map->from->size, dt, cw, ex, ey,
-                         OPPX_NESO_MOVE, pdev(set).counters, nullptr, nullptr),
-          "move_kernel");
+    const oppx_quad_cell* cells = quad_cells(arg2.dat, c2c_map);
+    if (g_quad.gnx > 0)
+        check(oppx_neso_step_grid(set->size, &parts, g_quad.gnx, g_quad.gny, dt, cw, ex, ey, OPPX_NESO_MOVE,
+                                  pdev(set).counters, nullptr, nullptr),
+              "move_kernel");
+    else
+        check(oppx_neso_step(set->size, &parts, cells, c2c_map->from->size, dt, cw, ex, ey, OPPX_NESO_MOVE,
+                             pdev(set).counters, nullptr, nullptr),
+              "move_kernel");
     arg1.dat->dirty_hd = Dirty::Host;
     p2c_map->p2c_dat->dirty_hd = Dirty::Host;
     oppx_move_counters c;

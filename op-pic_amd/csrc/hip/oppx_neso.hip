@@ -26,7 +26,13 @@ struct NesoArgs {
     int64_t n_max;
     oppx_move_counters* ctr;
     int32_t* __restrict__ hops;
+    int gnx, gny;  // grid mode: the mesh is the nx x ny periodic grid, its records computed in registers
+    double inv_gnx;
 };
+
+// Record of quad (ix, iy) of the periodic grid, as apps/app_neso_advection/advec_misc.h:115-143 builds
+// it: lower-left corner (x * cell_width, y * cell_width), neighbours wrapped on both axes.
+__device__ __forceinline__ int grid_wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 __device__ __forceinline__ double periodic_update(double p, double v, double dt, double ext) {
     p = fma(v, dt, p);
@@ -35,67 +41,134 @@ __device__ __forceinline__ double periodic_update(double p, double v, double dt,
     return fma(-ext, trunc(t / ext), t);
 }
 
-// Resident grid, grid-stride over the particles: the per-block statistics then reach the counters
-// from a few thousand blocks instead of one block per 256 particles (same-address atomics serialise).
+// Particle state as one grid-stride iteration loads it.
+struct NesoPart {
+    double x, y, vx, vy;
+    int cell, mx, my;
+};
+
+__device__ __forceinline__ NesoPart neso_load(const NesoArgs& P, int flags, int64_t i) {
+    const int64_t S = P.stride;
+    NesoPart q;
+    q.x = __builtin_nontemporal_load(P.pos + i);
+    q.y = __builtin_nontemporal_load(P.pos + S + i);
+    q.vx = q.vy = 0.0;
+    if (flags & OPPX_NESO_UPDATE) {
+        q.vx = __builtin_nontemporal_load(P.vel + i);
+        q.vy = __builtin_nontemporal_load(P.vel + S + i);
+    }
+    q.cell = kMaxCell;
+    q.mx = q.my = 0;
+    if (flags & OPPX_NESO_MOVE) {
+        q.cell = __builtin_nontemporal_load(P.p2c + i);
+        q.mx = __builtin_nontemporal_load(P.mdir + i);
+        q.my = __builtin_nontemporal_load(P.mdir + S + i);
+    }
+    return q;
+}
+
+// update_pos_kernel then move_kernel for particle i; returns the evaluations made.
+template <bool GRID>
+__device__ __forceinline__ int neso_particle(const NesoArgs& P, int flags, int64_t i, NesoPart q, bool& fault) {
+    const int64_t S = P.stride;
+    int h = 0;
+    double x = q.x, y = q.y;
+    if (flags & OPPX_NESO_UPDATE) {
+        x = periodic_update(x, q.vx, P.dt, P.ext_x);
+        y = periodic_update(y, q.vy, P.dt, P.ext_y);
+        __builtin_nontemporal_store(x, P.pos + i);
+        __builtin_nontemporal_store(y, P.pos + S + i);
+    }
+    int cell = q.cell;
+    if (cell == kMaxCell) return 0;
+    if (!((unsigned)cell < (unsigned)P.n_cells)) {
+        fault = true;
+        return 0;
+    }
+    int mx = q.mx, my = q.my;
+    const double cw = P.cw;
+    int ix = 0, iy = 0;
+    if (GRID) {
+        // cell = ix + nx * iy: the quotient from one FP64 product, then corrected by one step
+        // (the product's error stays below one for cells under 2^31)
+        iy = (int)((double)cell * P.inv_gnx);
+        ix = cell - iy * P.gnx;
+        if (ix < 0) { iy--; ix += P.gnx; }
+        else if (ix >= P.gnx) { iy++; ix -= P.gnx; }
+    }
+    for (;;) {
+        double llx, lly;
+        int4 nb = make_int4(0, 0, 0, 0);
+        if (GRID) {
+            llx = (double)ix * cw;
+            lly = (double)iy * cw;
+        } else {
+            const oppx_quad_cell c = P.cells[cell];
+            llx = c.llx;
+            lly = c.lly;
+            nb = make_int4(c.c2c[0], c.c2c[1], c.c2c[2], c.c2c[3]);
+        }
+        const double dx = x - llx, dy = y - lly;
+        h++;
+        if (h == 1) {
+            const bool xp = ((dx > cw) && (P.ext_x > 2 * dx - cw)) || ((dx < 0) && (P.ext_x < cw - 2 * dx));
+            const bool yp = ((dy > cw) && (P.ext_y > 2 * dy - cw)) || ((dy < 0) && (P.ext_y < cw - 2 * dy));
+            mx = xp ? 1 : -1;
+            my = yp ? 1 : -1;
+        }
+        int next = 0;
+        bool hop = false;
+        if ((dx >= 0.0) && (dx <= cw)) mx = 0;
+        else if (mx > 0) { next = nb.y; hop = true; if (GRID) ix = grid_wrap(ix + 1, P.gnx); }
+        else if (mx < 0) { next = nb.x; hop = true; if (GRID) ix = grid_wrap(ix - 1, P.gnx); }
+        if (!hop) {
+            if ((dy >= 0.0) && (dy <= cw)) my = 0;
+            else if (my > 0) { next = nb.w; hop = true; if (GRID) iy = grid_wrap(iy + 1, P.gny); }
+            else if (my < 0) { next = nb.z; hop = true; if (GRID) iy = grid_wrap(iy - 1, P.gny); }
+        }
+        if (!hop) break;  // MOVE_DONE
+        if (GRID) next = ix + P.gnx * iy;
+        if (!((unsigned)next < (unsigned)P.n_cells) || h >= OPPX_MAX_HOPS) {
+            fault = true;
+            break;
+        }
+        cell = next;
+    }
+    __builtin_nontemporal_store(mx, P.mdir + i);
+    __builtin_nontemporal_store(my, P.mdir + S + i);
+    if (!fault) __builtin_nontemporal_store(cell, P.p2c + i);
+    return h;
+}
+
+// Resident grid, grid-stride over the particles, two particles per lane per iteration (both loaded
+// before either is walked, so each lane keeps two particles' loads in flight); the per-block
+// statistics then reach the counters from a few thousand blocks instead of one block per 256
+// particles (same-address atomics serialise).
+template <bool GRID>
 __global__ void __launch_bounds__(kBlock) neso_step_kernel(NesoArgs P, int flags) {
     int64_t n = P.ctr->size;
     n = n < 0 ? 0 : (n > P.n_max ? P.n_max : n);
-    const int64_t S = P.stride;
     long long hsum = 0;
     int hmax = 0, nfault = 0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        int h = 0;
-        bool fault = false;
-        double x = __builtin_nontemporal_load(P.pos + i), y = __builtin_nontemporal_load(P.pos + S + i);
-        if (flags & OPPX_NESO_UPDATE) {
-            x = periodic_update(x, __builtin_nontemporal_load(P.vel + i), P.dt, P.ext_x);
-            y = periodic_update(y, __builtin_nontemporal_load(P.vel + S + i), P.dt, P.ext_y);
-            __builtin_nontemporal_store(x, P.pos + i);
-            __builtin_nontemporal_store(y, P.pos + S + i);
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += 2 * stride) {
+        const int64_t j = i + stride;
+        const NesoPart a = neso_load(P, flags, i);
+        NesoPart b{};
+        if (j < n) b = neso_load(P, flags, j);
+        bool fa = false, fb = false;
+        const int ha = neso_particle<GRID>(P, flags, i, a, fa);
+        if (P.hops && (flags & OPPX_NESO_MOVE)) P.hops[i] = ha;
+        hsum += ha;
+        hmax = hmax > ha ? hmax : ha;
+        nfault += fa ? 1 : 0;
+        if (j < n) {
+            const int hb = neso_particle<GRID>(P, flags, j, b, fb);
+            if (P.hops && (flags & OPPX_NESO_MOVE)) P.hops[j] = hb;
+            hsum += hb;
+            hmax = hmax > hb ? hmax : hb;
+            nfault += fb ? 1 : 0;
         }
-        int cell = (flags & OPPX_NESO_MOVE) ? __builtin_nontemporal_load(P.p2c + i) : kMaxCell;
-        if (cell != kMaxCell) {
-            if (!((unsigned)cell < (unsigned)P.n_cells)) {
-                fault = true;
-            } else {
-                int mx = __builtin_nontemporal_load(P.mdir + i), my = __builtin_nontemporal_load(P.mdir + S + i);
-                const double cw = P.cw;
-                for (;;) {
-                    const oppx_quad_cell q = P.cells[cell];
-                    const double dx = x - q.llx, dy = y - q.lly;
-                    h++;
-                    if (h == 1) {
-                        const bool xp = ((dx > cw) && (P.ext_x > 2 * dx - cw)) || ((dx < 0) && (P.ext_x < cw - 2 * dx));
-                        const bool yp = ((dy > cw) && (P.ext_y > 2 * dy - cw)) || ((dy < 0) && (P.ext_y < cw - 2 * dy));
-                        mx = xp ? 1 : -1;
-                        my = yp ? 1 : -1;
-                    }
-                    int next = 0;
-                    bool hop = false;
-                    if ((dx >= 0.0) && (dx <= cw)) mx = 0;
-                    else if (mx > 0) { next = q.c2c[1]; hop = true; }
-                    else if (mx < 0) { next = q.c2c[0]; hop = true; }
-                    if (!hop) {
-                        if ((dy >= 0.0) && (dy <= cw)) my = 0;
-                        else if (my > 0) { next = q.c2c[3]; hop = true; }
-                        else if (my < 0) { next = q.c2c[2]; hop = true; }
-                    }
-                    if (!hop) break;  // MOVE_DONE
-                    if (!((unsigned)next < (unsigned)P.n_cells) || h >= OPPX_MAX_HOPS) {
-                        fault = true;
-                        break;
-                    }
-                    cell = next;
-                }
-                __builtin_nontemporal_store(mx, P.mdir + i);
-                __builtin_nontemporal_store(my, P.mdir + S + i);
-                if (!fault) __builtin_nontemporal_store(cell, P.p2c + i);
-            }
-        }
-        if (P.hops && (flags & OPPX_NESO_MOVE)) P.hops[i] = h;
-        hsum += h;
-        hmax = hmax > h ? hmax : h;
-        nfault += fault ? 1 : 0;
     }
     __shared__ long long s_sum[kBlock / kWave];
     __shared__ int s_max[kBlock / kWave], s_flt[kBlock / kWave];
@@ -112,6 +185,23 @@ __global__ void __launch_bounds__(kBlock) neso_step_kernel(NesoArgs P, int flags
         if (m) atomicMax(&P.ctr->max_hops, m);
         if (f) atomicAdd(&P.ctr->faults, f);
     }
+}
+
+// Counts the records that differ, bit for bit, from the nx x ny periodic grid's.
+__global__ void quad_grid_check_kernel(const oppx_quad_cell* __restrict__ cells, int64_t n, int nx, int ny, double cw,
+                                       int32_t* __restrict__ mismatches) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (c < n) {
+        const int ix = (int)(c % nx), iy = (int)(c / nx);
+        const oppx_quad_cell q = cells[c];
+        bad = __double_as_longlong(q.llx) != __double_as_longlong((double)ix * cw) ||
+              __double_as_longlong(q.lly) != __double_as_longlong((double)iy * cw) ||
+              q.c2c[0] != grid_wrap(ix - 1, nx) + nx * iy || q.c2c[1] != grid_wrap(ix + 1, nx) + nx * iy ||
+              q.c2c[2] != ix + nx * grid_wrap(iy - 1, ny) || q.c2c[3] != ix + nx * grid_wrap(iy + 1, ny);
+    }
+    const unsigned long long m = __ballot(bad);
+    if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(mismatches, (int)__popcll(m));
 }
 
 __global__ void neso_reset_kernel(oppx_move_counters* ctr) {
@@ -135,27 +225,16 @@ __global__ void quad_cells_kernel(const double* __restrict__ ll, const int32_t* 
     out[c] = q;
 }
 
-}  // namespace
-}  // namespace oppx
+__global__ void zero_i32_kernel(int32_t* p) { *p = 0; }
 
-using namespace oppx;
-
-extern "C" {
-
-int oppx_quad_cells_build(const double* c_pos_ll, const int32_t* c2c, int64_t n_cells, oppx_quad_cell* cells,
-                          void* stream) {
-    OPPX_ARG_CHECK(c_pos_ll && c2c && cells && n_cells > 0 && n_cells < kMaxCell, "oppx_quad_cells_build: bad args");
-    quad_cells_kernel<<<grid_for(n_cells, 256), 256, 0, as_stream(stream)>>>(c_pos_ll, c2c, n_cells, cells);
-    OPPX_LAUNCH_CHECK("quad_cells");
-    return OPPX_OK;
-}
-
-int oppx_neso_step(int64_t n_max, const oppx_neso_parts* parts, const oppx_quad_cell* cells, int64_t n_cells,
-                   double dt, double cell_width, double ext_x, double ext_y, int flags,
+int neso_step_impl(int64_t n_max, const oppx_neso_parts* parts, const oppx_quad_cell* cells, int64_t n_cells,
+                   int gnx, int gny, double dt, double cell_width, double ext_x, double ext_y, int flags,
                    oppx_move_counters* counters, int32_t* hops, void* stream) {
+    const bool grid = gnx > 0;
     OPPX_ARG_CHECK(parts && counters, "oppx_neso_step: null pointer");
     OPPX_ARG_CHECK(flags >= 1 && flags <= 3, "oppx_neso_step: flags must be OPPX_NESO_UPDATE and/or OPPX_NESO_MOVE");
-    OPPX_ARG_CHECK(!(flags & OPPX_NESO_MOVE) || (cells && parts->mdir && parts->p2c), "oppx_neso_step: move needs cells, mdir, p2c");
+    OPPX_ARG_CHECK(!(flags & OPPX_NESO_MOVE) || ((cells || grid) && parts->mdir && parts->p2c),
+                   "oppx_neso_step: move needs cells, mdir, p2c");
     OPPX_ARG_CHECK(!(flags & OPPX_NESO_UPDATE) || (parts->pos && parts->vel), "oppx_neso_step: update needs pos, vel");
     OPPX_ARG_CHECK(n_cells > 0 && n_cells < kMaxCell, "oppx_neso_step: n_cells out of range");
     OPPX_ARG_CHECK(n_max >= 0 && n_max <= parts->stride, "oppx_neso_step: n_max beyond capacity");
@@ -179,12 +258,57 @@ int oppx_neso_step(int64_t n_max, const oppx_neso_parts* parts, const oppx_quad_
     P.n_max = n_max;
     P.ctr = counters;
     P.hops = hops;
-    unsigned g = resident_blocks(reinterpret_cast<const void*>(neso_step_kernel), kBlock);
+    P.gnx = gnx;
+    P.gny = gny;
+    P.inv_gnx = grid ? 1.0 / gnx : 0.0;
+    auto kern = grid ? neso_step_kernel<true> : neso_step_kernel<false>;
+    unsigned g = resident_blocks(reinterpret_cast<const void*>(kern), kBlock);
     const unsigned need = grid_for(n_max, kBlock);
     if (g > need) g = need;
-    neso_step_kernel<<<g, kBlock, 0, s>>>(P, flags);
+    kern<<<g, kBlock, 0, s>>>(P, flags);
     OPPX_LAUNCH_CHECK("neso_step");
     return OPPX_OK;
+}
+}  // namespace
+}  // namespace oppx
+
+using namespace oppx;
+
+extern "C" {
+
+int oppx_quad_cells_build(const double* c_pos_ll, const int32_t* c2c, int64_t n_cells, oppx_quad_cell* cells,
+                          void* stream) {
+    OPPX_ARG_CHECK(c_pos_ll && c2c && cells && n_cells > 0 && n_cells < kMaxCell, "oppx_quad_cells_build: bad args");
+    quad_cells_kernel<<<grid_for(n_cells, 256), 256, 0, as_stream(stream)>>>(c_pos_ll, c2c, n_cells, cells);
+    OPPX_LAUNCH_CHECK("quad_cells");
+    return OPPX_OK;
+}
+
+int oppx_quad_grid_check(const oppx_quad_cell* cells, int64_t n_cells, int nx, int ny, double cell_width,
+                         int32_t* mismatches, void* stream) {
+    OPPX_ARG_CHECK(cells && mismatches && nx > 0 && ny > 0 && (int64_t)nx * ny == n_cells && n_cells < kMaxCell,
+                   "oppx_quad_grid_check: bad arguments (n_cells must be nx * ny)");
+    hipStream_t s = as_stream(stream);
+    zero_i32_kernel<<<1, 1, 0, s>>>(mismatches);
+    quad_grid_check_kernel<<<grid_for(n_cells, 256), 256, 0, s>>>(cells, n_cells, nx, ny, cell_width, mismatches);
+    OPPX_LAUNCH_CHECK("quad_grid_check");
+    return OPPX_OK;
+}
+
+
+int oppx_neso_step(int64_t n_max, const oppx_neso_parts* parts, const oppx_quad_cell* cells, int64_t n_cells,
+                   double dt, double cell_width, double ext_x, double ext_y, int flags,
+                   oppx_move_counters* counters, int32_t* hops, void* stream) {
+    return neso_step_impl(n_max, parts, cells, n_cells, 0, 0, dt, cell_width, ext_x, ext_y, flags, counters, hops,
+                          stream);
+}
+
+int oppx_neso_step_grid(int64_t n_max, const oppx_neso_parts* parts, int nx, int ny, double dt, double cell_width,
+                        double ext_x, double ext_y, int flags, oppx_move_counters* counters, int32_t* hops,
+                        void* stream) {
+    OPPX_ARG_CHECK(nx > 0 && ny > 0 && (int64_t)nx * ny < kMaxCell, "oppx_neso_step_grid: nx, ny out of range");
+    return neso_step_impl(n_max, parts, nullptr, (int64_t)nx * ny, nx, ny, dt, cell_width, ext_x, ext_y, flags,
+                          counters, hops, stream);
 }
 
 }  // extern "C"

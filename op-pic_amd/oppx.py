@@ -125,6 +125,9 @@ _SIGS = {
     "oppx_quad_cells_build": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "oppx_neso_step": (c_int, [c_int64, POINTER(NesoParts), c_void_p, c_int64, c_double, c_double, c_double, c_double,
                                c_int, c_void_p, c_void_p, c_void_p]),
+    "oppx_quad_grid_check": (c_int, [c_void_p, c_int64, c_int, c_int, c_double, c_void_p, c_void_p]),
+    "oppx_neso_step_grid": (c_int, [c_int64, POINTER(NesoParts), c_int, c_int, c_double, c_double, c_double, c_double,
+                                    c_int, c_void_p, c_void_p, c_void_p]),
     "oppx_fempic_step_begin": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
     "oppx_hole_bits_words": (c_int64, [c_int64]),
     "oppx_node_cell_csr": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
@@ -684,7 +687,10 @@ class FempicBatch(FempicDevice):
 class NesoDevice:
     """NESO-advection particles + periodic quad mesh resident on the GPU (C-ABI driven)."""
 
-    def __init__(self, mesh: dict, pos: np.ndarray, vel: np.ndarray, p2c: np.ndarray, mdir=None, device="cuda"):
+    def __init__(self, mesh: dict, pos: np.ndarray, vel: np.ndarray, p2c: np.ndarray, mdir=None, device="cuda",
+                 grid: bool = True):
+        """grid=True: when the mesh is, bit for bit, the nx x ny periodic grid, the walk computes the
+        visited records instead of gathering them (same cells, hops and positions)."""
         import torch
         self.torch = torch
         self.dev = torch.device(device)
@@ -703,6 +709,12 @@ class NesoDevice:
         self.cells = torch.zeros(nc * 4, dtype=torch.float64, device=self.dev)  # 32 B per quad
         check(L.oppx_quad_cells_build(self.ll.data_ptr(), self.c2c.data_ptr(), nc, self.cells.data_ptr(),
                                       self.stream()), "quad_cells_build")
+        self.grid = False
+        if grid and self.nx * self.ny == nc:
+            bad = torch.zeros(1, dtype=torch.int32, device=self.dev)
+            check(L.oppx_quad_grid_check(self.cells.data_ptr(), nc, self.nx, self.ny, self.cw, bad.data_ptr(),
+                                         self.stream()), "quad_grid_check")
+            self.grid = int(bad.item()) == 0
         n = pos.shape[0]
         self.n = n
         cap = max(n, 1)
@@ -725,10 +737,37 @@ class NesoDevice:
         return self.torch.cuda.current_stream(self.dev).cuda_stream
 
     def step(self, dt, record_hops=False, do_update=True):
+        flags = (1 if do_update else 0) | 2
+        hops = self.hops.data_ptr() if record_hops else None
+        if self.grid:
+            check(lib().oppx_neso_step_grid(self.n, ctypes.byref(self.parts), self.nx, self.ny, dt, self.cw,
+                                            self.nx * self.cw, self.ny * self.cw, flags, self.counters.data_ptr(), hops,
+                                            self.stream()), "neso_step_grid")
+            return
         check(lib().oppx_neso_step(self.n, ctypes.byref(self.parts), self.cells.data_ptr(), self.nc, dt, self.cw,
-                                   self.nx * self.cw, self.ny * self.cw, (1 if do_update else 0) | 2,
-                                   self.counters.data_ptr(), self.hops.data_ptr() if record_hops else None,
+                                   self.nx * self.cw, self.ny * self.cw, flags, self.counters.data_ptr(), hops,
                                    self.stream()), "neso_step")
+
+    def sort(self):
+        """Stable reorder of the particle dats by cell (opp_particle_sort): the moved particles' cell
+        records are then read by neighbouring lanes. The swap buffers come in on the first call."""
+        torch = self.torch
+        if not hasattr(self, "_swap"):
+            cap = self.p2c.shape[0]
+            self._swap = [torch.empty_like(self.pos), torch.empty_like(self.vel), torch.empty_like(self.mdir),
+                          torch.empty_like(self.p2c)]
+            self.sort_scratch_bytes = int(lib().oppx_sort_scratch_bytes(cap))
+            self.sort_scratch = torch.zeros(self.sort_scratch_bytes, dtype=torch.uint8, device=self.dev)
+        cur = [self.pos, self.vel, self.mdir, self.p2c]
+        dims = [(2, 8), (2, 8), (2, 4), (1, 4)]
+        desc = lambda ts: (DatDesc * 4)(*[DatDesc(t.data_ptr(), d, e) for t, (d, e) in zip(ts, dims)])
+        check(lib().oppx_sort_by_cell(self.n, self.p2c.shape[0], self.counters.data_ptr(), self.nc, self.p2c.data_ptr(),
+                                      desc(cur), desc(self._swap), 4, None, 0, self.sort_scratch.data_ptr(),
+                                      self.sort_scratch_bytes, self.stream()), "sort_by_cell")
+        self.pos, self.vel, self.mdir, self.p2c = self._swap
+        self._swap = cur
+        self.parts = NesoParts(self.p2c.shape[0], self.pos.data_ptr(), self.vel.data_ptr(), self.mdir.data_ptr(),
+                               self.p2c.data_ptr())
 
     def read_counters(self):
         c = self.counters.cpu().numpy()
