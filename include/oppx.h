@@ -288,6 +288,57 @@ int oppx_fempic_batch_step_fused(int64_t n_max, const oppx_fempic_parts* parts, 
                                  int32_t* hops, double* node_charge, void* stream);
 
 /* ---------------------------------------------------------------------------------------
+ * FEM-PIC injection and the loops either side of the field solve (device)
+ * ------------------------------------------------------------------------------------- */
+
+/* Inlet faces of the injection (the reference app's iface set and its dats; device, AoS). */
+typedef struct oppx_inlet {
+    int64_t n_faces;
+    const int32_t* face_cell;  /* if2c_map [n_faces]: the tet behind the face */
+    const double* u;           /* iface_u_norm [n_faces][3] */
+    const double* v;           /* iface_v_norm [n_faces][3] */
+    const double* normal;      /* iface_norm [n_faces][3] */
+    const double* node_pos;    /* iface_n_pos [n_faces][9]; the first node is the origin */
+} oppx_inlet;
+
+/* opp_par_loop_injected__inject_ions_kernel (apps/app_fempic/kernels.h:63-90; loops
+ * apps_codegen/app_fempic_cg/{seq,hip}/inject_ions_kernel_loop.hpp) over particles
+ * [first, first + count), whose p2c holds their inlet face (opp_inc_part_count_with_distribution):
+ * position on the face from the k-th random pair (rand[k], rand[rand_stride + k], SoA), velocity
+ * ion_velocity * normal - (charge/mass) * E(cell) * dt/2, p2c = the face's cell. A face or cell
+ * index out of range counts in counters->faults (nullable) and leaves the particle as it was. */
+int oppx_fempic_inject(int64_t first, int64_t count, const oppx_fempic_parts* parts, const oppx_inlet* inlet,
+                       int64_t n_cells, const double* c_ef, const double* rand, int64_t rand_stride,
+                       double ion_velocity, double charge, double mass, double dt, oppx_move_counters* counters,
+                       void* stream);
+
+/* opp_par_loop_all__compute_node_charge_density_kernel (kernels.h:178-183):
+ * node_charge[i] *= spwt / node_volume[i]. */
+int oppx_fempic_node_charge_density(int64_t n_nodes, double* node_charge, const double* node_volume, double spwt,
+                                    void* stream);
+
+/* opp_par_loop_all__compute_electric_field_kernel (kernels.h:186-202): per tet and axis,
+ * c_ef -= sum_k c_sd[k][axis] * node_potential[c2n[k]] (c_sd: 12 doubles per tet, AoS). A c2n
+ * entry out of range counts in counters->faults (nullable) and leaves that tet's field. */
+int oppx_fempic_electric_field(int64_t n_cells, double* c_ef, const double* c_sd, const int32_t* c2n,
+                               const double* node_potential, int64_t n_nodes, oppx_move_counters* counters,
+                               void* stream);
+
+/* opp_par_loop_all__init_boundary_pot_kernel (kernels.h:48-59): node_bnd_pot = -1 * wall_potential
+ * on FIXED nodes (type 3), 0 elsewhere. */
+int oppx_fempic_boundary_potential(int64_t n_nodes, const int32_t* node_type, double wall_potential,
+                                   double* node_bnd_pot, void* stream);
+
+/* Scratch needed by oppx_max_reduce over n values (bytes). */
+size_t oppx_max_scratch_bytes(int64_t n);
+
+/* An OPP_MAX global over a dat (get_max_cef_kernel, get_final_max_values_kernel, kernels.h:205-224;
+ * MAX of opp_lib/include/opp_util.h:45): *result (device) = the fold m = init; m = MAX(v, m) over
+ * values[0..n) in order (|v| when abs_values), i.e. the first value attaining the maximum. */
+int oppx_max_reduce(int64_t n, const double* values, int abs_values, double init, double* result, void* scratch,
+                    size_t scratch_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------------------
  * NESO-advection hot path (device)
  * ------------------------------------------------------------------------------------- */
 

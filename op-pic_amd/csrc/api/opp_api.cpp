@@ -778,22 +778,29 @@ void opp_increase_particle_count(opp_set set, int insert_count) {
 }
 
 void opp_inc_part_count_with_distribution(opp_set set, int insert_count, opp_dat part_dist, bool calc_new) {
-    // the new particles' cells come from a cumulative distribution over part_dist's set
-    // (opp_lib/src/core/opp_increase_part_count_core.cpp:122-146)
+    // the new particles' relation holds their face of part_dist's set, filled as the seq core does
+    // (opp_lib/src/core/opp_increase_part_count_core.cpp:122-146: j advances by one whenever i reaches
+    // dist[j]); only the new slice travels to the device
     (void)calc_new;
     opp_increase_particle_count(set, insert_count);
     if (part_dist->dirty_hd == Dirty::Host) opp_download_dat(part_dist);
     const int* dist = (const int*)part_dist->data;
-    opp_dat r = set->mesh_relation_dat;
-    if (r->dirty_hd == Dirty::Host) opp_download_dat(r);
-    int* rel = (int*)r->data;
-    const int start = set->size - set->diff;
+    const int n_faces = part_dist->set->size;
+    const int n = set->diff, start = set->size - set->diff;
+    if (n <= 0) return;
+    std::vector<int> rel((size_t)n);
     int j = 0;
-    for (int i = 0; i < set->diff; i++) {
-        if (i >= dist[j]) j++;
-        rel[start + i] = j;
+    for (int i = 0; i < n; i++) {
+        if (j < n_faces && i >= dist[j]) j++;
+        rel[(size_t)i] = j;
     }
-    opp_upload_dat(r);
+    if (rel.back() >= n_faces)
+        opp_abort("opp_inc_part_count_with_distribution: insert_count exceeds the distribution's total");
+    opp_dat r = set->mesh_relation_dat;
+    if (r->dirty_hd != Dirty::Device)  // device copy current: the slice goes up now
+        check(oppx_memcpy_htod((int*)r->data_d + start, rel.data(), (size_t)n * sizeof(int), nullptr), "relation slice");
+    if (r->dirty_hd != Dirty::Host)  // host copy current (or the newer one): keep it so
+        memcpy((int*)r->data + start, rel.data(), (size_t)n * sizeof(int));
 }
 
 void opp_reset_num_particles_to_insert(opp_set set) { set->diff = 0; }

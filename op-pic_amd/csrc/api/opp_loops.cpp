@@ -31,7 +31,10 @@ struct QuadCache {
 static TetAuxCache g_tet;
 static QuadCache g_quad;
 
+void free_field_loop_buffers();
+
 void free_loop_caches() {
+    free_field_loop_buffers();
     dev_free(g_tet.aux);
     dev_free(g_quad.cells);
     g_tet = TetAuxCache();
@@ -206,6 +209,140 @@ void opp_par_loop_all__deposit_charge_on_nodes_kernel(opp_set set, opp_arg arg0,
           "deposit_charge_on_nodes");
     arg1.dat->dirty_hd = Dirty::Host;
     opp_profiler->end("deposit_charge_on_nodes_kernel");
+}
+
+// ---------------------------------------------------------------------------------------------
+// FEM-PIC injection and the loops either side of the field solve (fempic_opp.cpp:41-49)
+
+// Device workspace of the OPP_MAX globals: scratch for the partials and the result.
+static void* g_max_scratch = nullptr;
+static size_t g_max_scratch_bytes = 0;
+static double* g_max_result = nullptr;
+
+// An OPP_MAX global (`gbl`, host double holding the initial value) folded over n device values.
+static void max_global(const opp_arg& gbl, const double* values, int64_t n, int abs_values, const char* what) {
+    if (gbl.argtype != OPP_ARG_GBL || !gbl.data || gbl.acc != OPP_MAX) opp_abort(std::string(what) + ": expects an OPP_MAX global");
+    const size_t need = oppx_max_scratch_bytes(n);
+    if (need > g_max_scratch_bytes) {
+        dev_free(g_max_scratch);
+        g_max_scratch = dev_alloc(need);
+        g_max_scratch_bytes = need;
+    }
+    if (!g_max_result) g_max_result = (double*)dev_alloc(sizeof(double));
+    double init;
+    memcpy(&init, gbl.data, sizeof(double));
+    check(oppx_max_reduce(n, values, abs_values, init, g_max_result, g_max_scratch, g_max_scratch_bytes, nullptr), what);
+    check(oppx_memcpy_dtoh(gbl.data, g_max_result, sizeof(double), nullptr), what);
+}
+
+void oppx_api::free_field_loop_buffers() {
+    dev_free(g_max_scratch);
+    dev_free(g_max_result);
+    g_max_scratch = nullptr;
+    g_max_result = nullptr;
+    g_max_scratch_bytes = 0;
+}
+
+// apps_codegen/app_fempic_cg/hip/init_boundary_pot_kernel_loop.hpp; elemental apps/app_fempic/kernels.h:48-59
+void opp_par_loop_all__init_boundary_pot_kernel(opp_set set, opp_arg arg0, opp_arg arg1) {
+    opp_profiler->start("init_boundary_pot_kernel");
+    opp_arg args[2] = {arg0, arg1};
+    args_to_device(2, args);
+    check(oppx_fempic_boundary_potential(set->size, (const int32_t*)arg0.dat->data_d, const_real("CONST_wall_potential"),
+                                         (double*)arg1.dat->data_d, nullptr),
+          "init_boundary_pot");
+    args_written_on_device(2, args);
+    opp_profiler->end("init_boundary_pot_kernel");
+}
+
+// apps_codegen/app_fempic_cg/hip/inject_ions_kernel_loop.hpp; elemental apps/app_fempic/kernels.h:63-90. The
+// injected particles [size - diff, size) hold their inlet face (opp_inc_part_count_with_distribution); the
+// random pairs are read from the start of arg9's dat.
+void opp_par_loop_injected__inject_ions_kernel(opp_set set, opp_arg arg0, opp_arg arg1, opp_arg arg2, opp_arg arg3,
+                                               opp_arg arg4, opp_arg arg5, opp_arg arg6, opp_arg arg7, opp_arg arg8,
+                                               opp_arg arg9) {
+    opp_profiler->start("inject_ions_kernel");
+    opp_arg args[10] = {arg0, arg1, arg2, arg3, arg4, arg5, arg6, arg7, arg8, arg9};
+    args_to_device(10, args);
+    if (set->mesh_relation_dat->dirty_hd == Dirty::Device) opp_upload_dat(set->mesh_relation_dat);
+    const int64_t count = set->diff, first = (int64_t)set->size - set->diff;
+    if (arg3.argtype != OPP_ARG_MAP || !arg3.data_d || arg3.dim != 1)
+        opp_abort("inject_ions: expects the inlet-face -> cell map as argument 3");
+    if (!arg4.map || arg4.map->dim != 1) opp_abort("inject_ions: expects c_ef through the inlet-face -> cell map");
+    if (arg9.dat->set->size < count) opp_abort("inject_ions: fewer random pairs than injected particles");
+    if (count > 0) {
+        // every inlet face must name a tet: then the device never meets an index out of range
+        const int n_faces = arg3.size, n_cells = arg4.dat->set->size;
+        for (int f = 0; f < n_faces; f++)
+            if (((const int*)arg3.data)[f] < 0 || ((const int*)arg3.data)[f] >= n_cells)
+                opp_abort("inject_ions: inlet face " + std::to_string(f) + " names no tet");
+        oppx_inlet in{};
+        in.n_faces = n_faces;
+        in.face_cell = (const int32_t*)arg3.data_d;
+        in.u = (const double*)arg5.dat->data_d;
+        in.v = (const double*)arg6.dat->data_d;
+        in.normal = (const double*)arg7.dat->data_d;
+        in.node_pos = (const double*)arg8.dat->data_d;
+        oppx_fempic_parts parts{set->set_capacity, (double*)arg0.dat->data_d, (double*)arg1.dat->data_d, nullptr,
+                                (int32_t*)set->mesh_relation_dat->data_d};
+        check(oppx_fempic_inject(first, count, &parts, &in, n_cells, (const double*)arg4.dat->data_d,
+                                 (const double*)arg9.dat->data_d, arg9.dat->set->set_capacity,
+                                 const_real("CONST_ion_velocity"), const_real("CONST_charge"), const_real("CONST_mass"),
+                                 const_real("CONST_dt"), nullptr, nullptr),
+              "inject_ions");
+    }
+    args_written_on_device(10, args);
+    set->mesh_relation_dat->dirty_hd = Dirty::Host;
+    opp_profiler->end("inject_ions_kernel");
+}
+
+// apps_codegen/app_fempic_cg/hip/compute_node_charge_density_kernel_loop.hpp; elemental kernels.h:178-183
+void opp_par_loop_all__compute_node_charge_density_kernel(opp_set set, opp_arg arg0, opp_arg arg1) {
+    opp_profiler->start("compute_node_charge_density_kernel");
+    opp_arg args[2] = {arg0, arg1};
+    args_to_device(2, args);
+    check(oppx_fempic_node_charge_density(set->size, (double*)arg0.dat->data_d, (const double*)arg1.dat->data_d,
+                                          const_real("CONST_spwt"), nullptr),
+          "compute_node_charge_density");
+    args_written_on_device(2, args);
+    opp_profiler->end("compute_node_charge_density_kernel");
+}
+
+// apps_codegen/app_fempic_cg/hip/compute_electric_field_kernel_loop.hpp; elemental kernels.h:186-202
+void opp_par_loop_all__compute_electric_field_kernel(opp_set set, opp_arg arg0, opp_arg arg1, opp_arg arg2,
+                                                     opp_arg arg3, opp_arg arg4, opp_arg arg5) {
+    opp_profiler->start("compute_electric_field_kernel");
+    opp_arg args[6] = {arg0, arg1, arg2, arg3, arg4, arg5};
+    args_to_device(6, args);
+    if (!arg2.map || arg2.map->dim != 4 || arg2.idx != 0 || arg3.idx != 1 || arg4.idx != 2 || arg5.idx != 3 ||
+        arg2.dat != arg5.dat)
+        opp_abort("compute_electric_field: expects the node potential through c2n indices 0..3");
+    check(oppx_fempic_electric_field(set->size, (double*)arg0.dat->data_d, (const double*)arg1.dat->data_d,
+                                     arg2.map->map_d, (const double*)arg2.dat->data_d, arg2.dat->set->size, nullptr,
+                                     nullptr),
+          "compute_electric_field");
+    args_written_on_device(6, args);
+    opp_profiler->end("compute_electric_field_kernel");
+}
+
+// apps_codegen/app_fempic_cg/hip/get_max_cef_kernel_loop.hpp; elemental kernels.h:216-224
+void opp_par_loop_all__get_max_cef_kernel(opp_set set, opp_arg arg0, opp_arg arg1) {
+    opp_profiler->start("get_max_cef_kernel");
+    opp_arg args[2] = {arg0, arg1};
+    args_to_device(2, args);
+    max_global(arg1, (const double*)arg0.dat->data_d, (int64_t)set->size * arg0.dat->dim, 0, "get_max_cef");
+    opp_profiler->end("get_max_cef_kernel");
+}
+
+// apps_codegen/app_fempic_cg/hip/get_final_max_values_kernel_loop.hpp; elemental kernels.h:205-214
+void opp_par_loop_all__get_final_max_values_kernel(opp_set set, opp_arg arg0, opp_arg arg1, opp_arg arg2,
+                                                   opp_arg arg3) {
+    opp_profiler->start("get_final_max_values_kernel");
+    opp_arg args[4] = {arg0, arg1, arg2, arg3};
+    args_to_device(4, args);
+    max_global(arg1, (const double*)arg0.dat->data_d, set->size, 1, "get_final_max_values");
+    max_global(arg3, (const double*)arg2.dat->data_d, set->size, 1, "get_final_max_values");
+    opp_profiler->end("get_final_max_values_kernel");
 }
 
 // apps_codegen/app_fempic_cg/hip/move_kernel_loop.hpp:409-484 (opp_init_direct_hop_cg); overlay per

@@ -448,6 +448,13 @@ void opp_particle_move__move_kernel(opp_set, opp_map, opp_map, opp_arg, opp_arg,
 void opp_par_loop_all__deposit_charge_on_nodes_kernel(opp_set, opp_arg, opp_arg, opp_arg, opp_arg, opp_arg);
 void opp_init_direct_hop_cg(double, const opp_dat, const opp::BoundingBox&, opp_map, opp_map, opp_arg, opp_arg,
                             opp_arg, opp_arg);
+void opp_par_loop_all__init_boundary_pot_kernel(opp_set, opp_arg, opp_arg);
+void opp_par_loop_injected__inject_ions_kernel(opp_set, opp_arg, opp_arg, opp_arg, opp_arg, opp_arg, opp_arg, opp_arg,
+                                               opp_arg, opp_arg, opp_arg);
+void opp_par_loop_all__compute_node_charge_density_kernel(opp_set, opp_arg, opp_arg);
+void opp_par_loop_all__compute_electric_field_kernel(opp_set, opp_arg, opp_arg, opp_arg, opp_arg, opp_arg, opp_arg);
+void opp_par_loop_all__get_max_cef_kernel(opp_set, opp_arg, opp_arg);
+void opp_par_loop_all__get_final_max_values_kernel(opp_set, opp_arg, opp_arg, opp_arg, opp_arg);
 // NESO-advection
 void opp_par_loop_all__update_pos_kernel(opp_set, opp_arg, opp_arg);
 void opp_particle_move__move_kernel(opp_set, opp_map, opp_map, opp_arg, opp_arg, opp_arg);

@@ -64,6 +64,11 @@ class BatchInst(Structure):
                 ("k", c_double), ("dt", c_double)]
 
 
+class Inlet(Structure):
+    _fields_ = [("n_faces", c_int64), ("face_cell", c_void_p), ("u", c_void_p), ("v", c_void_p), ("normal", c_void_p),
+                ("node_pos", c_void_p)]
+
+
 class DatDesc(Structure):
     _fields_ = [("base", c_void_p), ("dim", c_int32), ("elem_bytes", c_int32)]
 
@@ -126,6 +131,14 @@ _SIGS = {
     "oppx_neso_step": (c_int, [c_int64, POINTER(NesoParts), c_void_p, c_int64, c_double, c_double, c_double, c_double,
                                c_int, c_void_p, c_void_p, c_void_p]),
     "oppx_quad_grid_check": (c_int, [c_void_p, c_int64, c_int, c_int, c_double, c_void_p, c_void_p]),
+    "oppx_fempic_inject": (c_int, [c_int64, c_int64, POINTER(FempicParts), POINTER(Inlet), c_int64, c_void_p, c_void_p,
+                                   c_int64, c_double, c_double, c_double, c_double, c_void_p, c_void_p]),
+    "oppx_fempic_node_charge_density": (c_int, [c_int64, c_void_p, c_void_p, c_double, c_void_p]),
+    "oppx_fempic_electric_field": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+                                           c_void_p]),
+    "oppx_fempic_boundary_potential": (c_int, [c_int64, c_void_p, c_double, c_void_p, c_void_p]),
+    "oppx_max_scratch_bytes": (c_size_t, [c_int64]),
+    "oppx_max_reduce": (c_int, [c_int64, c_void_p, c_int, c_double, c_void_p, c_void_p, c_size_t, c_void_p]),
     "oppx_neso_step_grid": (c_int, [c_int64, POINTER(NesoParts), c_int, c_int, c_double, c_double, c_double, c_double,
                                     c_int, c_void_p, c_void_p, c_void_p]),
     "oppx_fempic_step_begin": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),

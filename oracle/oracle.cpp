@@ -359,4 +359,80 @@ int64_t orc_neso_step(int64_t n, int nx, int ny, double cell_width, double dt, d
     return total;
 }
 
+// ---------------------------------------------------------------------------------------
+// Injection and the loops either side of the field solve.
+
+// opp_inc_part_count_with_distribution's relation fill, seq core
+// (opp_lib/src/core/opp_increase_part_count_core.cpp:122-146): new particle i of [0, count) gets
+// face j, where j advances by one whenever i reaches dist[j] (dist = cumulative counts).
+void orc_inc_part_distribution(int64_t start, int64_t count, const int* dist, int n_faces, int* rel) {
+    int j = 0;
+    for (int64_t i = 0; i < count; i++) {
+        if (j < n_faces && i >= dist[j]) j++;
+        rel[start + i] = j;
+    }
+}
+
+// inject_ions_kernel (apps/app_fempic/kernels.h:63-90) in the loop of
+// apps_codegen/app_fempic_cg/seq/inject_ions_kernel_loop.hpp: particles [first, first + count),
+// p2c = inlet face on entry, random pair k for the k-th of them. GCC contracts
+// a*u + b*v + n into fma(a, u, b*v) + n and n*iv - ((q/m)*E)*(dt/2) into fma(-((q/m)*E), dt/2, n*iv).
+void orc_fempic_inject(int64_t first, int64_t count, const int* if2c, const double* c_ef, const double* if_u,
+                       const double* if_v, const double* if_norm, const double* if_npos, const double* rnd,
+                       double* pos, double* vel, int* p2c, double ion_velocity, double charge, double mass,
+                       double dt) {
+    for (int64_t k = 0; k < count; k++) {
+        const int64_t i = first + k;
+        const int f = p2c[i];
+        const int c = if2c[f];
+        double a = rnd[2 * k], b = rnd[2 * k + 1];
+        if ((a + b) > 1) {
+            a = (1 - a);
+            b = (1 - b);
+        }
+        for (int d = 0; d < 3; d++) {
+            pos[3 * i + d] = std::fma(a, if_u[3 * f + d], b * if_v[3 * f + d]) + if_npos[9 * f + d];
+            const double v0 = if_norm[3 * f + d] * ion_velocity;
+            vel[3 * i + d] = std::fma(-(charge / mass * c_ef[3 * c + d]), 0.5 * dt, v0);
+        }
+        p2c[i] = c;
+    }
+}
+
+// compute_node_charge_density_kernel (kernels.h:178-183)
+void orc_node_charge_density(int64_t n, double* q, const double* vol, double spwt) {
+    for (int64_t i = 0; i < n; i++) q[i] *= (spwt / vol[i]);
+}
+
+// compute_electric_field_kernel (kernels.h:186-202): GCC contracts (c1 + c2 + c3 + c4) into
+// fma(sd3, p3, fma(sd2, p2, fma(sd0, p0, sd1*p1))).
+void orc_electric_field(int64_t n_cells, double* ef, const double* sd, const int* c2n, const double* pot) {
+    for (int64_t c = 0; c < n_cells; c++) {
+        const double p0 = pot[c2n[4 * c]], p1 = pot[c2n[4 * c + 1]], p2 = pot[c2n[4 * c + 2]],
+                     p3 = pot[c2n[4 * c + 3]];
+        for (int d = 0; d < 3; d++) {
+            double t = std::fma(sd[12 * c + d], p0, sd[12 * c + 3 + d] * p1);
+            t = std::fma(sd[12 * c + 6 + d], p2, t);
+            t = std::fma(sd[12 * c + 9 + d], p3, t);
+            ef[3 * c + d] -= t;
+        }
+    }
+}
+
+// init_boundary_pot_kernel (kernels.h:48-59)
+void orc_boundary_potential(int64_t n, const int* type, double wall, double* bnd) {
+    for (int64_t i = 0; i < n; i++) bnd[i] = type[i] == 3 ? -1 * wall : 0;
+}
+
+// An OPP_MAX global as the seq loops fold it: m = MAX(v, m) in index order
+// (get_max_cef_kernel / get_final_max_values_kernel, kernels.h:205-224; MAX of opp_util.h:45).
+double orc_max_fold(int64_t n, const double* x, int abs_values, double init) {
+    double m = init;
+    for (int64_t i = 0; i < n; i++) {
+        const double v = abs_values ? std::fabs(x[i]) : x[i];
+        m = (v > m) ? v : m;
+    }
+    return m;
+}
+
 }  // extern "C"

@@ -61,6 +61,28 @@ int64_t orc_neso_step(int64_t n, int nx, int ny, double cell_width, double dt, d
                       const int* c2c, const double* c_pos_ll, double* pos, const double* vel, int* mdir,
                       int* p2c, int* hops, int threads);
 
+/* opp_inc_part_count_with_distribution's relation fill (seq core): rel[start + i] = inlet face of
+ * new particle i of [0, count) from the cumulative counts dist (pinned: reference loop fixture). */
+void orc_inc_part_distribution(int64_t start, int64_t count, const int* dist, int n_faces, int* rel);
+
+/* inject_ions over particles [first, first + count) (pinned: reference loop fixture). */
+void orc_fempic_inject(int64_t first, int64_t count, const int* if2c, const double* c_ef, const double* if_u,
+                       const double* if_v, const double* if_norm, const double* if_npos, const double* rnd,
+                       double* pos, double* vel, int* p2c, double ion_velocity, double charge, double mass,
+                       double dt);
+
+/* compute_node_charge_density (pinned: reference loop fixture). */
+void orc_node_charge_density(int64_t n, double* q, const double* vol, double spwt);
+
+/* compute_electric_field, c_ef accumulated (pinned: reference loop fixture). */
+void orc_electric_field(int64_t n_cells, double* ef, const double* sd, const int* c2n, const double* pot);
+
+/* init_boundary_pot (pinned: reference loop fixture). */
+void orc_boundary_potential(int64_t n, const int* type, double wall, double* bnd);
+
+/* OPP_MAX fold in index order (pinned: reference loop fixture). */
+double orc_max_fold(int64_t n, const double* x, int abs_values, double init);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,13 @@ _SIGS = {
                                           c_void_p, c_int]),
     "orc_neso_step": (c_int64, [c_int64, c_int, c_int, c_double, c_double, c_double, c_double, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "orc_inc_part_distribution": (None, [c_int64, c_int64, c_void_p, c_int, c_void_p]),
+    "orc_fempic_inject": (None, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_double]),
+    "orc_node_charge_density": (None, [c_int64, c_void_p, c_void_p, c_double]),
+    "orc_electric_field": (None, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "orc_boundary_potential": (None, [c_int64, c_void_p, c_double, c_void_p]),
+    "orc_max_fold": (c_double, [c_int64, c_void_p, c_int, c_double]),
 }
 
 _lib = None
@@ -141,6 +148,72 @@ class FempicState:
         n = self.n
         return dict(n=n, pos=self.pos[:n].copy(), vel=self.vel[:n].copy(), lc=self.lc[:n].copy(),
                     p2c=self.p2c[:n].copy(), charge=self.charge.copy())
+
+
+class FempicLoop(FempicState):
+    """The FEM-PIC main loop minus the field solve (apps/app_fempic/fempic.cpp:138-231): injection
+    through an inlet, push, move, hole fill, deposit, node charge density, electric field from a
+    given node potential, and the loop's OPP_MAX globals. Starts with no particles."""
+
+    def __init__(self, mesh, c_ef, sd, n_vol, n_type, n_pot, inlet, rand, capacity, overlay=None, bmin=None,
+                 gs=None, bmax=None):
+        z = np.zeros((capacity, 3))
+        super().__init__(mesh, c_ef, z, z, np.zeros(capacity, np.int32), overlay=overlay, bmin=bmin, gs=gs,
+                         bmax=bmax)
+        self.n, self.diff = 0, 0
+        self.nc = int(mesh["n_cells"])
+        self.sd = np.ascontiguousarray(sd, np.float64)
+        self.n_vol = np.ascontiguousarray(n_vol, np.float64)
+        self.n_type = np.ascontiguousarray(n_type, np.int32)
+        self.n_pot = np.ascontiguousarray(n_pot, np.float64)
+        self.inlet = {k: np.ascontiguousarray(inlet[k]) for k in ("if2c", "if_u", "if_v", "if_norm", "if_npos",
+                                                                  "if_dist")}
+        self.rand = np.ascontiguousarray(rand, np.float64)
+
+    def inc_distribution(self, count):
+        """opp_inc_part_count_with_distribution: count new particles holding their inlet face."""
+        if self.n + count > self.p2c.shape[0]:
+            raise ValueError("capacity exceeded")
+        d = self.inlet["if_dist"]
+        lib().orc_inc_part_distribution(self.n, count, _p(d), d.shape[0], _p(self.p2c))
+        self.n += count
+        self.diff = count
+
+    def inject(self, ion_velocity, charge, mass, dt):
+        i = self.inlet
+        lib().orc_fempic_inject(self.n - self.diff, self.diff, _p(i["if2c"]), _p(self.c_ef), _p(i["if_u"]),
+                                _p(i["if_v"]), _p(i["if_norm"]), _p(i["if_npos"]), _p(self.rand), _p(self.pos),
+                                _p(self.vel), _p(self.p2c), ion_velocity, charge, mass, dt)
+
+    def boundary_potential(self, wall):
+        out = np.zeros(self.nn)
+        lib().orc_boundary_potential(self.nn, _p(self.n_type), wall, _p(out))
+        return out
+
+    def field_side(self, spwt):
+        """compute_node_charge_density, then c_ef reset and compute_electric_field; returns the
+        loop's maxima (max c_ef component, max |charge density|, max |potential|)."""
+        lib().orc_node_charge_density(self.nn, _p(self.charge), _p(self.n_vol), spwt)
+        self.c_ef[:] = 0.0
+        lib().orc_electric_field(self.nc, _p(self.c_ef), _p(self.sd), _p(self.c2n), _p(self.n_pot))
+        L = lib()
+        return (L.orc_max_fold(3 * self.nc, _p(self.c_ef), 0, 0.0), L.orc_max_fold(self.nn, _p(self.charge), 1, 0.0),
+                L.orc_max_fold(self.nn, _p(self.n_pot), 1, 0.0))
+
+    def loop_step(self, inject_count, first, ion_velocity, charge, mass, dt, spwt):
+        """One main-loop iteration; returns (size before the move, size after, total hops, max hops,
+        maxima)."""
+        if not first:
+            self.inc_distribution(inject_count)
+        before = self.n
+        self.inject(ion_velocity, charge, mass, dt)
+        self.charge[:] = 0.0
+        self.push(charge, mass, dt)
+        removed, hops = self.move()
+        self.holefill(removed)
+        self.deposit()
+        mx = self.field_side(spwt)
+        return before, self.n, int(hops.sum()), int(hops.max()) if hops.size else 0, mx
 
 
 def dh_overlay_bruteforce(mesh, bmin, bmax, gs, threads=0):

@@ -130,3 +130,56 @@ def test_neso_api_app_matches_reference(name):
         assert np.array_equal(rd("final_p2c", np.int32), g["out_final_p2c"])
         assert np.array_equal(rd("final_pos", np.float64), g["out_final_pos"].ravel())
         assert np.array_equal(rd("final_mdir", np.int32), g["out_final_mdir"].ravel())
+
+
+@pytest.mark.parametrize("name", ["fempic_loop_small.npz", "fempic_loop_dh.npz"])
+def test_fempic_loop_app_matches_reference(name):
+    """The main loop minus the field solve through the drop-in (injection, push, move, deposit, node
+    charge density, electric field, OPP_MAX globals) against the reference seq build: particles,
+    injected records, boundary potential, field and the field maxima bit-exact; charge density and
+    its maximum within 1e-12 relative (FP64 atomic deposit)."""
+    g = np.load(golden_path(name), allow_pickle=False)
+    nx = int(g["meta_nx"])
+    m = oppx.box_mesh(nx, nx, nx)
+    ef = oppx.box_field_sines(m)
+    with tempfile.TemporaryDirectory() as td:
+        ind, outd = os.path.join(td, "in"), os.path.join(td, "out")
+        os.makedirs(ind)
+        os.makedirs(outd)
+        for k in ["node_pos", "c2n", "c2c", "c_det", "c_vol"]:
+            m[k].tofile(os.path.join(ind, k + ".bin"))
+        ef.tofile(os.path.join(ind, "c_ef.bin"))
+        for k in ("sd", "n_vol", "n_type", "n_pot", "if2c", "if_u", "if_v", "if_norm", "if_npos", "if_dist", "rand"):
+            g[f"in_{k}"].tofile(os.path.join(ind, ("c_sd" if k == "sd" else k) + ".bin"))
+        f = lambda k: float(g[f"meta_{k}"])
+        lines = [f"INT nnodes = {m['n_nodes']}", f"INT ncells = {m['n_cells']}", f"INT nfaces = {g['in_if2c'].size}",
+                 f"INT nrand = {g['in_rand'].shape[0]}", f"INT inject_count = {int(g['meta_inject'])}",
+                 f"INT num_steps = {int(g['meta_steps'])}", f"REAL dt = {f('dt')!r}", f"REAL mass = {f('mass')!r}",
+                 f"REAL charge = {f('charge')!r}", f"REAL spwt = {f('spwt')!r}",
+                 f"REAL ion_velocity = {f('ion_velocity')!r}", f"REAL wall_potential = {f('wall_potential')!r}",
+                 f"REAL grid_spacing = {f('gs')!r}", "REAL bminx = 0.0", "REAL bminy = 0.0", "REAL bminz = 0.0",
+                 "REAL bmaxx = 1.0", "REAL bmaxy = 1.0", "REAL bmaxz = 1.0",
+                 f"BOOL opp_global_move = {'true' if int(g['meta_dh']) else 'false'}",
+                 "BOOL opp_dh_data_generate = true", "REAL opp_allocation_multiple = 1.1",
+                 "BOOL opp_profile_all = false", "STRING opp_fill = HoleFill_All", "INT opp_fill_period = 4"]
+        par = os.path.join(td, "case.param")
+        with open(par, "w") as fh:
+            fh.write("\n".join(lines) + "\n")
+        run_app(app("fempic_loop_app"), par, ind, outd)
+        rd = lambda k, t: np.fromfile(os.path.join(outd, k + ".bin"), dtype=t)
+        assert np.array_equal(rd("bnd_pot", np.float64), g["out_bnd_pot"])
+        assert np.array_equal(rd("dist1_p2c", np.int32), g["out_dist1_p2c"])
+        for k, t in (("pos", np.float64), ("vel", np.float64), ("p2c", np.int32)):
+            assert np.array_equal(rd("inj1_" + k, t), g["out_inj1_" + k]), "injected " + k
+        rows = [ln.split() for ln in open(os.path.join(outd, "steps.txt")).read().splitlines() if ln.strip()]
+        st = np.array([[int(x) for x in r[:4]] for r in rows], np.int64)
+        mx = np.array([[float.fromhex(x) for x in r[4:7]] for r in rows])
+        ref, rmx = g["out_steps_iter_before_after_hops_maxhops"], g["out_steps_max_cef_chg_pot"]
+        assert np.array_equal(st, ref[:, [0, 1, 2, 4]])
+        assert np.array_equal(mx[:, 0], rmx[:, 0]) and np.array_equal(mx[:, 2], rmx[:, 2])
+        assert np.all(np.abs(mx[:, 1] - rmx[:, 1]) <= DEPOSIT_RTOL * np.abs(rmx[:, 1]))
+        for t in ("s1", "final"):
+            for k, dt_ in (("pos", np.float64), ("vel", np.float64), ("lc", np.float64), ("p2c", np.int32),
+                           ("ef", np.float64)):
+                assert np.array_equal(rd(f"{t}_{k}", dt_), g[f"out_{t}_{k}"]), f"{t} {k}"
+            assert charge_close(rd(f"{t}_charge", np.float64), g[f"out_{t}_charge"]), f"{t} charge"
