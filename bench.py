@@ -80,10 +80,10 @@ def cpu_baseline(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, budget_s):
                        f"{t_tot:.1f}s")
 
 
-def load_traffic(workload_tag):
+def load_traffic(workload_tag, kernel="fempic_first_kernel"):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter summary
     (profiles/<kernel>_<workload>.md holds a line `traffic_bytes_per_launch: <B>`), or None."""
-    path = os.path.join(ROOT, "profiles", f"fempic_first_kernel_{workload_tag}.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload_tag}.md")
     try:
         with open(path) as f:
             for line in f:
@@ -308,6 +308,8 @@ def bench_neso(args):
         torch.cuda.empty_cache()
         return wall, ms, c, used_grid
 
+    copy_gbs = copy_peak_gbs(torch)
+    log(f"device copy rate {copy_gbs:.0f} GB/s")
     wall, ms, c, used_grid = run(True, args.steps, args.warmup)
     g_wall, g_ms, _, _ = run(False, min(args.steps, 16), 2)
     alg = 64.0 * n + 32.0 * nx * nx  # R pos 16, vel 16, mdir 8, p2c 4; W pos 16, mdir 8, p2c 4 (B/particle)
@@ -334,7 +336,8 @@ def bench_neso(args):
         "gathered_records": {"ms_per_step": 1e3 * g_wall / len(g_ms), "value": n * len(g_ms) / g_wall,
                              "p50_step_ms": float(np.percentile(g_ms, 50))},
         "roofline": {"kernel": "neso_step_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic("c3", "neso_step_kernel"), "measured_copy_gbs": copy_gbs},
     }
 
 
