@@ -435,4 +435,176 @@ double orc_max_fold(int64_t n, const double* x, int abs_values, double init) {
     return m;
 }
 
+// ---------------------------------------------------------------------------------------
+// CabanaPIC move with per-hop current deposit.
+
+namespace {
+
+// weight_current_to_accumulator_kernel (apps/app_cabanapic/kernels.h:109-136) with CALC_J (:11-25).
+// The reference GCC build fuses v5 = q*ux*uy*uz/3 into each of the twelve v0 += v5 / v1 -= v5 /
+// v2 -= v5 / v3 += v5 (it is formed first, and all its uses are additions), and v1 = v4*d into both
+// v0 = v4 - v1 and v1 += v4.
+void cabana_weight(double* acc, double q, double ux, double uy, double uz, double dx, double dy, double dz) {
+    const double w = q * ux * uy * uz;  // v5 = w * (1/3), fused below
+    const double third = 1.0 / 3.0;
+    auto calc_j = [&](double u, double dA, double dB, double* out) {
+        const double v4 = q * u;
+        double v0 = std::fma(-v4, dA, v4);
+        double v1 = std::fma(v4, dA, v4);
+        const double p = 1.0 + dB;
+        const double v2 = v0 * p;
+        const double v3 = v1 * p;
+        const double m = 1.0 - dB;
+        v0 = v0 * m;
+        v1 = v1 * m;
+        out[0] += std::fma(w, third, v0);
+        out[1] += std::fma(-w, third, v1);
+        out[2] += std::fma(-w, third, v2);
+        out[3] += std::fma(w, third, v3);
+    };
+    calc_j(ux, dy, dz, acc + 0);
+    calc_j(uy, dz, dx, acc + 4);
+    calc_j(uz, dx, dy, acc + 8);
+}
+
+enum { kCabDone = 0, kCabMove = 1 };
+
+// move_deposit_kernel (apps/app_cabanapic/kernels.h:139-353); `first` is OPP_DO_ONCE. Contractions
+// of the reference build: the interpolated E and B as fma chains, the Boris scalars and cross
+// products as written below; products whose uses straddle a sqrt stay unfused.
+int cabana_eval(bool first, double* vel, double* pos, double* streak, double weight, const double* ip, double* acc,
+                const int* c2c6, int* cell, double qsp, double qdt_2mc, const double cdt_d[3]) {
+    if (first) {
+        const double ex = ip[0], dexdy = ip[1], dexdz = ip[2], d2exdydz = ip[3];
+        const double ey = ip[4], deydz = ip[5], deydx = ip[6], d2eydzdx = ip[7];
+        const double ez = ip[8], dezdx = ip[9], dezdy = ip[10], d2ezdxdy = ip[11];
+        double cbx = ip[12];
+        const double dcbxdx = ip[13];
+        double cby = ip[14];
+        const double dcbydy = ip[15];
+        double cbz = ip[16];
+        const double dcbzdz = ip[17];
+        const double dx = pos[0], dy = pos[1], dz = pos[2];
+        // hax = qdt_2mc * t stays a product: its second use sits after the errno branch GCC puts
+        // around sqrt (-fmath-errno), in another basic block, so neither use is fused
+        const double tx = std::fma(dz, std::fma(dy, d2exdydz, dexdz), std::fma(dy, dexdy, ex));
+        const double ty = std::fma(dx, std::fma(dz, d2eydzdx, deydx), std::fma(dz, deydz, ey));
+        const double tz = std::fma(dy, std::fma(dx, d2ezdxdy, dezdy), std::fma(dx, dezdx, ez));
+        cbx = std::fma(dx, dcbxdx, cbx);
+        cby = std::fma(dy, dcbydy, cby);
+        cbz = std::fma(dz, dcbzdz, cbz);
+        const double hax = qdt_2mc * tx, hay = qdt_2mc * ty, haz = qdt_2mc * tz;
+        double ux = vel[0] + hax;
+        double uy = vel[1] + hay;
+        double uz = vel[2] + haz;
+        double v0 = qdt_2mc / std::sqrt(1.0 + std::fma(ux, ux, std::fma(uy, uy, uz * uz)));
+        double v1 = std::fma(cbx, cbx, std::fma(cby, cby, cbz * cbz));
+        double v2 = (v0 * v0) * v1;
+        double v3 = v0 * std::fma(v2, std::fma(v2, 2.0 / 15.0, 1.0 / 3.0), 1.0);
+        double v4 = v3 / std::fma(v1, v3 * v3, 1.0);
+        v4 = v4 + v4;
+        v0 = std::fma(v3, std::fma(uy, cbz, -(uz * cby)), ux);
+        v1 = std::fma(v3, std::fma(uz, cbx, -(ux * cbz)), uy);
+        v2 = std::fma(v3, std::fma(ux, cby, -(uy * cbx)), uz);
+        ux = std::fma(v4, std::fma(v1, cbz, -(v2 * cby)), ux);
+        uy = std::fma(v4, std::fma(v2, cbx, -(v0 * cbz)), uy);
+        uz = std::fma(v4, std::fma(v0, cby, -(v1 * cbx)), uz);
+        ux = ux + hax;
+        uy = uy + hay;
+        uz = uz + haz;
+        vel[0] = ux;
+        vel[1] = uy;
+        vel[2] = uz;
+        v0 = 1.0 / std::sqrt(1.0 + std::fma(ux, ux, std::fma(uy, uy, uz * uz)));
+        ux = ux * cdt_d[0];
+        uy = uy * cdt_d[1];
+        uz = uz * cdt_d[2];
+        ux = ux * v0;
+        uy = uy * v0;
+        uz = uz * v0;
+        v0 = dx + ux;
+        v1 = dy + uy;
+        v2 = dz + uz;
+        v3 = v0 + ux;
+        v4 = v1 + uy;
+        const double v5 = v2 + uz;
+        if (v3 <= 1.0 && v4 <= 1.0 && v5 <= 1.0 && -v3 <= 1.0 && -v4 <= 1.0 && -v5 <= 1.0) {
+            pos[0] = v3;
+            pos[1] = v4;
+            pos[2] = v5;
+            cabana_weight(acc, weight * qsp, ux, uy, uz, v0, v1, v2);
+            return kCabDone;
+        }
+        streak[0] = ux;
+        streak[1] = uy;
+        streak[2] = uz;
+    }
+    double s_midx = pos[0], s_midy = pos[1], s_midz = pos[2];
+    double s_dispx = streak[0], s_dispy = streak[1], s_dispz = streak[2];
+    const double s_dir[3] = {(s_dispx > 0) ? 1.0 : -1.0, (s_dispy > 0) ? 1.0 : -1.0, (s_dispz > 0) ? 1.0 : -1.0};
+    const double v0 = (s_dispx == 0) ? 3.4e38 : (s_dir[0] - s_midx) / s_dispx;
+    const double v1 = (s_dispy == 0) ? 3.4e38 : (s_dir[1] - s_midy) / s_dispy;
+    const double v2 = (s_dispz == 0) ? 3.4e38 : (s_dir[2] - s_midz) / s_dispz;
+    double v3 = 2;
+    int axis = 3;
+    if (v0 < v3) v3 = v0, axis = 0;
+    if (v1 < v3) v3 = v1, axis = 1;
+    if (v2 < v3) v3 = v2, axis = 2;
+    v3 *= 0.5;
+    s_dispx *= v3;
+    s_dispy *= v3;
+    s_dispz *= v3;
+    s_midx += s_dispx;
+    s_midy += s_dispy;
+    s_midz += s_dispz;
+    cabana_weight(acc, weight * qsp, s_dispx, s_dispy, s_dispz, s_midx, s_midy, s_midz);
+    streak[0] -= s_dispx;
+    streak[1] -= s_dispy;
+    streak[2] -= s_dispz;
+    pos[0] += (s_dispx + s_dispx);
+    pos[1] += (s_dispy + s_dispy);
+    pos[2] += (s_dispz + s_dispz);
+    if (axis != 3) {
+        const double d = s_dir[axis];
+        pos[axis] = d;
+        int face = axis;
+        if (d > 0) face += 3;
+        *cell = c2c6[face];
+        pos[axis] = -d;
+        return kCabMove;
+    }
+    return kCabDone;
+}
+
+}  // namespace
+
+// The move_deposit loop of apps_codegen/app_cabanapic_cg/seq/move_deposit_kernel_loop.hpp over
+// particles [0, n) in order (pos/vel/streak AoS, 3 per particle; interp 18 and acc 12 per cell;
+// c2c6 the six face neighbours {x-, y-, z-, x+, y+, z+}). acc is accumulated, not cleared. Returns
+// the total evaluations; hops (nullable) receives them per particle.
+int64_t orc_cabana_move_deposit(int64_t n, const int* c2c6, const double* interp, double* acc, double* pos,
+                                double* vel, double* streak, const double* weight, int* p2c, double qsp,
+                                double qdt_2mc, const double* cdt_d, int* hops) {
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (p2c[i] == kMaxCell) {
+            if (hops) hops[i] = 0;
+            continue;
+        }
+        bool first = true;
+        int h = 0;
+        for (;;) {
+            const int c = p2c[i];
+            const int r = cabana_eval(first, vel + 3 * i, pos + 3 * i, streak + 3 * i, weight[i], interp + 18 * (int64_t)c,
+                                      acc + 12 * (int64_t)c, c2c6 + 6 * (int64_t)c, p2c + i, qsp, qdt_2mc, cdt_d);
+            first = false;
+            h++;
+            if (r != kCabMove) break;
+        }
+        if (hops) hops[i] = h;
+        total += h;
+    }
+    return total;
+}
+
 }  // extern "C"

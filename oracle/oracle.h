@@ -83,6 +83,15 @@ void orc_boundary_potential(int64_t n, const int* type, double wall, double* bnd
 /* OPP_MAX fold in index order (pinned: reference loop fixture). */
 double orc_max_fold(int64_t n, const double* x, int abs_values, double init);
 
+/* CabanaPIC move_deposit loop over particles [0, n) in order (apps/app_cabanapic/kernels.h:109-353;
+ * seq loop apps_codegen/app_cabanapic_cg/seq/move_deposit_kernel_loop.hpp): Boris push on the first
+ * evaluation, then streak by streak with the current of each streak added to acc (12 per cell) and
+ * a hop through the six face neighbours c2c6 (x-, y-, z-, x+, y+, z+). Returns total evaluations
+ * (pinned: reference fixtures). */
+int64_t orc_cabana_move_deposit(int64_t n, const int* c2c6, const double* interp, double* acc, double* pos,
+                                double* vel, double* streak, const double* weight, int* p2c, double qsp,
+                                double qdt_2mc, const double* cdt_d, int* hops);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,8 @@ _SIGS = {
     "orc_electric_field": (None, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "orc_boundary_potential": (None, [c_int64, c_void_p, c_double, c_void_p]),
     "orc_max_fold": (c_double, [c_int64, c_void_p, c_int, c_double]),
+    "orc_cabana_move_deposit": (c_int64, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_double, c_double, c_void_p, c_void_p]),
 }
 
 _lib = None
@@ -214,6 +216,17 @@ class FempicLoop(FempicState):
         self.deposit()
         mx = self.field_side(spwt)
         return before, self.n, int(hops.sum()), int(hops.max()) if hops.size else 0, mx
+
+
+def cabana_move_deposit(c2c6, interp, acc, pos, vel, streak, weight, p2c, qsp, qdt_2mc, cdt_d):
+    """CabanaPIC move_deposit over every particle (AoS arrays, updated in place); returns the
+    per-particle evaluation counts."""
+    n = p2c.shape[0]
+    hops = np.zeros(n, np.int32)
+    cd = np.ascontiguousarray(cdt_d, np.float64)
+    lib().orc_cabana_move_deposit(n, _p(c2c6), _p(interp), _p(acc), _p(pos), _p(vel), _p(streak), _p(weight),
+                                  _p(p2c), qsp, qdt_2mc, _p(cd), _p(hops))
+    return hops
 
 
 def dh_overlay_bruteforce(mesh, bmin, bmax, gs, threads=0):
