@@ -339,6 +339,37 @@ int oppx_max_reduce(int64_t n, const double* values, int abs_values, double init
                     size_t scratch_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------------
+ * CabanaPIC move with per-hop current deposit (device)
+ * ------------------------------------------------------------------------------------- */
+
+/* CabanaPIC particle state (device pointers, SoA with `stride`); positions are cell-local in [-1, 1]. */
+typedef struct oppx_cabana_parts {
+    int64_t stride;
+    double* pos;               /* [3*stride] p_pos */
+    double* vel;               /* [3*stride] p_vel */
+    double* streak;            /* [3*stride] p_streak_mid */
+    const double* weight;      /* [stride]   p_weight */
+    int32_t* p2c;              /* [stride] */
+} oppx_cabana_parts;
+
+/* Structured-mesh data the move reads (device, AoS). */
+typedef struct oppx_cabana_mesh {
+    int64_t n_cells;
+    const int32_t* c2c6;       /* [n_cells][6] face neighbours x-, y-, z-, x+, y+, z+ (c2c_non_ghost_map) */
+    const double* interp;      /* [n_cells][18] c_interp */
+} oppx_cabana_mesh;
+
+/* opp_particle_move__move_deposit_kernel (apps/app_cabanapic/kernels.h:109-353; loops
+ * apps_codegen/app_cabanapic_cg/{seq,hip}/move_deposit_kernel_loop.hpp) over counters->size
+ * particles: Boris push on the first evaluation, then streak by streak with each streak's current
+ * added to acc ([n_cells][12], accumulated, not cleared) and a hop to the face neighbour. cdt_d is a
+ * host array of 3. The counters' hop statistics are reset first; `hops` (nullable) receives the
+ * evaluations per particle; a neighbour index out of range or OPPX_MAX_HOPS counts in faults. */
+int oppx_cabana_move_deposit(int64_t n_max, const oppx_cabana_parts* parts, const oppx_cabana_mesh* mesh, double* acc,
+                             double qsp, double qdt_2mc, const double cdt_d[3], oppx_move_counters* counters,
+                             int32_t* hops, void* stream);
+
+/* ---------------------------------------------------------------------------------------
  * NESO-advection hot path (device)
  * ------------------------------------------------------------------------------------- */
 

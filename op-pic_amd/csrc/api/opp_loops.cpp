@@ -482,6 +482,43 @@ void opp_particle_move__move_kernel(opp_set set, opp_map c2c_map, opp_map p2c_ma
     opp_profiler->end("move_kernel");
 }
 
+// =============================================================================================
+// CabanaPIC
+// =============================================================================================
+
+// apps_codegen/app_cabanapic_cg/hip/move_deposit_kernel_loop.hpp; elemental apps/app_cabanapic/kernels.h:109-353.
+// c2c_map is the six-face non-ghost neighbour map (x-, y-, z-, x+, y+, z+).
+void opp_particle_move__move_deposit_kernel(opp_set set, opp_map c2c_map, opp_map p2c_map, opp_arg arg0, opp_arg arg1,
+                                            opp_arg arg2, opp_arg arg3, opp_arg arg4, opp_arg arg5) {
+    opp_profiler->start("move_deposit_kernel");
+    opp_arg args[6] = {arg0, arg1, arg2, arg3, arg4, arg5};
+    args_to_device(6, args);
+    if (p2c_map->p2c_dat->dirty_hd == Dirty::Device) opp_upload_dat(p2c_map->p2c_dat);
+    if (c2c_map->dim != 6) opp_abort("move_deposit: expects the six-face neighbour map");
+    opp_init_particle_move(set, 6, args);
+    sync_size_to_device(set);
+    int bytes = 0;
+    const char* cd = opp_const_data("CONST_cdt_d", &bytes);
+    if (!cd || bytes < 3 * (int)sizeof(double)) opp_abort("constant not declared: CONST_cdt_d");
+    double cdt_d[3];
+    memcpy(cdt_d, cd, sizeof(cdt_d));
+    oppx_cabana_parts parts{set->set_capacity, (double*)arg1.dat->data_d, (double*)arg0.dat->data_d,
+                            (double*)arg2.dat->data_d, (const double*)arg3.dat->data_d,
+                            (int32_t*)p2c_map->p2c_dat->data_d};
+    oppx_cabana_mesh mesh{c2c_map->from->size, c2c_map->map_d, (const double*)arg4.dat->data_d};
+    check(oppx_cabana_move_deposit(set->size, &parts, &mesh, (double*)arg5.dat->data_d, const_real("CONST_qsp"),
+                                   const_real("CONST_qdt_2mc"), cdt_d, pdev(set).counters, nullptr, nullptr),
+          "move_deposit_kernel");
+    args_written_on_device(6, args);
+    p2c_map->p2c_dat->dirty_hd = Dirty::Host;
+    oppx_move_counters c;
+    read_counters(set, &c);
+    if (c.faults) opp_abort("particle move: " + std::to_string(c.faults) + " walks hit an invalid cell or the hop cap");
+    OPP_move_max_hops = c.max_hops;
+    opp_finalize_particle_move(set);
+    opp_profiler->end("move_deposit_kernel");
+}
+
 // NESO's direct-hop setup (advec_opp.cpp:43): the periodic NESO walk does not use an overlay here
 void opp_init_direct_hop_cg(double grid_spacing, const opp_dat c_gbl_id, const opp::BoundingBox& b_box,
                             opp_map c2c_map, opp_map p2c_map, opp_arg arg0, opp_arg arg1, opp_arg arg2) {

@@ -1,0 +1,321 @@
+// CabanaPIC particle move with per-hop current deposit on gfx950 (apps/app_cabanapic/kernels.h:
+// 109-353; generated loop apps_codegen/app_cabanapic_cg/hip/move_deposit_kernel_loop.hpp): on the
+// first evaluation a Boris push and the streak midpoint test; then streak by streak, each streak's
+// current weighted into the 12 accumulator entries of the cell it crosses and the particle handed to
+// the face neighbour, until its track ends inside a cell.
+//
+// One lane per particle on a resident grid that strides over the particle range. Deposits go into
+// a workgroup LDS table keyed by cell (ds_add_f64), flushed with one device-scope atomic per
+// (cell, entry) after every tile: particles of one cell share a block, so this removes most of the
+// global atomics. The arithmetic is the reference GCC build's, FMA for FMA (the file is built with
+// -ffp-contract=off): the interpolated fields and the Boris cross products are FMA chains, the
+// half-advance products stay unfused (their second use lies past the errno branch GCC places around
+// sqrt), v5 = q*ux*uy*uz/3 is fused into all twelve accumulator terms and v1 = v4*d into both of
+// its uses in CALC_J.
+
+#include "oppx_internal.h"
+
+namespace oppx {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kCabTable = 128;  // cells per workgroup table (a 256-particle tile spans a few)
+constexpr int kCabProbe = 8;
+
+struct CabArgs {
+    int64_t n_cells;
+    const int32_t* __restrict__ c2c6;
+    const double* __restrict__ interp;
+    double* __restrict__ acc;
+    int64_t stride;
+    double* __restrict__ pos;
+    double* __restrict__ vel;
+    double* __restrict__ streak;
+    const double* __restrict__ weight;
+    int32_t* __restrict__ p2c;
+    double qsp, qdt_2mc, cdx, cdy, cdz;
+    int64_t n_max;
+    oppx_move_counters* ctr;
+    int32_t* __restrict__ hops;
+};
+
+struct CabTable {
+    int key[kCabTable];
+    double val[kCabTable][12];
+};
+
+__device__ __forceinline__ void cab_table_clear(CabTable& T) {
+    for (int q = threadIdx.x; q < kCabTable; q += blockDim.x) {
+        T.key[q] = -1;
+#pragma unroll
+        for (int j = 0; j < 12; j++) T.val[q][j] = 0.0;
+    }
+}
+
+// The 12 values of one streak go to the cell's table entry, or straight to global memory when no
+// entry is free within kCabProbe probes.
+__device__ __forceinline__ void cab_table_add(CabTable& T, double* __restrict__ acc, int cell, const double* v) {
+    uint32_t h = ((uint32_t)cell * 2654435761u) >> 25;  // 7 bits
+    for (int p = 0; p < kCabProbe; p++) {
+        int k = T.key[h];
+        if (k == -1) {
+            const int prev = atomicCAS(&T.key[h], -1, cell);
+            k = prev == -1 ? cell : prev;
+        }
+        if (k == cell) {
+#pragma unroll
+            for (int j = 0; j < 12; j++) atomicAdd(&T.val[h][j], v[j]);
+            return;
+        }
+        h = (h + 1) & (kCabTable - 1);
+    }
+#pragma unroll
+    for (int j = 0; j < 12; j++) atomicAdd(acc + 12 * (int64_t)cell + j, v[j]);
+}
+
+__device__ __forceinline__ void cab_table_flush(CabTable& T, double* __restrict__ acc) {
+    for (int e = threadIdx.x; e < kCabTable * 12; e += blockDim.x) {
+        const int q = e / 12, j = e - 12 * q;
+        const int k = T.key[q];
+        if (k >= 0) atomicAdd(acc + 12 * (int64_t)k + j, T.val[q][j]);
+    }
+}
+
+// CALC_J(u, dA, dB) of apps/app_cabanapic/kernels.h:11-25, with v5 = w / 3 fused into its four uses.
+__device__ __forceinline__ void cab_calc_j(double q, double w, double u, double dA, double dB, double* out) {
+    const double third = 1.0 / 3.0;
+    const double v4 = q * u;
+    double v0 = fma(-v4, dA, v4);
+    double v1 = fma(v4, dA, v4);
+    const double p = 1.0 + dB;
+    const double v2 = v0 * p;
+    const double v3 = v1 * p;
+    const double m = 1.0 - dB;
+    v0 = v0 * m;
+    v1 = v1 * m;
+    out[0] = fma(w, third, v0);
+    out[1] = fma(-w, third, v1);
+    out[2] = fma(-w, third, v2);
+    out[3] = fma(w, third, v3);
+}
+
+// weight_current_to_accumulator_kernel (kernels.h:109-136) into v[12]: jfx 0..3, jfy 4..7, jfz 8..11
+__device__ __forceinline__ void cab_weight(double q, double ux, double uy, double uz, double dx, double dy, double dz,
+                                           double* v) {
+    const double w = q * ux * uy * uz;
+    cab_calc_j(q, w, ux, dy, dz, v);
+    cab_calc_j(q, w, uy, dz, dx, v + 4);
+    cab_calc_j(q, w, uz, dx, dy, v + 8);
+}
+
+__global__ void __launch_bounds__(kBlock) cabana_move_deposit_kernel(CabArgs P) {
+    __shared__ CabTable T;
+    int64_t n = P.ctr->size;
+    n = n < 0 ? 0 : (n > P.n_max ? P.n_max : n);
+    const int64_t S = P.stride;
+    long long hsum = 0;
+    int hmax = 0, nfault = 0;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
+        cab_table_clear(T);
+        __syncthreads();
+        const int64_t i = base + threadIdx.x;
+        int h = 0;
+        if (i < n) {
+            int cell = P.p2c[i];
+            if (cell != kMaxCell) {
+                double px = P.pos[i], py = P.pos[S + i], pz = P.pos[2 * S + i];
+                double sx = P.streak[i], sy = P.streak[S + i], sz = P.streak[2 * S + i];
+                double ux0 = P.vel[i], uy0 = P.vel[S + i], uz0 = P.vel[2 * S + i];
+                const double q = P.weight[i] * P.qsp;
+                bool fault = !((unsigned)cell < (unsigned)P.n_cells), done = false;
+                double v[12];
+                // first evaluation (OPP_DO_ONCE): Boris push, then either a move inside the cell
+                // or the first streak
+                if (!fault) {
+                    const double* ip = P.interp + 18 * (int64_t)cell;
+                    const double ex = ip[0], dexdy = ip[1], dexdz = ip[2], d2exdydz = ip[3];
+                    const double ey = ip[4], deydz = ip[5], deydx = ip[6], d2eydzdx = ip[7];
+                    const double ez = ip[8], dezdx = ip[9], dezdy = ip[10], d2ezdxdy = ip[11];
+                    const double dx = px, dy = py, dz = pz;
+                    const double tx = fma(dz, fma(dy, d2exdydz, dexdz), fma(dy, dexdy, ex));
+                    const double ty = fma(dx, fma(dz, d2eydzdx, deydx), fma(dz, deydz, ey));
+                    const double tz = fma(dy, fma(dx, d2ezdxdy, dezdy), fma(dx, dezdx, ez));
+                    const double cbx = fma(dx, ip[13], ip[12]);
+                    const double cby = fma(dy, ip[15], ip[14]);
+                    const double cbz = fma(dz, ip[17], ip[16]);
+                    const double hax = P.qdt_2mc * tx, hay = P.qdt_2mc * ty, haz = P.qdt_2mc * tz;
+                    double ux = ux0 + hax, uy = uy0 + hay, uz = uz0 + haz;
+                    double v0 = P.qdt_2mc / sqrt(1.0 + fma(ux, ux, fma(uy, uy, uz * uz)));
+                    double v1 = fma(cbx, cbx, fma(cby, cby, cbz * cbz));
+                    double v2 = (v0 * v0) * v1;
+                    double v3 = v0 * fma(v2, fma(v2, 2.0 / 15.0, 1.0 / 3.0), 1.0);
+                    double v4 = v3 / fma(v1, v3 * v3, 1.0);
+                    v4 = v4 + v4;
+                    v0 = fma(v3, fma(uy, cbz, -(uz * cby)), ux);
+                    v1 = fma(v3, fma(uz, cbx, -(ux * cbz)), uy);
+                    v2 = fma(v3, fma(ux, cby, -(uy * cbx)), uz);
+                    ux = fma(v4, fma(v1, cbz, -(v2 * cby)), ux);
+                    uy = fma(v4, fma(v2, cbx, -(v0 * cbz)), uy);
+                    uz = fma(v4, fma(v0, cby, -(v1 * cbx)), uz);
+                    ux = ux + hax;
+                    uy = uy + hay;
+                    uz = uz + haz;
+                    P.vel[i] = ux;
+                    P.vel[S + i] = uy;
+                    P.vel[2 * S + i] = uz;
+                    v0 = 1.0 / sqrt(1.0 + fma(ux, ux, fma(uy, uy, uz * uz)));
+                    ux = (ux * P.cdx) * v0;
+                    uy = (uy * P.cdy) * v0;
+                    uz = (uz * P.cdz) * v0;
+                    v0 = dx + ux;
+                    v1 = dy + uy;
+                    v2 = dz + uz;
+                    v3 = v0 + ux;
+                    v4 = v1 + uy;
+                    const double v5 = v2 + uz;
+                    h = 1;
+                    if (v3 <= 1.0 && v4 <= 1.0 && v5 <= 1.0 && -v3 <= 1.0 && -v4 <= 1.0 && -v5 <= 1.0) {
+                        px = v3;
+                        py = v4;
+                        pz = v5;
+                        cab_weight(q, ux, uy, uz, v0, v1, v2, v);
+                        cab_table_add(T, P.acc, cell, v);
+                        done = true;
+                    } else {
+                        sx = ux;
+                        sy = uy;
+                        sz = uz;
+                    }
+                }
+                // streaks: each evaluation deposits one streak and hops across the face it ends on
+                bool first_streak = true;
+                while (!done && !fault) {
+                    if (!first_streak) h++;
+                    first_streak = false;
+                    const double dirx = (sx > 0) ? 1.0 : -1.0, diry = (sy > 0) ? 1.0 : -1.0, dirz = (sz > 0) ? 1.0 : -1.0;
+                    const double f0 = (sx == 0) ? 3.4e38 : (dirx - px) / sx;
+                    const double f1 = (sy == 0) ? 3.4e38 : (diry - py) / sy;
+                    const double f2 = (sz == 0) ? 3.4e38 : (dirz - pz) / sz;
+                    double f = 2;
+                    int axis = 3;
+                    if (f0 < f) f = f0, axis = 0;
+                    if (f1 < f) f = f1, axis = 1;
+                    if (f2 < f) f = f2, axis = 2;
+                    f *= 0.5;
+                    const double dsx = sx * f, dsy = sy * f, dsz = sz * f;
+                    const double mx = px + dsx, my = py + dsy, mz = pz + dsz;
+                    cab_weight(q, dsx, dsy, dsz, mx, my, mz, v);
+                    cab_table_add(T, P.acc, cell, v);
+                    sx -= dsx;
+                    sy -= dsy;
+                    sz -= dsz;
+                    px += (dsx + dsx);
+                    py += (dsy + dsy);
+                    pz += (dsz + dsz);
+                    if (axis == 3) break;
+                    const double d = axis == 0 ? dirx : (axis == 1 ? diry : dirz);
+                    const int face = axis + (d > 0 ? 3 : 0);
+                    const int nb = P.c2c6[6 * (int64_t)cell + face];
+                    // the entry point is exactly on the opposite face of the neighbour
+                    if (axis == 0) px = -d;
+                    if (axis == 1) py = -d;
+                    if (axis == 2) pz = -d;
+                    if (!((unsigned)nb < (unsigned)P.n_cells) || h + 1 >= OPPX_MAX_HOPS) {
+                        fault = true;
+                        break;
+                    }
+                    cell = nb;
+                }
+                if (fault) {
+                    nfault++;
+                } else {
+                    P.pos[i] = px;
+                    P.pos[S + i] = py;
+                    P.pos[2 * S + i] = pz;
+                    P.streak[i] = sx;
+                    P.streak[S + i] = sy;
+                    P.streak[2 * S + i] = sz;
+                    P.p2c[i] = cell;
+                }
+            }
+            if (P.hops) P.hops[i] = h;
+        }
+        hsum += h;
+        hmax = hmax > h ? hmax : h;
+        __syncthreads();
+        cab_table_flush(T, P.acc);
+        __syncthreads();
+    }
+    __shared__ long long s_sum[kBlock / kWave];
+    __shared__ int s_max[kBlock / kWave], s_flt[kBlock / kWave];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const long long hs = wave_sum_i64(hsum);
+    const int hm = wave_max_i32(hmax), nf = wave_sum_i32(nfault);
+    if (lane == 0) { s_sum[w] = hs; s_max[w] = hm; s_flt[w] = nf; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        int m = 0, f = 0;
+        for (int q = 0; q < kBlock / kWave; q++) { t += s_sum[q]; m = m > s_max[q] ? m : s_max[q]; f += s_flt[q]; }
+        if (t) atomicAdd((unsigned long long*)&P.ctr->total_hops, (unsigned long long)t);
+        if (m) atomicMax(&P.ctr->max_hops, m);
+        if (f) atomicAdd(&P.ctr->faults, f);
+    }
+}
+
+__global__ void cabana_reset_kernel(oppx_move_counters* ctr) {
+    ctr->removed = 0;
+    ctr->holes_low = 0;
+    ctr->max_hops = 0;
+    ctr->total_hops = 0;
+    ctr->faults = 0;
+}
+
+}  // namespace
+}  // namespace oppx
+
+using namespace oppx;
+
+extern "C" {
+
+int oppx_cabana_move_deposit(int64_t n_max, const oppx_cabana_parts* parts, const oppx_cabana_mesh* mesh, double* acc,
+                             double qsp, double qdt_2mc, const double cdt_d[3], oppx_move_counters* counters,
+                             int32_t* hops, void* stream) {
+    OPPX_ARG_CHECK(parts && mesh && acc && cdt_d && counters, "oppx_cabana_move_deposit: null pointer");
+    OPPX_ARG_CHECK(parts->pos && parts->vel && parts->streak && parts->weight && parts->p2c,
+                   "oppx_cabana_move_deposit: incomplete particle dats");
+    OPPX_ARG_CHECK(mesh->c2c6 && mesh->interp && mesh->n_cells > 0 && mesh->n_cells < kMaxCell,
+                   "oppx_cabana_move_deposit: bad mesh");
+    OPPX_ARG_CHECK(n_max >= 0 && n_max <= parts->stride, "oppx_cabana_move_deposit: n_max beyond capacity");
+    hipStream_t s = as_stream(stream);
+    cabana_reset_kernel<<<1, 1, 0, s>>>(counters);
+    OPPX_LAUNCH_CHECK("cabana_reset");
+    if (n_max == 0) return OPPX_OK;
+    CabArgs P{};
+    P.n_cells = mesh->n_cells;
+    P.c2c6 = mesh->c2c6;
+    P.interp = mesh->interp;
+    P.acc = acc;
+    P.stride = parts->stride;
+    P.pos = parts->pos;
+    P.vel = parts->vel;
+    P.streak = parts->streak;
+    P.weight = parts->weight;
+    P.p2c = parts->p2c;
+    P.qsp = qsp;
+    P.qdt_2mc = qdt_2mc;
+    P.cdx = cdt_d[0];
+    P.cdy = cdt_d[1];
+    P.cdz = cdt_d[2];
+    P.n_max = n_max;
+    P.ctr = counters;
+    P.hops = hops;
+    unsigned g = resident_blocks(reinterpret_cast<const void*>(cabana_move_deposit_kernel), kBlock);
+    const unsigned need = grid_for(n_max, kBlock);
+    if (g > need) g = need;
+    cabana_move_deposit_kernel<<<g, kBlock, 0, s>>>(P);
+    OPPX_LAUNCH_CHECK("cabana_move_deposit");
+    return OPPX_OK;
+}
+
+}  // extern "C"

@@ -455,6 +455,9 @@ void opp_par_loop_all__compute_node_charge_density_kernel(opp_set, opp_arg, opp_
 void opp_par_loop_all__compute_electric_field_kernel(opp_set, opp_arg, opp_arg, opp_arg, opp_arg, opp_arg, opp_arg);
 void opp_par_loop_all__get_max_cef_kernel(opp_set, opp_arg, opp_arg);
 void opp_par_loop_all__get_final_max_values_kernel(opp_set, opp_arg, opp_arg, opp_arg, opp_arg);
+// CabanaPIC (apps_codegen/app_cabanapic_cg/cabana_opp.cpp)
+void opp_particle_move__move_deposit_kernel(opp_set, opp_map, opp_map, opp_arg, opp_arg, opp_arg, opp_arg, opp_arg,
+                                            opp_arg);
 // NESO-advection
 void opp_par_loop_all__update_pos_kernel(opp_set, opp_arg, opp_arg);
 void opp_particle_move__move_kernel(opp_set, opp_map, opp_map, opp_arg, opp_arg, opp_arg);

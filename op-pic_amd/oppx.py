@@ -69,6 +69,15 @@ class Inlet(Structure):
                 ("node_pos", c_void_p)]
 
 
+class CabanaParts(Structure):
+    _fields_ = [("stride", c_int64), ("pos", c_void_p), ("vel", c_void_p), ("streak", c_void_p), ("weight", c_void_p),
+                ("p2c", c_void_p)]
+
+
+class CabanaMesh(Structure):
+    _fields_ = [("n_cells", c_int64), ("c2c6", c_void_p), ("interp", c_void_p)]
+
+
 class DatDesc(Structure):
     _fields_ = [("base", c_void_p), ("dim", c_int32), ("elem_bytes", c_int32)]
 
@@ -138,6 +147,8 @@ _SIGS = {
                                            c_void_p]),
     "oppx_fempic_boundary_potential": (c_int, [c_int64, c_void_p, c_double, c_void_p, c_void_p]),
     "oppx_max_scratch_bytes": (c_size_t, [c_int64]),
+    "oppx_cabana_move_deposit": (c_int, [c_int64, POINTER(CabanaParts), POINTER(CabanaMesh), c_void_p, c_double,
+                                         c_double, POINTER(c_double), c_void_p, c_void_p, c_void_p]),
     "oppx_max_reduce": (c_int, [c_int64, c_void_p, c_int, c_double, c_void_p, c_void_p, c_size_t, c_void_p]),
     "oppx_neso_step_grid": (c_int, [c_int64, POINTER(NesoParts), c_int, c_int, c_double, c_double, c_double, c_double,
                                     c_int, c_void_p, c_void_p, c_void_p]),

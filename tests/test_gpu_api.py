@@ -183,3 +183,45 @@ def test_fempic_loop_app_matches_reference(name):
                            ("ef", np.float64)):
                 assert np.array_equal(rd(f"{t}_{k}", dt_), g[f"out_{t}_{k}"]), f"{t} {k}"
             assert charge_close(rd(f"{t}_charge", np.float64), g[f"out_{t}_charge"]), f"{t} charge"
+
+
+@pytest.mark.parametrize("name", ["cabana_small.npz", "cabana_grid16.npz"])
+def test_cabana_api_app_matches_reference(name):
+    """CabanaPIC's move_deposit through the drop-in: particles bit-exact (digests for the larger
+    case), accumulators within 1e-12 relative, per-step max hops as the reference's."""
+    import hashlib
+
+    from test_oracle_cabana import cabana_case
+
+    g = np.load(golden_path(name), allow_pickle=False)
+    d = cabana_case(g)
+    with tempfile.TemporaryDirectory() as td:
+        ind, outd = os.path.join(td, "in"), os.path.join(td, "out")
+        os.makedirs(ind)
+        os.makedirs(outd)
+        for k, fn in (("c2c6", "c2ngc"), ("interp", "c_interp"), ("pos", "p_pos"), ("vel", "p_vel"),
+                      ("weight", "p_weight"), ("p2c", "p2c")):
+            np.ascontiguousarray(d[k]).tofile(os.path.join(ind, fn + ".bin"))
+        cd = g["meta_cdt_d"]
+        lines = [f"INT ncells = {d['n_cells']}", f"INT nparts = {d['p2c'].size}", f"INT num_steps = {int(g['meta_steps'])}",
+                 f"REAL qsp = {float(g['meta_qsp'])!r}", f"REAL qdt_half_mc = {float(g['meta_qdt_2mc'])!r}",
+                 f"REAL cdt_dx = {float(cd[0])!r}", f"REAL cdt_dy = {float(cd[1])!r}", f"REAL cdt_dz = {float(cd[2])!r}",
+                 "REAL opp_allocation_multiple = 1.1", "BOOL opp_profile_all = false", "STRING opp_fill = HoleFill_All",
+                 "INT opp_fill_period = 4", "BOOL opp_global_move = false"]
+        par = os.path.join(td, "case.param")
+        with open(par, "w") as fh:
+            fh.write("\n".join(lines) + "\n")
+        run_app(app("cabana_api_app"), par, ind, outd)
+        rd = lambda k, t: np.fromfile(os.path.join(outd, k + ".bin"), dtype=t)
+        st = np.loadtxt(os.path.join(outd, "steps.txt"), ndmin=2).astype(np.int64)
+        ref = g["out_steps_iter_size_hops_maxhops"]
+        assert np.array_equal(st, ref[:, [0, 1, 3]])
+        for t in ("s1", "final"):
+            r = g[f"out_{t}_acc"]
+            assert np.abs(rd(f"{t}_acc", np.float64) - r).max() <= DEPOSIT_RTOL * np.abs(r).max(), f"{t} acc"
+            for k, dt_ in (("pos", np.float64), ("vel", np.float64), ("streak", np.float64), ("p2c", np.int32)):
+                a = rd(f"{t}_{k}", dt_)
+                if f"out_{t}_{k}" in g:
+                    assert np.array_equal(a, g[f"out_{t}_{k}"]), f"{t} {k}"
+                else:
+                    assert hashlib.sha256(a.tobytes()).hexdigest() == str(g[f"sha_{t}_{k}"]), f"{t} {k}"
