@@ -341,6 +341,146 @@ def bench_neso(args):
     }
 
 
+def cabana_deck(nx=40, ny=40, nz=60, nppc=750, v0=0.0866025403784439, c=1.0, qsp=-1.0, me=1.0, grid_len=1.0):
+    """Constants of the CabanaPIC deck (apps/app_cabanapic/cabana_defs.h:320-380) for the reference's
+    two-stream configuration (apps/app_cabanapic/configs/cabana.param)."""
+    gam = 1.0 / math.sqrt(1.0 - v0 * v0)
+    lx, ly, lz = grid_len, 0.628318530717959 * (gam * math.sqrt(gam)), grid_len
+    dx, dy, dz = lx / nx, ly / ny, lz / nz
+    w1 = (nx / lx) ** 2 + (ny / ly) ** 2 + (nz / lz) ** 2
+    dt = 0.99 * math.sqrt(1 / w1) / c
+    we = 2.0 * lx * ly * lz / (nx * ny * nz * nppc)
+    return dict(nx=nx, ny=ny, nz=nz, nppc=nppc, v0=v0, gam=gam, dt=dt, qsp=qsp, qdt_2mc=qsp * dt / (2 * me * c),
+                cdt_d=(c * dt / dx, c * dt / dy, c * dt / dz), we=we)
+
+
+def cabana_two_stream(deck):
+    """Two-stream particles as apps/app_cabanapic/cabana_misc.h:173-240 places them: per cell, pairs of
+    counter-streaming particles at cell-local (0, x, 0), velocity +-v0*gam*(1 + na*sign) along x."""
+    nx, ny, nz, nppc = deck["nx"], deck["ny"], deck["nz"], deck["nppc"]
+    nc = nx * ny * nz
+    cell = np.repeat(np.arange(nc, dtype=np.int64), nppc)
+    k = np.tile(np.arange(nppc, dtype=np.int64), nc)
+    pi2 = cell * nppc + k
+    sign = np.where(pi2 % 2 == 0, 1.0, -1.0)
+    dxp = 2.0 / nppc
+    x = ((2 * (pi2 // 2)) % nppc) * dxp + 0.5 * dxp - 1.0
+    na = 0.0001 * np.sin(2.0 * 3.1415926 * ((x + 1.0 + cell * 2) / (2 * ny)))
+    n = cell.size
+    pos = np.zeros((n, 3))
+    pos[:, 1] = x
+    vel = np.zeros((n, 3))
+    vel[:, 0] = sign * deck["v0"] * deck["gam"] * (1.0 + na * sign)
+    return pos, vel, np.full(n, deck["we"]), cell.astype(np.int32)
+
+
+def bench_cabana(args):
+    """CabanaPIC move with per-hop current deposit on the reference deck: 40x40x60 periodic cells, 750
+    particles per cell (72 M), two-stream velocities, fixed synthetic interpolated fields. A step is
+    the accumulator reset plus the move_deposit pass."""
+    import ctypes
+
+    import torch
+
+    import oppx
+
+    deck = cabana_deck()
+    nx, ny, nz = deck["nx"], deck["ny"], deck["nz"]
+    nc = nx * ny * nz
+    t0 = time.time()
+    pos, vel, weight, p2c = cabana_two_stream(deck)
+    i = np.arange(nc)
+    ix, iy, iz = i % nx, (i // nx) % ny, i // (nx * ny)
+    cid = lambda a, b, c_: (a % nx) + nx * ((b % ny) + ny * (c_ % nz))
+    c2c6 = np.stack([cid(ix - 1, iy, iz), cid(ix, iy - 1, iz), cid(ix, iy, iz - 1), cid(ix + 1, iy, iz),
+                     cid(ix, iy + 1, iz), cid(ix, iy, iz + 1)], axis=1).astype(np.int32)
+    rng = np.random.default_rng(7)
+    interp = rng.normal(size=(nc, 18)) * np.array([0.5, 0.2, 0.2, 0.1] * 3 + [0.5, 0.2] * 3)
+    n = p2c.size
+    log(f"cabana inputs in {time.time() - t0:.1f}s: {nc} cells, {n} particles")
+    t = lambda a, dt_: torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype=dt_)
+    d_pos, d_vel = t(pos.T, torch.float64), t(vel.T, torch.float64)
+    d_streak = torch.zeros((3, n), dtype=torch.float64, device="cuda")
+    d_w, d_p2c = t(weight, torch.float64), t(p2c, torch.int32)
+    d_c6, d_ip = t(c2c6, torch.int32), t(interp, torch.float64)
+    acc = torch.zeros((nc, 12), dtype=torch.float64, device="cuda")
+    ctr = torch.zeros(oppx.COUNTER_WORDS, dtype=torch.int32, device="cuda")
+    ctr[0] = n
+    parts = oppx.CabanaParts(n, d_pos.data_ptr(), d_vel.data_ptr(), d_streak.data_ptr(), d_w.data_ptr(),
+                             d_p2c.data_ptr())
+    mesh = oppx.CabanaMesh(nc, d_c6.data_ptr(), d_ip.data_ptr())
+    cd = (ctypes.c_double * 3)(*deck["cdt_d"])
+    stream = torch.cuda.current_stream().cuda_stream
+    L = oppx.lib()
+
+    def step():
+        acc.zero_()
+        oppx.check(L.oppx_cabana_move_deposit(n, ctypes.byref(parts), ctypes.byref(mesh), acc.data_ptr(), deck["qsp"],
+                                              deck["qdt_2mc"], cd, ctr.data_ptr(), None, stream), "cabana")
+
+    copy_gbs = copy_peak_gbs(torch)
+    log(f"device copy rate {copy_gbs:.0f} GB/s")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t1 = time.perf_counter()
+    for a, b in evs:
+        a.record()
+        step()
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    ms = np.array([a.elapsed_time(b) for a, b in evs])
+    c = ctr.cpu().numpy()
+    hops = int(c[4:6].view(np.int64)[0])
+    log(f"cabana: {1e3 * wall / args.steps:.3f} ms/step, p50 {np.percentile(ms, 50):.3f} ms, "
+        f"{hops / n:.3f} evaluations/particle in the last step, faults {int(c[6])}")
+    # R pos 24, vel 24, streak 24, weight 8, p2c 4; W pos 24, vel 24, streak 24, p2c 4 per particle;
+    # per cell: interp 144 R, acc 96 RW (zeroing included), c2c 24 R
+    alg = 160.0 * n + (144 + 96 + 96 + 24) * nc
+    achieved = alg / (ms.mean() * 1e-3) / 1e9
+    out = {
+        "metric": "CabanaPIC particle moves/s (Boris push + streak walk + current deposit)",
+        "value": n * args.steps / wall,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"CabanaPIC two-stream deck, {nx}x{ny}x{nz} periodic cells, {deck['nppc']} "
+                               f"particles/cell ({n}), fixed synthetic interpolated fields"},
+        "p50_step_ms": float(np.percentile(ms, 50)),
+        "p99_step_ms": float(np.percentile(ms, 99)),
+        "roofline": {"kernel": "cabana_move_deposit_kernel", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic("cabana", "cabana_move_deposit_kernel"), "measured_copy_gbs": copy_gbs},
+    }
+    if not args.no_cpu:
+        # the oracle (one host thread, particle order) on a bounded slice of the same particles
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        m = min(n, 2_000_000)
+        sl = slice(0, m)
+        hp, hv, hs = pos[sl].copy(), vel[sl].copy(), np.zeros((m, 3))
+        hacc = np.zeros((nc, 12))
+        hc = p2c[sl].copy()
+        t2 = time.perf_counter()
+        oracle.cabana_move_deposit(c2c6, interp, hacc, hp, hv, hs, weight[sl].copy(), hc, deck["qsp"],
+                                   deck["qdt_2mc"], deck["cdt_d"])
+        dt_cpu = time.perf_counter() - t2
+        out["cpu_baseline"] = {"value": m / dt_cpu, "unit": "particles/s", "cores": 1, "kind": "port",
+                               "sample": f"one step of the first {m} particles"}
+        log(f"cpu baseline {m / dt_cpu:.3e} particles/s (1 thread)")
+    return out
+
+
 def bench_serve(args):
     """Serving stream: Poisson arrivals of 10-step segments, sizes as the ensemble's plus 5% large
     requests, FP32 compensated deposit, admission by an HBM budget; continuous batching against the
@@ -419,7 +559,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tag", default="c2", help="workload tag of the profile summaries")
-    ap.add_argument("--workload", choices=["single", "ensemble", "serve", "neso"], default="single",
+    ap.add_argument("--workload", choices=["single", "ensemble", "serve", "neso", "cabana"], default="single",
                     help="single: the large single instance (the headline line); ensemble: the batched ensemble; "
                          "serve: the serving stream; neso: the NESO-advection pure move")
     ap.add_argument("--neso-nx", type=int, default=1024)
@@ -442,6 +582,9 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "cabana":
+        print(json.dumps(bench_cabana(args)), flush=True)
+        return
     if args.workload == "neso":
         print(json.dumps(bench_neso(args)), flush=True)
         return

@@ -7,7 +7,8 @@
 // One lane per particle on a resident grid that strides over the particle range. Deposits go into
 // a workgroup LDS table keyed by cell (ds_add_f64), flushed with one device-scope atomic per
 // (cell, entry) after every tile: particles of one cell share a block, so this removes most of the
-// global atomics. The arithmetic is the reference GCC build's, FMA for FMA (the file is built with
+// global atomics; a wave whose first evaluations all deposit into one cell sums them with shuffles
+// first. The arithmetic is the reference GCC build's, FMA for FMA (the file is built with
 // -ffp-contract=off): the interpolated fields and the Boris cross products are FMA chains, the
 // half-advance products stay unfused (their second use lies past the errno branch GCC places around
 // sqrt), v5 = q*ux*uy*uz/3 is fused into all twelve accumulator terms and v1 = v4*d into both of
@@ -19,7 +20,7 @@ namespace oppx {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kCabTable = 128;  // cells per workgroup table (a 256-particle tile spans a few)
+constexpr int kCabTable = 64;  // cells per workgroup table (a 256-particle tile spans a few)
 constexpr int kCabProbe = 8;
 
 struct CabArgs {
@@ -39,6 +40,11 @@ struct CabArgs {
     int32_t* __restrict__ hops;
 };
 
+template <typename T>
+__device__ __forceinline__ T ld_nt(const T* p) { return __builtin_nontemporal_load(p); }
+template <typename T>
+__device__ __forceinline__ void st_nt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 struct CabTable {
     int key[kCabTable];
     double val[kCabTable][12];
@@ -55,7 +61,7 @@ __device__ __forceinline__ void cab_table_clear(CabTable& T) {
 // The 12 values of one streak go to the cell's table entry, or straight to global memory when no
 // entry is free within kCabProbe probes.
 __device__ __forceinline__ void cab_table_add(CabTable& T, double* __restrict__ acc, int cell, const double* v) {
-    uint32_t h = ((uint32_t)cell * 2654435761u) >> 25;  // 7 bits
+    uint32_t h = ((uint32_t)cell * 2654435761u) >> 26;  // 6 bits
     for (int p = 0; p < kCabProbe; p++) {
         int k = T.key[h];
         if (k == -1) {
@@ -73,12 +79,19 @@ __device__ __forceinline__ void cab_table_add(CabTable& T, double* __restrict__ 
     for (int j = 0; j < 12; j++) atomicAdd(acc + 12 * (int64_t)cell + j, v[j]);
 }
 
+// Flushes the used entries and leaves the table cleared for the next tile (the caller separates the
+// tile's adds from the flush, and the flush from the next tile's adds, with barriers).
 __device__ __forceinline__ void cab_table_flush(CabTable& T, double* __restrict__ acc) {
     for (int e = threadIdx.x; e < kCabTable * 12; e += blockDim.x) {
         const int q = e / 12, j = e - 12 * q;
         const int k = T.key[q];
-        if (k >= 0) atomicAdd(acc + 12 * (int64_t)k + j, T.val[q][j]);
+        if (k >= 0) {
+            atomicAdd(acc + 12 * (int64_t)k + j, T.val[q][j]);
+            T.val[q][j] = 0.0;
+        }
     }
+    __syncthreads();  // every entry's values are read before its key is released
+    for (int q = threadIdx.x; q < kCabTable; q += blockDim.x) T.key[q] = -1;
 }
 
 // CALC_J(u, dA, dB) of apps/app_cabanapic/kernels.h:11-25, with v5 = w / 3 fused into its four uses.
@@ -115,18 +128,22 @@ __global__ void __launch_bounds__(kBlock) cabana_move_deposit_kernel(CabArgs P) 
     const int64_t S = P.stride;
     long long hsum = 0;
     int hmax = 0, nfault = 0;
+    cab_table_clear(T);
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
     for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
-        cab_table_clear(T);
-        __syncthreads();
         const int64_t i = base + threadIdx.x;
         int h = 0;
+        // the deposit of a lane's first evaluation waits for the whole wave (see below)
+        double v1st[12];
+        int c1st = -1;
         if (i < n) {
-            int cell = P.p2c[i];
+            int cell = ld_nt(P.p2c + i);
             if (cell != kMaxCell) {
-                double px = P.pos[i], py = P.pos[S + i], pz = P.pos[2 * S + i];
-                double sx = P.streak[i], sy = P.streak[S + i], sz = P.streak[2 * S + i];
-                double ux0 = P.vel[i], uy0 = P.vel[S + i], uz0 = P.vel[2 * S + i];
-                const double q = P.weight[i] * P.qsp;
+                double px = ld_nt(P.pos + i), py = ld_nt(P.pos + S + i), pz = ld_nt(P.pos + 2 * S + i);
+                double sx = ld_nt(P.streak + i), sy = ld_nt(P.streak + S + i), sz = ld_nt(P.streak + 2 * S + i);
+                double ux0 = ld_nt(P.vel + i), uy0 = ld_nt(P.vel + S + i), uz0 = ld_nt(P.vel + 2 * S + i);
+                const double q = ld_nt(P.weight + i) * P.qsp;
                 bool fault = !((unsigned)cell < (unsigned)P.n_cells), done = false;
                 double v[12];
                 // first evaluation (OPP_DO_ONCE): Boris push, then either a move inside the cell
@@ -160,9 +177,9 @@ __global__ void __launch_bounds__(kBlock) cabana_move_deposit_kernel(CabArgs P) 
                     ux = ux + hax;
                     uy = uy + hay;
                     uz = uz + haz;
-                    P.vel[i] = ux;
-                    P.vel[S + i] = uy;
-                    P.vel[2 * S + i] = uz;
+                    st_nt(P.vel + i, ux);
+                    st_nt(P.vel + S + i, uy);
+                    st_nt(P.vel + 2 * S + i, uz);
                     v0 = 1.0 / sqrt(1.0 + fma(ux, ux, fma(uy, uy, uz * uz)));
                     ux = (ux * P.cdx) * v0;
                     uy = (uy * P.cdy) * v0;
@@ -178,8 +195,8 @@ __global__ void __launch_bounds__(kBlock) cabana_move_deposit_kernel(CabArgs P) 
                         px = v3;
                         py = v4;
                         pz = v5;
-                        cab_weight(q, ux, uy, uz, v0, v1, v2, v);
-                        cab_table_add(T, P.acc, cell, v);
+                        cab_weight(q, ux, uy, uz, v0, v1, v2, v1st);
+                        c1st = cell;
                         done = true;
                     } else {
                         sx = ux;
@@ -191,7 +208,6 @@ __global__ void __launch_bounds__(kBlock) cabana_move_deposit_kernel(CabArgs P) 
                 bool first_streak = true;
                 while (!done && !fault) {
                     if (!first_streak) h++;
-                    first_streak = false;
                     const double dirx = (sx > 0) ? 1.0 : -1.0, diry = (sy > 0) ? 1.0 : -1.0, dirz = (sz > 0) ? 1.0 : -1.0;
                     const double f0 = (sx == 0) ? 3.4e38 : (dirx - px) / sx;
                     const double f1 = (sy == 0) ? 3.4e38 : (diry - py) / sy;
@@ -204,8 +220,14 @@ __global__ void __launch_bounds__(kBlock) cabana_move_deposit_kernel(CabArgs P) 
                     f *= 0.5;
                     const double dsx = sx * f, dsy = sy * f, dsz = sz * f;
                     const double mx = px + dsx, my = py + dsy, mz = pz + dsz;
-                    cab_weight(q, dsx, dsy, dsz, mx, my, mz, v);
-                    cab_table_add(T, P.acc, cell, v);
+                    if (first_streak) {  // still the first evaluation
+                        cab_weight(q, dsx, dsy, dsz, mx, my, mz, v1st);
+                        c1st = cell;
+                    } else {
+                        cab_weight(q, dsx, dsy, dsz, mx, my, mz, v);
+                        cab_table_add(T, P.acc, cell, v);
+                    }
+                    first_streak = false;
                     sx -= dsx;
                     sy -= dsy;
                     sz -= dsz;
@@ -229,16 +251,36 @@ __global__ void __launch_bounds__(kBlock) cabana_move_deposit_kernel(CabArgs P) 
                 if (fault) {
                     nfault++;
                 } else {
-                    P.pos[i] = px;
-                    P.pos[S + i] = py;
-                    P.pos[2 * S + i] = pz;
-                    P.streak[i] = sx;
-                    P.streak[S + i] = sy;
-                    P.streak[2 * S + i] = sz;
-                    P.p2c[i] = cell;
+                    st_nt(P.pos + i, px);
+                    st_nt(P.pos + S + i, py);
+                    st_nt(P.pos + 2 * S + i, pz);
+                    st_nt(P.streak + i, sx);
+                    st_nt(P.streak + S + i, sy);
+                    st_nt(P.streak + 2 * S + i, sz);
+                    st_nt(P.p2c + i, cell);
                 }
             }
             if (P.hops) P.hops[i] = h;
+        }
+        // First-evaluation deposits: in cell order nearly every wave's particles share one cell, and
+        // 64 lanes adding to the same 12 LDS words would serialise; such a wave sums its 12 values
+        // with a butterfly and one lane adds them. A wave spanning several cells adds lane by lane.
+        const unsigned long long has = __ballot(c1st >= 0);
+        if (has) {
+            const int lead = __shfl(c1st, __ffsll((long long)has) - 1);
+            if ((has & ~__ballot(c1st == lead)) == 0ull) {
+                double sum[12];
+#pragma unroll
+                for (int j = 0; j < 12; j++) {
+                    double x = c1st >= 0 ? v1st[j] : 0.0;
+#pragma unroll
+                    for (int o = 1; o < kWave; o <<= 1) x += __shfl_xor(x, o);
+                    sum[j] = x;
+                }
+                if (lane == 0) cab_table_add(T, P.acc, lead, sum);
+            } else if (c1st >= 0) {
+                cab_table_add(T, P.acc, c1st, v1st);
+            }
         }
         hsum += h;
         hmax = hmax > h ? hmax : h;
@@ -248,7 +290,7 @@ __global__ void __launch_bounds__(kBlock) cabana_move_deposit_kernel(CabArgs P) 
     }
     __shared__ long long s_sum[kBlock / kWave];
     __shared__ int s_max[kBlock / kWave], s_flt[kBlock / kWave];
-    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int w = threadIdx.x / kWave;
     const long long hs = wave_sum_i64(hsum);
     const int hm = wave_max_i32(hmax), nf = wave_sum_i32(nfault);
     if (lane == 0) { s_sum[w] = hs; s_max[w] = hm; s_flt[w] = nf; }
