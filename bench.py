@@ -57,6 +57,17 @@ def algorithmic_bytes(n_part, n_cells, n_nodes):
     return 136 * n_part + 192 * n_cells + 16 * n_nodes
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, budget_s):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -74,10 +85,66 @@ def cpu_baseline(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, budget_s):
         t_tot += time.perf_counter() - t0
         tot_parts += n_before
         steps += 1
-    return dict(value=tot_parts / t_tot, unit="particles/s", cores=threads, kind="port",
+    return dict(value=tot_parts / t_tot, unit="particles/s", cores=threads, kind="port", cpu=cpu_model(),
                 sample=f"oracle (C++/OpenMP restatement, {threads} threads) on every {stride}th particle of the same "
                        f"input ({n0} particles, the full {m['n_cells']}-tet mesh and overlay), {steps} steps, "
                        f"{t_tot:.1f}s")
+
+
+def cpu_baseline_reference(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, steps=10, stride=10):
+    """The reference's own OpenMP build (oracle/_ref/fempic_ref_omp, compiled from the reference
+    sources where they were available) on the host's cores: the full mesh, every `stride`-th particle
+    of the same input, `steps` steps of push + move (direct hop, overlay read from a file in the
+    reference's format, as opp_dh_data_generate = false does) + deposit + hole fill; only the
+    steps' loops are timed. None when the build is absent."""
+    import subprocess
+    import tempfile
+
+    import oppx
+
+    binary = os.path.join(ROOT, "oracle", "_ref", "fempic_ref_omp")
+    if not os.path.exists(binary):
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    sl = slice(0, None, stride)
+    with tempfile.TemporaryDirectory() as td:
+        ind, outd = os.path.join(td, "in"), os.path.join(td, "out")
+        os.makedirs(ind)
+        os.makedirs(outd)
+        for k in ("node_pos", "c2n", "c2c", "c_det", "c_vol"):
+            np.ascontiguousarray(m[k]).tofile(os.path.join(ind, k + ".bin"))
+        np.ascontiguousarray(ef).tofile(os.path.join(ind, "c_ef.bin"))
+        np.ascontiguousarray(pos[sl]).tofile(os.path.join(ind, "p_pos.bin"))
+        np.ascontiguousarray(vel[sl]).tofile(os.path.join(ind, "p_vel.bin"))
+        np.ascontiguousarray(p2c[sl], np.int32).tofile(os.path.join(ind, "p2c.bin"))
+        oppx.dh_overlay_save(os.path.join(td, oppx.dh_overlay_filename(m["n_cells"], gs)), ov)
+        n = int(p2c[sl].shape[0])
+        lines = [f"INT nnodes = {m['n_nodes']}", f"INT ncells = {m['n_cells']}", f"INT nparts = {n}",
+                 f"INT num_steps = {steps}", f"REAL dt = {dt!r}", f"REAL mass = {ms!r}", f"REAL charge = {q!r}",
+                 f"REAL grid_spacing = {gs!r}", f"REAL bminx = {float(bmin[0])!r}", f"REAL bminy = {float(bmin[1])!r}",
+                 f"REAL bminz = {float(bmin[2])!r}", f"REAL bmaxx = {float(bmax[0])!r}",
+                 f"REAL bmaxy = {float(bmax[1])!r}", f"REAL bmaxz = {float(bmax[2])!r}",
+                 "BOOL opp_global_move = true", "BOOL opp_dh_data_generate = false",
+                 f"STRING opp_dh_mesh_folder = {td}/", "REAL opp_allocation_multiple = 1.1",
+                 "BOOL opp_profile_all = false", "STRING opp_fill = HoleFill_All", "INT opp_fill_period = 4"]
+        par = os.path.join(td, "case.param")
+        with open(par, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+        try:
+            r = subprocess.run([binary, par, ind, outd], env=env, capture_output=True, text=True, timeout=240)
+        except subprocess.TimeoutExpired:
+            return None
+        if r.returncode != 0:
+            log(f"reference CPU baseline failed (rc {r.returncode}): {r.stderr[-300:]}")
+            return None
+        loop_s = float(open(os.path.join(outd, "loop_seconds.txt")).read())
+        st = np.loadtxt(os.path.join(outd, "steps.txt"), ndmin=2)
+        moved = float(st[:, 1].sum())
+    return dict(value=moved / loop_s, unit="particles/s", cores=threads, kind="reference", cpu=cpu_model(),
+                sample=f"the reference's OpenMP build ({threads} threads, OMP_PROC_BIND=close) on every {stride}th "
+                       f"particle of the same input ({n} particles, the full {m['n_cells']}-tet mesh, overlay read "
+                       f"from file), {steps} steps of push + move + deposit + hole fill, {loop_s:.2f}s of loops")
 
 
 def load_traffic(workload_tag, kernel="fempic_first_kernel"):
@@ -475,7 +542,7 @@ def bench_cabana(args):
         oracle.cabana_move_deposit(c2c6, interp, hacc, hp, hv, hs, weight[sl].copy(), hc, deck["qsp"],
                                    deck["qdt_2mc"], deck["cdt_d"])
         dt_cpu = time.perf_counter() - t2
-        out["cpu_baseline"] = {"value": m / dt_cpu, "unit": "particles/s", "cores": 1, "kind": "port",
+        out["cpu_baseline"] = {"value": m / dt_cpu, "unit": "particles/s", "cores": 1, "kind": "port", "cpu": cpu_model(),
                                "sample": f"one step of the first {m} particles"}
         log(f"cpu baseline {m / dt_cpu:.3e} particles/s (1 thread)")
     return out
@@ -682,15 +749,23 @@ def main():
     except Exception as e:  # reported, never required
         log(f"fp32 deposit report unavailable: {e}")
 
-    cpu = None
+    cpu = cpu_port = None
     if not args.no_cpu:
         try:
             del gpu
             torch.cuda.empty_cache()
-            cpu = cpu_baseline(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, args.cpu_budget)
-            log(f"cpu baseline {cpu['value']:.3e} particles/s ({cpu['cores']} threads)")
+            cpu_port = cpu_baseline(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, args.cpu_budget)
+            log(f"cpu baseline (oracle) {cpu_port['value']:.3e} particles/s ({cpu_port['cores']} threads)")
         except Exception as e:  # the baseline is reported, never required
-            log(f"cpu baseline unavailable: {e}")
+            log(f"oracle cpu baseline unavailable: {e}")
+        try:
+            cpu = cpu_baseline_reference(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, steps=16, stride=2)
+            if cpu:
+                log(f"cpu baseline (reference omp build) {cpu['value']:.3e} particles/s ({cpu['cores']} threads)")
+        except Exception as e:
+            log(f"reference cpu baseline unavailable: {e}")
+        if cpu is None:
+            cpu, cpu_port = cpu_port, None
 
     fill = "hole-fill" if args.fill == "holefill" or args.sort_every <= 0 else f"hole-fill, sort by cell every {args.sort_every} steps"
     out = {
@@ -732,6 +807,7 @@ def main():
         },
         "deposit_fp32_compensated": f32,
         "cpu_baseline": cpu,
+        "cpu_port_baseline": cpu_port,
     }
     print(json.dumps(out), flush=True)
 

@@ -10,6 +10,7 @@
 //
 // usage: fempic_ref <params> <in_dir> <out_dir>
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -127,9 +128,11 @@ int main(int argc, char** argv) {
 
     FILE* stats = fopen((out + "/steps.txt").c_str(), "w");
     std::vector<int> hops_first, hops_last;
+    double loop_s = 0.0;  // wall time of the steps' loops alone (no set-up, no dumps)
     for (OPP_main_loop_iter = 0; OPP_main_loop_iter < steps; OPP_main_loop_iter++) {
         oppx_ref_hops.assign((size_t)particle_set->set_capacity + 1, 0);
         const int before = particle_set->size;
+        const auto t_step = std::chrono::steady_clock::now();
 
         opp_reset_dat(n_charge_den, opp_zero_double16);
         opp_par_loop_all__calculate_new_pos_vel_kernel(particle_set, opp_arg_dat(c_ef_d, p2c_map, OPP_READ),
@@ -143,6 +146,7 @@ int main(int argc, char** argv) {
             opp_arg_dat(n_charge_den, 1, c2n_map, p2c_map, OPP_INC),
             opp_arg_dat(n_charge_den, 2, c2n_map, p2c_map, OPP_INC),
             opp_arg_dat(n_charge_den, 3, c2n_map, p2c_map, OPP_INC));
+        loop_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_step).count();
 
         long long tot = 0;
         int mx = 0;
@@ -164,6 +168,10 @@ int main(int argc, char** argv) {
         if (OPP_main_loop_iter == steps - 1) hops_last.assign(oppx_ref_hops.begin(), oppx_ref_hops.begin() + before);
     }
     fclose(stats);
+    if (FILE* tf = fopen((out + "/loop_seconds.txt").c_str(), "w")) {
+        fprintf(tf, "%.6f\n", loop_s);
+        fclose(tf);
+    }
 
     const int sz = particle_set->size;
     write_arr<double>(out + "/final_pos.bin", (const double*)p_pos_d->data, 3 * (size_t)sz);
