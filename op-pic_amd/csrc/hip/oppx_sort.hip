@@ -49,41 +49,38 @@ __device__ __forceinline__ unsigned long long digit_peers(uint32_t d, unsigned l
     return peers;
 }
 
-// keys0 = p2c with MAX_CELL_INDEX mapped to n_cells (the last bucket), vals0 = index; count removed
-__global__ void __launch_bounds__(kB) sort_prepare(const int32_t* __restrict__ p2c, const oppx_move_counters* ctr,
-                                                   int64_t n_max, int64_t n_cells, uint32_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ vals, unsigned long long* __restrict__ n_removed) {
-    const int64_t n = live_n(ctr, n_max);
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
-    bool rem = false;
-    if (i < n) {
-        const int32_t c = p2c[i];
-        rem = (c == kMaxCell);
-        keys[i] = rem ? (uint32_t)n_cells : (uint32_t)c;
-        vals[i] = (uint32_t)i;
-    }
-    const unsigned long long m = __ballot(rem);
-    // per-wave counts land on one of kRemSlots counters (block id modulo kRemSlots): one shared
-    // counter would serialise every wave's atomic in one L2 channel
-    if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(n_removed + (blockIdx.x % kRemSlots), (unsigned long long)__popcll(m));
+// Key of a particle: its cell, MAX_CELL_INDEX mapped to n_cells (the last bucket). The first digit
+// pass reads the keys from p2c and takes the particle index as the value, so no key/value arrays
+// are written beforehand; it also counts the removed particles.
+__device__ __forceinline__ uint32_t cell_key(int32_t c, uint32_t n_cells) {
+    return c == kMaxCell ? n_cells : (uint32_t)c;
 }
 
-template <int ITEMS>
-__global__ void __launch_bounds__(kB) sort_hist(const uint32_t* __restrict__ keys, const oppx_move_counters* ctr,
-                                                int64_t n_max, int shift, uint32_t* __restrict__ hist, int64_t nblocks) {
+template <int ITEMS, bool FIRST>
+__global__ void __launch_bounds__(kB) sort_hist(const uint32_t* __restrict__ keys, const int32_t* __restrict__ p2c,
+                                                uint32_t n_cells, const oppx_move_counters* ctr, int64_t n_max,
+                                                int shift, uint32_t* __restrict__ hist, int64_t nblocks,
+                                                unsigned long long* __restrict__ n_removed) {
     __shared__ uint32_t s_h[kBins];
     s_h[threadIdx.x] = 0;
     __syncthreads();
     const int64_t n = live_n(ctr, n_max);
     const int lane = threadIdx.x & (kWave - 1);
+    unsigned long long removed = 0;
     for (int r = 0; r < ITEMS; r++) {
         const int64_t i = (int64_t)blockIdx.x * (kB * ITEMS) + r * kB + threadIdx.x;
         const bool ok = i < n;
-        const uint32_t d = ok ? (keys[i] >> shift) & 0xFFu : 0u;
+        uint32_t k = 0;
+        if (ok) k = FIRST ? cell_key(p2c[i], n_cells) : keys[i];
+        if (FIRST) removed += (unsigned long long)__popcll(__ballot(ok && k == n_cells));
+        const uint32_t d = (k >> shift) & 0xFFu;
         const unsigned long long act = __ballot(ok);
         const unsigned long long peers = digit_peers(d, act);
         if (ok && lane == __ffsll((long long)peers) - 1) atomicAdd(&s_h[d], (uint32_t)__popcll(peers));
     }
+    // per-wave removed counts land on one of kRemSlots counters (block id modulo kRemSlots): one
+    // shared counter would serialise every wave's atomic in one L2 channel
+    if (FIRST && lane == 0 && removed) atomicAdd(n_removed + (blockIdx.x % kRemSlots), removed);
     __syncthreads();
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = s_h[threadIdx.x];
 }
@@ -93,8 +90,9 @@ __global__ void __launch_bounds__(kB) sort_hist(const uint32_t* __restrict__ key
 // order). The block then lays its elements out in LDS in digit order and writes each digit's run to
 // its global offset with consecutive threads on consecutive addresses, so the global writes are
 // whole runs instead of one scattered word per lane.
-template <int ITEMS, bool LAST>
+template <int ITEMS, bool LAST, bool FIRST>
 __global__ void __launch_bounds__(kB) sort_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                   const int32_t* __restrict__ p2c, uint32_t n_cells,
                                                    const oppx_move_counters* ctr, int64_t n_max, int shift,
                                                    const uint32_t* __restrict__ offs, int64_t nblocks,
                                                    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
@@ -113,8 +111,13 @@ __global__ void __launch_bounds__(kB) sort_scatter(const uint32_t* __restrict__ 
     for (int r = 0; r < ITEMS; r++) {
         const int li = r * kB + threadIdx.x;
         const bool ok = li < m;
-        key[r] = ok ? kin[base + li] : 0u;
-        val[r] = ok ? vin[base + li] : 0u;
+        if (FIRST) {
+            key[r] = ok ? cell_key(p2c[base + li], n_cells) : 0u;
+            val[r] = (uint32_t)(base + li);
+        } else {
+            key[r] = ok ? kin[base + li] : 0u;
+            val[r] = ok ? vin[base + li] : 0u;
+        }
     }
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
@@ -356,26 +359,35 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
     SortScratch sc = carve(scratch, n_max, items);
     zero_u64<<<1, kRemSlots, 0, s>>>(sc.n_removed);  // a kernel, not a memset node, so the sort can be captured
     if (n_max > 0) {
-        sort_prepare<<<grid_for(n_max, kB), kB, 0, s>>>(p2c, counters, n_max, n_cells, sc.ka, sc.va, sc.n_removed);
-        OPPX_LAUNCH_CHECK("sort_prepare");
         int bits = 1;
         while (bits < 32 && ((int64_t)1 << bits) <= n_cells) bits++;  // keys are in [0, n_cells]
         uint32_t *kin = sc.ka, *vin = sc.va, *kout = sc.kb, *vout = sc.vb;
+        const uint32_t kc = (uint32_t)n_cells;
         for (int shift = 0; shift < bits; shift += 8) {
-            if (items == 4) sort_hist<4><<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, counters, n_max, shift, sc.hist, sc.nblocks);
-            else if (items == 8) sort_hist<8><<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, counters, n_max, shift, sc.hist, sc.nblocks);
-            else sort_hist<16><<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, counters, n_max, shift, sc.hist, sc.nblocks);
+            const bool first = shift == 0;
+#define OPPX_HIST(I, F) sort_hist<I, F><<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, p2c, kc, counters, n_max, shift, \
+                                                                           sc.hist, sc.nblocks, sc.n_removed)
+            if (items == 4) { if (first) OPPX_HIST(4, true); else OPPX_HIST(4, false); }
+            else if (items == 8) { if (first) OPPX_HIST(8, true); else OPPX_HIST(8, false); }
+            else { if (first) OPPX_HIST(16, true); else OPPX_HIST(16, false); }
+#undef OPPX_HIST
             OPPX_LAUNCH_CHECK("sort_hist");
             scan_reduce<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
             scan_sums<<<1, kB, 0, s>>>(sc.sums, sc.scan_blocks);
             scan_apply<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
             OPPX_LAUNCH_CHECK("sort_scan");
             const bool last = shift + 8 >= bits;  // the keys of the last pass are not needed
-#define OPPX_SCATTER(I, LST) sort_scatter<I, LST><<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, vin, counters, n_max, \
-                                                                                       shift, sc.hist, sc.nblocks, kout, vout)
-            if (items == 4) { if (last) OPPX_SCATTER(4, true); else OPPX_SCATTER(4, false); }
-            else if (items == 8) { if (last) OPPX_SCATTER(8, true); else OPPX_SCATTER(8, false); }
-            else { if (last) OPPX_SCATTER(16, true); else OPPX_SCATTER(16, false); }
+#define OPPX_SCATTER(I, LST, F) sort_scatter<I, LST, F><<<(unsigned)sc.nblocks, kB, 0, s>>>( \
+    kin, vin, p2c, kc, counters, n_max, shift, sc.hist, sc.nblocks, kout, vout)
+#define OPPX_SCATTER_LF(I)                                                     \
+    do {                                                                       \
+        if (last) { if (first) OPPX_SCATTER(I, true, true); else OPPX_SCATTER(I, true, false); } \
+        else { if (first) OPPX_SCATTER(I, false, true); else OPPX_SCATTER(I, false, false); }   \
+    } while (0)
+            if (items == 4) OPPX_SCATTER_LF(4);
+            else if (items == 8) OPPX_SCATTER_LF(8);
+            else OPPX_SCATTER_LF(16);
+#undef OPPX_SCATTER_LF
 #undef OPPX_SCATTER
             OPPX_LAUNCH_CHECK("sort_scatter");
             uint32_t* t = kin; kin = kout; kout = t;
