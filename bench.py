@@ -11,7 +11,8 @@ gather applied by the next step's move. State stays resident in HBM.
 
 Prints one JSON line: particles/s = sum over timed steps of live particles / wall time, with the
 dominant kernel's roofline (algorithmic bytes / HIP-event duration vs 8 TB/s), p50/p99 step
-latency, and the oracle (C++/OpenMP restatement) timed on this host's cores on a bounded sample.
+latency, and the reference's own OpenMP build (the C++/OpenMP oracle beside it) timed on this host's
+cores on a bounded sample.
 """
 
 from __future__ import annotations
@@ -86,14 +87,14 @@ def cpu_baseline(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, budget_s):
         tot_parts += n_before
         steps += 1
     return dict(value=tot_parts / t_tot, unit="particles/s", cores=threads, kind="port", cpu=cpu_model(),
-                sample=f"oracle (C++/OpenMP restatement, {threads} threads) on every {stride}th particle of the same "
+                sample=f"oracle (C++/OpenMP restatement, {threads} threads) on one particle in {stride} of the same "
                        f"input ({n0} particles, the full {m['n_cells']}-tet mesh and overlay), {steps} steps, "
                        f"{t_tot:.1f}s")
 
 
 def cpu_baseline_reference(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, steps=10, stride=10):
     """The reference's own OpenMP build (oracle/_ref/fempic_ref_omp, compiled from the reference
-    sources where they were available) on the host's cores: the full mesh, every `stride`-th particle
+    sources where they were available) on the host's cores: the full mesh, one particle in `stride`
     of the same input, `steps` steps of push + move (direct hop, overlay read from a file in the
     reference's format, as opp_dh_data_generate = false does) + deposit + hole fill; only the
     steps' loops are timed. None when the build is absent."""
@@ -142,8 +143,8 @@ def cpu_baseline_reference(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, 
         st = np.loadtxt(os.path.join(outd, "steps.txt"), ndmin=2)
         moved = float(st[:, 1].sum())
     return dict(value=moved / loop_s, unit="particles/s", cores=threads, kind="reference", cpu=cpu_model(),
-                sample=f"the reference's OpenMP build ({threads} threads, OMP_PROC_BIND=close) on every {stride}th "
-                       f"particle of the same input ({n} particles, the full {m['n_cells']}-tet mesh, overlay read "
+                sample=f"the reference's OpenMP build ({threads} threads, OMP_PROC_BIND=close) on one particle in {stride} "
+                       f"of the same input ({n} particles, the full {m['n_cells']}-tet mesh, overlay read "
                        f"from file), {steps} steps of push + move + deposit + hole fill, {loop_s:.2f}s of loops")
 
 
