@@ -44,7 +44,8 @@ extern "C" {
  * ------------------------------------------------------------------------------------- */
 
 /* Per-tet data the multi-hop walk reads besides c_det: the mover's coefficient
- * (1/6)/volume (apps/app_fempic/kernels.h:110) and the face neighbours c2c (-1 = boundary). */
+ * (1/6)/volume (apps/app_fempic/kernels.h:110) and the face neighbours c2c (negative = boundary;
+ * an entry at or past n_cells is reported as a fault). */
 typedef struct oppx_tet_aux {
     double c2;
     int32_t c2c[4];

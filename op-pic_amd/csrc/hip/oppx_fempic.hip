@@ -801,16 +801,16 @@ __global__ void step_begin_kernel(oppx_move_counters* __restrict__ ctr, double* 
         for (int64_t q = j; q < n_nodes; q += (int64_t)gridDim.x * blockDim.x) charge[q] = 0.0;
 }
 
+// c2c entries are copied as they are: a negative entry is a boundary face (the walk removes the
+// particle, as the reference's `opp_c2c[min_i] >= 0` test does, apps/app_fempic/kernels.h:148-159);
+// an entry at or past n_cells is a corrupt map, which the walk reports as a fault.
 __global__ void tet_aux_kernel(const double* __restrict__ vol, const int32_t* __restrict__ c2c, int64_t n,
                                oppx_tet_aux* __restrict__ aux) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     oppx_tet_aux a;
     a.c2 = (1.0 / 6.0) / vol[c];  // coefficient2 of apps/app_fempic/kernels.h:110, IEEE division
-    for (int q = 0; q < 4; q++) {
-        const int v = c2c[4 * c + q];
-        a.c2c[q] = (v >= -1 && v < n) ? v : -2;  // -2: invalid entry, rejected by the walk as a fault
-    }
+    for (int q = 0; q < 4; q++) a.c2c[q] = c2c[4 * c + q];
     a.pad[0] = a.pad[1] = 0;
     aux[c] = a;
 }

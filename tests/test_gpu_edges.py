@@ -1,6 +1,6 @@
-"""Edge cases of the injection, field-side, CabanaPIC and NESO-grid entry points: empty ranges are
-no-ops, MAX_CELL_INDEX particles are skipped, and indices out of range are counted as faults and
-leave the data they would have touched unchanged (every kernel checks an index before using it)."""
+"""Edge cases of the injection, field-side, FEM-PIC walk, CabanaPIC and NESO-grid entry points: empty
+ranges are no-ops, MAX_CELL_INDEX particles are skipped, and indices out of range are counted as faults
+and leave the data they would have touched unchanged (every kernel checks an index before using it)."""
 
 import ctypes
 
@@ -104,3 +104,62 @@ def test_neso_grid_empty():
     assert g.grid
     g.step(0.1)
     assert g.read_counters()["total_hops"] == 0
+
+
+@pytest.mark.parametrize("two_pass", [False, True])
+def test_fempic_walk_boundary_and_corrupt_neighbour(two_pass):
+    """A walk leaving through a negative c2c entry removes the particle (any negative value, as the
+    reference's `opp_c2c[min_i] >= 0` test does); an entry at or past n_cells is a corrupt map,
+    reported as a fault with the particle's p2c left as it was."""
+    m = oppx.box_mesh(1, 1, 1)
+    nc, nn = int(m["n_cells"]), int(m["n_nodes"])
+    L = oppx.lib()
+    outside = np.array([[1.5, 0.5, 0.5], [0.5, -0.5, 0.5], [0.5, 0.5, 2.0]])  # beyond three different faces
+    inside = np.array([[0.3, 0.6, 0.45]])
+    pos_h = np.vstack([outside, inside])
+    n = len(pos_h)
+    results = {}
+    for tag, fill in (("boundary", -1), ("negative", -7), ("corrupt", nc + 3)):
+        c2c = m["c2c"].copy()
+        c2c[c2c < 0] = fill
+        det, vol = dev(m["c_det"].reshape(-1), torch.float64), dev(m["c_vol"], torch.float64)
+        c2c_d, c2n = dev(c2c.reshape(-1), torch.int32), dev(m["c2n"].reshape(-1), torch.int32)
+        aux = torch.zeros(nc * ctypes.sizeof(oppx.TetAux) // 8, dtype=torch.float64, device="cuda")
+        oppx.check(L.oppx_tet_aux_build(vol.data_ptr(), c2c_d.data_ptr(), nc, aux.data_ptr(), stream()), "aux")
+        ef = torch.zeros(3 * nc, dtype=torch.float64, device="cuda")
+        mesh = oppx.TetMesh(nc, nn, det.data_ptr(), aux.data_ptr(), c2n.data_ptr(), ef.data_ptr())
+        pos = dev(pos_h.T, torch.float64)
+        vel = torch.zeros((3, n), dtype=torch.float64, device="cuda")
+        lc = torch.zeros((4, n), dtype=torch.float64, device="cuda")
+        p2c = dev(np.zeros(n, np.int32), torch.int32)
+        parts = oppx.FempicParts(n, pos.data_ptr(), vel.data_ptr(), lc.data_ptr(), p2c.data_ptr())
+        holes = torch.zeros(int(L.oppx_hole_bits_words(n)), dtype=torch.int64, device="cuda")
+        charge = torch.zeros(nn, dtype=torch.float64, device="cuda")
+        ctr = counters(n)
+        if two_pass:
+            wb = int(L.oppx_fempic_work_bytes(n))
+            work = torch.zeros(max(wb, 8), dtype=torch.uint8, device="cuda")
+            oppx.check(L.oppx_fempic_step_begin(ctr.data_ptr(), charge.data_ptr(), nn, stream()), "begin")
+            oppx.check(L.oppx_fempic_step_fused(n, ctypes.byref(parts), ctypes.byref(mesh), None, 0.0, 0.0,
+                                                ctr.data_ptr(), holes.data_ptr(), work.data_ptr(), wb, None,
+                                                charge.data_ptr(), stream()), "step")
+        else:
+            oppx.check(L.oppx_fempic_move(n, ctypes.byref(parts), ctypes.byref(mesh), None, ctr.data_ptr(),
+                                          holes.data_ptr(), None, 0, None, stream()), "move")
+        torch.cuda.synchronize()
+        results[tag] = (p2c.cpu().numpy(), ctr.cpu().numpy(), charge.cpu().numpy())
+    for tag in ("boundary", "negative"):
+        p, c, q = results[tag]
+        assert list(p[:3]) == [oppx.MAX_CELL_INDEX] * 3 and 0 <= p[3] < nc, tag
+        assert c[1] == 3 and c[6] == 0, tag  # removed, faults
+        if two_pass:
+            assert abs(q.sum() - 1.0) < 1e-12, tag  # the one remaining particle deposits its weights (sum 1)
+    p, c, q = results["corrupt"]
+    assert c[6] == 3 and c[1] == 0
+    assert p[3] == results["boundary"][0][3]
+    if two_pass:
+        # a walk handed to the second pass already holds the cell it stopped at
+        assert ((p[:3] >= 0) & (p[:3] < nc)).all()
+        assert abs(q.sum() - 1.0) < 1e-12
+    else:
+        assert list(p[:3]) == [0, 0, 0]
