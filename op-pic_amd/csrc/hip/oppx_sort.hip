@@ -363,6 +363,8 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
         while (bits < 32 && ((int64_t)1 << bits) <= n_cells) bits++;  // keys are in [0, n_cells]
         uint32_t *kin = sc.ka, *vin = sc.va, *kout = sc.kb, *vout = sc.vb;
         const uint32_t kc = (uint32_t)n_cells;
+        // only the permutation is wanted: the last digit pass writes it straight into perm_out
+        const bool direct = n_dats == 0 && perm_out != nullptr;
         for (int shift = 0; shift < bits; shift += 8) {
             const bool first = shift == 0;
 #define OPPX_HIST(I, F) sort_hist<I, F><<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, p2c, kc, counters, n_max, shift, \
@@ -377,8 +379,9 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
             scan_apply<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
             OPPX_LAUNCH_CHECK("sort_scan");
             const bool last = shift + 8 >= bits;  // the keys of the last pass are not needed
+            uint32_t* vdst = (last && direct) ? reinterpret_cast<uint32_t*>(perm_out) : vout;
 #define OPPX_SCATTER(I, LST, F) sort_scatter<I, LST, F><<<(unsigned)sc.nblocks, kB, 0, s>>>( \
-    kin, vin, p2c, kc, counters, n_max, shift, sc.hist, sc.nblocks, kout, vout)
+    kin, vin, p2c, kc, counters, n_max, shift, sc.hist, sc.nblocks, kout, vdst)
 #define OPPX_SCATTER_LF(I)                                                     \
     do {                                                                       \
         if (last) { if (first) OPPX_SCATTER(I, true, true); else OPPX_SCATTER(I, true, false); } \
@@ -393,8 +396,10 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
             uint32_t* t = kin; kin = kout; kout = t;
             t = vin; vin = vout; vout = t;
         }
-        sort_gather<<<grid_for(n_max, kB), kB, 0, s>>>(vin, counters, n_max, T, perm_out);
-        OPPX_LAUNCH_CHECK("sort_gather");
+        if (!direct) {
+            sort_gather<<<grid_for(n_max, kB), kB, 0, s>>>(vin, counters, n_max, T, perm_out);
+            OPPX_LAUNCH_CHECK("sort_gather");
+        }
     }
     sort_finalize<<<1, 1, 0, s>>>(counters, n_max, sc.n_removed, drop_removed);
     OPPX_LAUNCH_CHECK("sort_finalize");

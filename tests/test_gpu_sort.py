@@ -27,6 +27,34 @@ def test_gpu_sort_matches_oracle(n, n_cells, frac_removed, seed, items):
         oppx.set_tuning(**saved)
 
 
+@pytest.mark.parametrize("n,n_cells,frac_removed,seed", [(1, 5, 0.0, 0), (70001, 300, 0.3, 2),
+                                                         (300000, 6_000_000, 0.05, 3), (5000, 7, 0.0, 4)])
+def test_gpu_sort_permutation_only(n, n_cells, frac_removed, seed):
+    """n_dats = 0 with perm_out (the lazy policy's sort): the last digit pass writes the permutation
+    itself; it must equal the one the same sort gives with dats, and the stable order of the keys."""
+    rng = np.random.default_rng(seed)
+    p2c = rng.integers(0, n_cells, n).astype(np.int32)
+    p2c[rng.random(n) < frac_removed] = oppx.MAX_CELL_INDEX
+    if seed == 4:
+        p2c[:] = 3
+    L = oppx.lib()
+    cap = n + 5
+    k_in = torch.zeros(cap, dtype=torch.int32, device="cuda")
+    k_in[:n] = torch.from_numpy(p2c)
+    perm = torch.full((cap,), -7, dtype=torch.int32, device="cuda")
+    ctr = torch.zeros(oppx.COUNTER_WORDS, dtype=torch.int32, device="cuda")
+    ctr[0] = n
+    sb = int(L.oppx_sort_scratch_bytes(n))
+    scratch = torch.zeros(sb, dtype=torch.uint8, device="cuda")
+    oppx.check(L.oppx_sort_by_cell(n, cap, ctr.data_ptr(), n_cells, k_in.data_ptr(), None, None, 0, perm.data_ptr(), 0,
+                                   scratch.data_ptr(), sb, torch.cuda.current_stream().cuda_stream), "sort")
+    got = perm.cpu().numpy()
+    assert np.array_equal(got[:n], np.argsort(p2c, kind="stable"))
+    assert (got[n:] == -7).all()  # nothing written past the live range
+    c = ctr.cpu().numpy()
+    assert c[0] == n and c[1] == int((p2c == oppx.MAX_CELL_INDEX).sum())  # drop_removed = 0 keeps the count
+
+
 def _sort_case(n, n_cells, frac_removed, seed):
     rng = np.random.default_rng(seed)
     p2c = rng.integers(0, n_cells, n).astype(np.int32)
