@@ -22,7 +22,7 @@ constexpr int kElems = kB * kItems;
 constexpr int kMinSortItems = 4;     // smallest radix block (elements per thread) the scratch is sized for
 constexpr int kBins = 256;
 constexpr int kMaxDats = 16;
-constexpr int kRemSlots = 256;  // removed-particle counters of sort_prepare
+constexpr int kRemSlots = 256;  // removed-particle counters (block id modulo kRemSlots)
 
 struct GatherTable {
     int n;
@@ -56,6 +56,9 @@ __device__ __forceinline__ uint32_t cell_key(int32_t c, uint32_t n_cells) {
     return c == kMaxCell ? n_cells : (uint32_t)c;
 }
 
+// Per-block digit counts of one pass. All of a thread's keys are loaded before any is counted
+// (independent loads in flight). Counting needs no ranks: a wave whose live lanes share one digit adds
+// once (nearly sorted keys), any other wave adds per lane.
 template <int ITEMS, bool FIRST>
 __global__ void __launch_bounds__(kB) sort_hist(const uint32_t* __restrict__ keys, const int32_t* __restrict__ p2c,
                                                 uint32_t n_cells, const oppx_move_counters* ctr, int64_t n_max,
@@ -66,17 +69,29 @@ __global__ void __launch_bounds__(kB) sort_hist(const uint32_t* __restrict__ key
     __syncthreads();
     const int64_t n = live_n(ctr, n_max);
     const int lane = threadIdx.x & (kWave - 1);
-    unsigned long long removed = 0;
+    const int64_t base = (int64_t)blockIdx.x * (kB * ITEMS);
+    uint32_t key[ITEMS];
+#pragma unroll
     for (int r = 0; r < ITEMS; r++) {
-        const int64_t i = (int64_t)blockIdx.x * (kB * ITEMS) + r * kB + threadIdx.x;
-        const bool ok = i < n;
-        uint32_t k = 0;
-        if (ok) k = FIRST ? cell_key(p2c[i], n_cells) : keys[i];
+        const int64_t i = base + r * kB + threadIdx.x;
+        key[r] = i < n ? (FIRST ? cell_key(p2c[i], n_cells) : keys[i]) : 0u;
+    }
+    unsigned long long removed = 0;
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const bool ok = base + r * kB + threadIdx.x < n;
+        const uint32_t k = key[r];
         if (FIRST) removed += (unsigned long long)__popcll(__ballot(ok && k == n_cells));
         const uint32_t d = (k >> shift) & 0xFFu;
         const unsigned long long act = __ballot(ok);
-        const unsigned long long peers = digit_peers(d, act);
-        if (ok && lane == __ffsll((long long)peers) - 1) atomicAdd(&s_h[d], (uint32_t)__popcll(peers));
+        if (act == 0ull) continue;  // uniform over the wave
+        const int lead = __ffsll((long long)act) - 1;
+        const uint32_t d_lead = (uint32_t)__shfl((int)d, lead);
+        if (__ballot(ok && d == d_lead) == act) {
+            if (lane == lead) atomicAdd(&s_h[d], (uint32_t)__popcll(act));
+        } else if (ok) {
+            atomicAdd(&s_h[d], 1u);
+        }
     }
     // per-wave removed counts land on one of kRemSlots counters (block id modulo kRemSlots): one
     // shared counter would serialise every wave's atomic in one L2 channel
@@ -85,33 +100,38 @@ __global__ void __launch_bounds__(kB) sort_hist(const uint32_t* __restrict__ key
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = s_h[threadIdx.x];
 }
 
-// Scatter of one digit pass. Every element's stable rank among the block's elements of its digit
-// comes from wave digit matching, round by round (rounds in index order, waves in order, lanes in
-// order). The block then lays its elements out in LDS in digit order and writes each digit's run to
-// its global offset with consecutive threads on consecutive addresses, so the global writes are
-// whole runs instead of one scattered word per lane.
-template <int ITEMS, bool LAST, bool FIRST>
-__global__ void __launch_bounds__(kB) sort_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                   const int32_t* __restrict__ p2c, uint32_t n_cells,
-                                                   const oppx_move_counters* ctr, int64_t n_max, int shift,
-                                                   const uint32_t* __restrict__ offs, int64_t nblocks,
-                                                   uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
-    constexpr int E = kB * ITEMS;
-    __shared__ uint32_t s_run[kBins];               // elements of each digit ranked so far (then: digit start)
-    __shared__ uint32_t s_wave[kB / kWave][kBins];  // per-wave digit counts of the current round
-    __shared__ uint32_t s_key[E], s_val[E];
-    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    const unsigned long long below = (1ull << lane) - 1ull;
-    const int64_t base = (int64_t)blockIdx.x * E;
-    const int64_t n = live_n(ctr, n_max);
-    const int m = (int)((n - base) < E ? (n - base) : E);  // elements of this block (> 0 for every launched block)
-    s_run[threadIdx.x] = 0;
-    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+// ---- one digit pass over a tile of 256 * ITEMS consecutive elements ----
+// Wave w of the block owns the 64 * ITEMS consecutive elements [w * 64 * ITEMS, (w + 1) * 64 * ITEMS)
+// of the tile and takes them 64 at a time (round r: element w * 64 * ITEMS + r * 64 + lane), so an
+// element's stable rank among the tile's elements of its digit is: that digit's elements in the
+// waves before (summed once, after all rounds), plus the wave's own in earlier rounds (a per-wave
+// count in LDS, updated round by round without block barriers), plus the lanes below holding the
+// same digit (8 ballots). The tile is then laid out in LDS in digit order and each digit's run is
+// written to its global offset with consecutive threads on consecutive addresses, so the global
+// writes are whole runs instead of one scattered word per lane.
+template <int ITEMS>
+struct TileLds {
+    uint32_t run[kBins];               // the tile's elements of each digit, then the digit's start in the tile
+    uint32_t wave[kB / kWave][kBins];  // per-wave digit counts, then their exclusive prefix over the waves
+    uint32_t dst[kBins];               // global offset of each digit's run
+    uint32_t tot[kB / kWave];          // wave totals of the digit-start scan
+    uint32_t key[kB * ITEMS], val[kB * ITEMS];
+};
+
+template <int ITEMS>
+__device__ __forceinline__ int tile_elem(int r) {
+    return (int)(threadIdx.x / kWave) * (kWave * ITEMS) + r * kWave + (int)(threadIdx.x & (kWave - 1));
+}
+
+template <int ITEMS, bool FIRST>
+__device__ __forceinline__ void tile_load(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                          const int32_t* __restrict__ p2c, uint32_t n_cells, int64_t base, int m,
+                                          uint32_t (&key)[ITEMS], uint32_t (&val)[ITEMS]) {
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
-        const int li = r * kB + threadIdx.x;
+        const int li = tile_elem<ITEMS>(r);
         const bool ok = li < m;
-        if (FIRST) {
+        if (FIRST) {  // keys straight from p2c, the particle index as the value
             key[r] = ok ? cell_key(p2c[base + li], n_cells) : 0u;
             val[r] = (uint32_t)(base + li);
         } else {
@@ -119,47 +139,71 @@ __global__ void __launch_bounds__(kB) sort_scatter(const uint32_t* __restrict__ 
             val[r] = ok ? vin[base + li] : 0u;
         }
     }
+}
+
+// Ranks of the tile's elements inside their wave's part; leaves the tile's count of every digit in
+// L.run and the exclusive prefix over the waves in L.wave. L.wave must be zeroed (and that made
+// visible) by the caller.
+template <int ITEMS>
+__device__ __forceinline__ void tile_rank(TileLds<ITEMS>& L, const uint32_t (&key)[ITEMS], int m, int shift,
+                                          uint32_t (&rank)[ITEMS]) {
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
-        for (int q = 0; q < kB / kWave; q++) s_wave[q][threadIdx.x] = 0;
-        __syncthreads();
-        const bool ok = r * kB + (int)threadIdx.x < m;
+        const bool ok = tile_elem<ITEMS>(r) < m;
         const uint32_t d = (key[r] >> shift) & 0xFFu;
         const unsigned long long peers = digit_peers(d, __ballot(ok));
-        if (ok && lane == __ffsll((long long)peers) - 1) s_wave[w][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        uint32_t before = s_run[d];
-        for (int q = 0; q < w; q++) before += s_wave[q][d];
+        // every lane reads the wave's count before the lowest lane of each digit group adds the group
+        // (one wave's LDS operations complete in program order)
+        const uint32_t before = L.wave[w][d];
         rank[r] = before + (uint32_t)__popcll(peers & below);
-        __syncthreads();
-        uint32_t add = 0;
-        for (int q = 0; q < kB / kWave; q++) add += s_wave[q][threadIdx.x];
-        s_run[threadIdx.x] += add;
+        if (ok && lane == __ffsll((long long)peers) - 1) L.wave[w][d] = before + (uint32_t)__popcll(peers);
     }
     __syncthreads();
-    // digit starts inside the block: exclusive scan of the counts (one digit per thread)
-    {
-        const uint32_t c = s_run[threadIdx.x];
-        uint32_t inc = c;
+    uint32_t run = 0;
+    for (int q = 0; q < kB / kWave; q++) {
+        const uint32_t c = L.wave[q][threadIdx.x];
+        L.wave[q][threadIdx.x] = run;
+        run += c;
+    }
+    L.run[threadIdx.x] = run;
+    __syncthreads();
+}
+
+// Digit starts inside the tile: exclusive scan of L.run (one digit per thread), in place.
+template <int ITEMS>
+__device__ __forceinline__ void tile_digit_starts(TileLds<ITEMS>& L) {
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const uint32_t c = L.run[threadIdx.x];
+    uint32_t inc = c;
 #pragma unroll
-        for (int dd = 1; dd < kWave; dd <<= 1) {
-            const uint32_t t = __shfl_up(inc, dd);
-            if (lane >= dd) inc += t;
-        }
-        if (lane == kWave - 1) s_wave[0][w] = inc;
-        __syncthreads();
-        uint32_t off = 0;
-        for (int q = 0; q < w; q++) off += s_wave[0][q];
-        __syncthreads();
-        s_run[threadIdx.x] = off + inc - c;
+    for (int dd = 1; dd < kWave; dd <<= 1) {
+        const uint32_t t = __shfl_up(inc, dd);
+        if (lane >= dd) inc += t;
     }
+    if (lane == kWave - 1) L.tot[w] = inc;
     __syncthreads();
+    uint32_t off = 0;
+    for (int q = 0; q < w; q++) off += L.tot[q];
+    L.run[threadIdx.x] = off + inc - c;
+    __syncthreads();
+}
+
+// Digit-ordered layout in LDS, then the runs to L.dst[d] + (position in the run). Writes at or past
+// `limit` are dropped (never taken by a correct sort; a bound on every store).
+template <int ITEMS, bool LAST>
+__device__ __forceinline__ void tile_write(TileLds<ITEMS>& L, const uint32_t (&key)[ITEMS], const uint32_t (&val)[ITEMS],
+                                           const uint32_t (&rank)[ITEMS], int m, int shift, uint64_t limit,
+                                           uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    const int w = threadIdx.x / kWave;
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
-        if (r * kB + (int)threadIdx.x < m) {
-            const uint32_t pos = s_run[(key[r] >> shift) & 0xFFu] + rank[r];
-            s_key[pos] = key[r];
-            s_val[pos] = val[r];
+        if (tile_elem<ITEMS>(r) < m) {
+            const uint32_t d = (key[r] >> shift) & 0xFFu;
+            const uint32_t pos = L.run[d] + L.wave[w][d] + rank[r];
+            L.key[pos] = key[r];
+            L.val[pos] = val[r];
         }
     }
     __syncthreads();
@@ -167,13 +211,37 @@ __global__ void __launch_bounds__(kB) sort_scatter(const uint32_t* __restrict__ 
     for (int r = 0; r < ITEMS; r++) {
         const int p = r * kB + threadIdx.x;
         if (p < m) {
-            const uint32_t k = s_key[p];
+            const uint32_t k = L.key[p];
             const uint32_t d = (k >> shift) & 0xFFu;
-            const uint64_t dst = (uint64_t)offs[(int64_t)d * nblocks + blockIdx.x] + (uint32_t)(p - (int)s_run[d]);
-            if (!LAST) kout[dst] = k;
-            vout[dst] = s_val[p];
+            const uint64_t dst = (uint64_t)L.dst[d] + (uint32_t)(p - (int)L.run[d]);
+            if (dst < limit) {
+                if (!LAST) kout[dst] = k;
+                vout[dst] = L.val[p];
+            }
         }
     }
+}
+
+// Scatter of one digit pass with the (digit, block) offsets from the scanned per-block histograms.
+template <int ITEMS, bool LAST, bool FIRST>
+__global__ void __launch_bounds__(kB) sort_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                   const int32_t* __restrict__ p2c, uint32_t n_cells,
+                                                   const oppx_move_counters* ctr, int64_t n_max, int shift,
+                                                   const uint32_t* __restrict__ offs, int64_t nblocks,
+                                                   uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    __shared__ TileLds<ITEMS> L;
+    const int64_t base = (int64_t)blockIdx.x * (kB * ITEMS);
+    const int64_t n = live_n(ctr, n_max);
+    // elements of this tile (<= 0 for a tile past the live count: nothing is loaded or written)
+    const int m = (int)((n - base) < kB * ITEMS ? (n - base) : kB * ITEMS);
+    for (int q = 0; q < kB / kWave; q++) L.wave[q][threadIdx.x] = 0;
+    L.dst[threadIdx.x] = offs[(int64_t)threadIdx.x * nblocks + blockIdx.x];
+    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+    tile_load<ITEMS, FIRST>(kin, vin, p2c, n_cells, base, m, key, val);
+    __syncthreads();
+    tile_rank<ITEMS>(L, key, m, shift, rank);
+    tile_digit_starts<ITEMS>(L);
+    tile_write<ITEMS, LAST>(L, key, val, rank, m, shift, (uint64_t)n, kout, vout);
 }
 
 // ---- exclusive scan of a uint32 array (chunks of kElems) ----
@@ -291,6 +359,12 @@ struct SortScratch {
     int64_t nblocks, hist_n, scan_blocks;
 };
 
+// int32 words of the histogram region: per-block histograms at the smallest radix block
+int64_t hist_words(int64_t n_max) {
+    const int64_t nb = n_max > 0 ? (n_max + kB * kMinSortItems - 1) / (kB * kMinSortItems) : 1;
+    return nb * kBins;
+}
+
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 SortScratch carve(void* base, int64_t n_max, int items) {
@@ -306,7 +380,7 @@ SortScratch carve(void* base, int64_t n_max, int items) {
     s.va = reinterpret_cast<uint32_t*>(p); p += arr;
     s.kb = reinterpret_cast<uint32_t*>(p); p += arr;
     s.vb = reinterpret_cast<uint32_t*>(p); p += arr;
-    s.hist = reinterpret_cast<uint32_t*>(p); p += align256((size_t)s.hist_n * sizeof(uint32_t));
+    s.hist = reinterpret_cast<uint32_t*>(p); p += align256((size_t)hist_words(n_max) * sizeof(uint32_t));
     s.sums = reinterpret_cast<uint32_t*>(p); p += align256((size_t)s.scan_blocks * sizeof(uint32_t));
     s.n_removed = reinterpret_cast<unsigned long long*>(p);
     return s;
@@ -315,9 +389,8 @@ SortScratch carve(void* base, int64_t n_max, int items) {
 size_t scratch_need(int64_t n_max) {
     const int64_t per_block = (int64_t)kB * kMinSortItems;  // the largest histogram any sort_items needs
     const int64_t nb = n_max > 0 ? (n_max + per_block - 1) / per_block : 1;
-    const int64_t hist_n = nb * kBins;
-    const int64_t scan_blocks = (hist_n + kElems - 1) / kElems;
-    return 4 * align256((size_t)(n_max > 0 ? n_max : 1) * 4) + align256((size_t)hist_n * 4) +
+    const int64_t scan_blocks = (nb * kBins + kElems - 1) / kElems;
+    return 4 * align256((size_t)(n_max > 0 ? n_max : 1) * 4) + align256((size_t)hist_words(n_max) * 4) +
            align256((size_t)scan_blocks * 4) + kRemSlots * sizeof(unsigned long long);
 }
 
@@ -357,29 +430,33 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
     const int items = g_tuning.sort_items;
     OPPX_ARG_CHECK(items == 4 || items == 8 || items == 16, "oppx_sort_by_cell: tuning.sort_items must be 4, 8 or 16");
     SortScratch sc = carve(scratch, n_max, items);
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) <= n_cells) bits++;  // keys are in [0, n_cells]
+    const int passes = (bits + 7) / 8;
     zero_u64<<<1, kRemSlots, 0, s>>>(sc.n_removed);  // a kernel, not a memset node, so the sort can be captured
+    OPPX_LAUNCH_CHECK("sort_reset");
     if (n_max > 0) {
-        int bits = 1;
-        while (bits < 32 && ((int64_t)1 << bits) <= n_cells) bits++;  // keys are in [0, n_cells]
         uint32_t *kin = sc.ka, *vin = sc.va, *kout = sc.kb, *vout = sc.vb;
         const uint32_t kc = (uint32_t)n_cells;
         // only the permutation is wanted: the last digit pass writes it straight into perm_out
         const bool direct = n_dats == 0 && perm_out != nullptr;
-        for (int shift = 0; shift < bits; shift += 8) {
-            const bool first = shift == 0;
+        for (int pass = 0; pass < passes; pass++) {
+            const int shift = 8 * pass;
+            const bool first = pass == 0;
+            const bool last = pass == passes - 1;  // the keys of the last pass are not needed
+            uint32_t* vdst = (last && direct) ? reinterpret_cast<uint32_t*>(perm_out) : vout;
+            {
 #define OPPX_HIST(I, F) sort_hist<I, F><<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, p2c, kc, counters, n_max, shift, \
                                                                            sc.hist, sc.nblocks, sc.n_removed)
-            if (items == 4) { if (first) OPPX_HIST(4, true); else OPPX_HIST(4, false); }
-            else if (items == 8) { if (first) OPPX_HIST(8, true); else OPPX_HIST(8, false); }
-            else { if (first) OPPX_HIST(16, true); else OPPX_HIST(16, false); }
+                if (items == 4) { if (first) OPPX_HIST(4, true); else OPPX_HIST(4, false); }
+                else if (items == 8) { if (first) OPPX_HIST(8, true); else OPPX_HIST(8, false); }
+                else { if (first) OPPX_HIST(16, true); else OPPX_HIST(16, false); }
 #undef OPPX_HIST
-            OPPX_LAUNCH_CHECK("sort_hist");
-            scan_reduce<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
-            scan_sums<<<1, kB, 0, s>>>(sc.sums, sc.scan_blocks);
-            scan_apply<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
-            OPPX_LAUNCH_CHECK("sort_scan");
-            const bool last = shift + 8 >= bits;  // the keys of the last pass are not needed
-            uint32_t* vdst = (last && direct) ? reinterpret_cast<uint32_t*>(perm_out) : vout;
+                OPPX_LAUNCH_CHECK("sort_hist");
+                scan_reduce<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
+                scan_sums<<<1, kB, 0, s>>>(sc.sums, sc.scan_blocks);
+                scan_apply<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
+                OPPX_LAUNCH_CHECK("sort_scan");
 #define OPPX_SCATTER(I, LST, F) sort_scatter<I, LST, F><<<(unsigned)sc.nblocks, kB, 0, s>>>( \
     kin, vin, p2c, kc, counters, n_max, shift, sc.hist, sc.nblocks, kout, vdst)
 #define OPPX_SCATTER_LF(I)                                                     \
@@ -387,12 +464,13 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
         if (last) { if (first) OPPX_SCATTER(I, true, true); else OPPX_SCATTER(I, true, false); } \
         else { if (first) OPPX_SCATTER(I, false, true); else OPPX_SCATTER(I, false, false); }   \
     } while (0)
-            if (items == 4) OPPX_SCATTER_LF(4);
-            else if (items == 8) OPPX_SCATTER_LF(8);
-            else OPPX_SCATTER_LF(16);
+                if (items == 4) OPPX_SCATTER_LF(4);
+                else if (items == 8) OPPX_SCATTER_LF(8);
+                else OPPX_SCATTER_LF(16);
 #undef OPPX_SCATTER_LF
 #undef OPPX_SCATTER
-            OPPX_LAUNCH_CHECK("sort_scatter");
+                OPPX_LAUNCH_CHECK("sort_scatter");
+            }
             uint32_t* t = kin; kin = kout; kout = t;
             t = vin; vin = vout; vout = t;
         }

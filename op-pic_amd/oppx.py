@@ -292,7 +292,7 @@ def neso_particles(n: int, seed: int, m: dict, vmax: float):
 TUNING_FIELDS = ("tile", "refill_min", "split", "persistent", "sort_items", "inline_hops", "first_block")
 
 
-def set_tuning(tile=256, refill_min=16, split=0, persistent=0, sort_items=8, inline_hops=64, first_block=256):
+def set_tuning(tile=256, refill_min=16, split=0, persistent=0, sort_items=16, inline_hops=64, first_block=256):
     """Process-wide launch shape of the walk and the sort (results do not depend on it)."""
     t = Tuning(tile, refill_min, split, persistent, sort_items, inline_hops, first_block)
     check(lib().oppx_set_tuning(ctypes.byref(t)), "set_tuning")
