@@ -270,9 +270,10 @@ int oppx_fempic_step_fused(int64_t n_max, const oppx_fempic_parts* parts, const 
 /* The fused step with a pending sort applied on the fly: particle i reads its pos/vel/p2c (and, for
  * fields the step does not produce, lc) at perm[i] of `src` and writes every field at i of `parts`
  * (separate buffers, same stride). A Sort_All / Sort_Periodic finalize can then compute only the
- * permutation (oppx_sort_by_cell with n_dats = 0 and perm_out) and leave the record gather to the
- * next step, which reads the records anyway (no reference counterpart: the reference gathers every
- * dat in the finalize). Needs the two-pass move (tuning.inline_hops >= 1 and its workspace). */
+ * permutation (oppx_sort_by_cell with perm_out and at most the p2c dat) and leave the record gather to
+ * the next step, which reads the records anyway (no reference counterpart: the reference gathers every
+ * dat in the finalize). src->p2c == NULL: the sort already wrote the sorted p2c into parts->p2c, which
+ * is then read in place. Needs the two-pass move (tuning.inline_hops >= 1 and its workspace). */
 int oppx_fempic_step_fused_gather(int64_t n_max, const int32_t* perm, const oppx_fempic_parts* src,
                                   const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh, const oppx_dh_grid* dh,
                                   double k, double dt, oppx_move_counters* counters, uint64_t* hole_bits, void* work,
@@ -466,8 +467,9 @@ size_t oppx_sort_scratch_bytes(int64_t n_max);
 /* opp_particle_sort and the Sort_All / Sort_Periodic finalize (opp_lib/src/core/opp_particle_organize_core.cpp:98-121;
  * opp_lib/src/omp/opp_particle_mover.cpp:223-265; hip opp_lib/src/hip/opp_particle_sorter.cpp:188-267): orders the
  * counters->size particles by p2c, removed particles (MAX_CELL_INDEX) last, stable by (p2c, index). Each dat in
- * `dats` is gathered into the matching separate buffer of `out_dats` (same dims; the caller swaps them in).
- * perm_out (nullable) receives the source index of every output slot. drop_removed != 0 also applies the
+ * `dats` is gathered into the matching separate buffer of `out_dats` (same dims; the caller swaps them in);
+ * a dat whose base is `p2c` itself is written from the sorted keys instead of gathered. perm_out
+ * (nullable) receives the source index of every output slot. drop_removed != 0 also applies the
  * finalize: counters->removed = number removed, counters->size -= removed. */
 int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counters, int64_t n_cells,
                       const int32_t* p2c, const oppx_dat_desc* dats, const oppx_dat_desc* out_dats, int n_dats,

@@ -383,7 +383,9 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
         }
     }
 
-    int cell = (live && !fault) ? ld(sp2c + src) : kMaxCell;
+    // the sort may have written the sorted p2c into the target already (src_p2c null): read in place
+    const bool p2c_in_place = GATHER && P.src_p2c == nullptr;
+    int cell = (live && !fault) ? (p2c_in_place ? ld_stream(P.p2c + i) : ld(sp2c + src)) : kMaxCell;
     // a particle already holding MAX_CELL_INDEX is skipped, as the reference loops do
     // (apps_codegen/app_fempic_cg/seq/move_kernel_loop.hpp:92-94)
     const bool skip = live && !fault && cell == kMaxCell;
@@ -462,7 +464,7 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
                 st_stream(P.vel + d * S + i, svel[d * S + src]);
             }
         }
-        if (!p2c_done) st_stream(P.p2c + i, sp2c[src]);
+        if (!p2c_done && !p2c_in_place) st_stream(P.p2c + i, sp2c[src]);
         if (!lc_done)
             for (int d = 0; d < 4; d++) st_stream(P.lc + d * S + i, P.src_lc[d * S + src]);
         if (P.hops && !(live && !skip && !fault)) P.hops[i] = h;
@@ -1050,7 +1052,7 @@ int oppx_fempic_step_fused_gather(int64_t n_max, const int32_t* perm, const oppx
     StepArgs P;
     int rc = fill_args(P, parts, mesh, dh, counters);
     if (rc) return rc;
-    OPPX_ARG_CHECK(perm && src && src->pos && src->vel && src->lc && src->p2c,
+    OPPX_ARG_CHECK(perm && src && src->pos && src->vel && src->lc,
                    "oppx_fempic_step_fused_gather: permutation and source state required");
     OPPX_ARG_CHECK(src->stride == parts->stride, "oppx_fempic_step_fused_gather: source and target strides differ");
     OPPX_ARG_CHECK(src->pos != parts->pos && src->vel != parts->vel && src->lc != parts->lc && src->p2c != parts->p2c,

@@ -191,11 +191,13 @@ __device__ __forceinline__ void tile_digit_starts(TileLds<ITEMS>& L) {
 }
 
 // Digit-ordered layout in LDS, then the runs to L.dst[d] + (position in the run). Writes at or past
-// `limit` are dropped (never taken by a correct sort; a bound on every store).
+// `limit` are dropped (never taken by a correct sort; a bound on every store). The last pass writes no
+// keys, except as sorted p2c into p2c_out (nullable; key n_cells back to MAX_CELL_INDEX).
 template <int ITEMS, bool LAST>
 __device__ __forceinline__ void tile_write(TileLds<ITEMS>& L, const uint32_t (&key)[ITEMS], const uint32_t (&val)[ITEMS],
                                            const uint32_t (&rank)[ITEMS], int m, int shift, uint64_t limit,
-                                           uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+                                           uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                           int32_t* __restrict__ p2c_out, uint32_t n_cells) {
     const int w = threadIdx.x / kWave;
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
@@ -216,6 +218,7 @@ __device__ __forceinline__ void tile_write(TileLds<ITEMS>& L, const uint32_t (&k
             const uint64_t dst = (uint64_t)L.dst[d] + (uint32_t)(p - (int)L.run[d]);
             if (dst < limit) {
                 if (!LAST) kout[dst] = k;
+                else if (p2c_out) p2c_out[dst] = k == n_cells ? kMaxCell : (int32_t)k;
                 vout[dst] = L.val[p];
             }
         }
@@ -228,7 +231,8 @@ __global__ void __launch_bounds__(kB) sort_scatter(const uint32_t* __restrict__ 
                                                    const int32_t* __restrict__ p2c, uint32_t n_cells,
                                                    const oppx_move_counters* ctr, int64_t n_max, int shift,
                                                    const uint32_t* __restrict__ offs, int64_t nblocks,
-                                                   uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+                                                   uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                   int32_t* __restrict__ p2c_out) {
     __shared__ TileLds<ITEMS> L;
     const int64_t base = (int64_t)blockIdx.x * (kB * ITEMS);
     const int64_t n = live_n(ctr, n_max);
@@ -241,7 +245,7 @@ __global__ void __launch_bounds__(kB) sort_scatter(const uint32_t* __restrict__ 
     __syncthreads();
     tile_rank<ITEMS>(L, key, m, shift, rank);
     tile_digit_starts<ITEMS>(L);
-    tile_write<ITEMS, LAST>(L, key, val, rank, m, shift, (uint64_t)n, kout, vout);
+    tile_write<ITEMS, LAST>(L, key, val, rank, m, shift, (uint64_t)n, kout, vout, p2c_out, n_cells);
 }
 
 // ---- exclusive scan of a uint32 array (chunks of kElems) ----
@@ -414,17 +418,23 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
                    "oppx_sort_by_cell: 0..16 dats with output buffers");
     OPPX_ARG_CHECK(scratch_bytes >= scratch_need(n_max), "oppx_sort_by_cell: scratch too small");
     GatherTable T{};
-    T.n = n_dats;
+    T.n = 0;
     T.stride = stride;
+    int32_t* p2c_out = nullptr;  // the p2c dat itself is written from the keys by the last pass
     for (int q = 0; q < n_dats; q++) {
         OPPX_ARG_CHECK(dats[q].base && out_dats[q].base && dats[q].dim > 0 && dats[q].dim == out_dats[q].dim &&
                            dats[q].elem_bytes == out_dats[q].elem_bytes &&
                            (dats[q].elem_bytes == 4 || dats[q].elem_bytes == 8) && dats[q].base != out_dats[q].base,
                        "oppx_sort_by_cell: bad dat descriptor (dims/sizes must match, output must be a separate buffer)");
-        T.in[q] = dats[q].base;
-        T.out[q] = out_dats[q].base;
-        T.dim[q] = dats[q].dim;
-        T.eb[q] = dats[q].elem_bytes;
+        if (dats[q].base == p2c && dats[q].dim == 1 && dats[q].elem_bytes == 4 && !p2c_out) {
+            p2c_out = reinterpret_cast<int32_t*>(out_dats[q].base);
+            continue;
+        }
+        T.in[T.n] = dats[q].base;
+        T.out[T.n] = out_dats[q].base;
+        T.dim[T.n] = dats[q].dim;
+        T.eb[T.n] = dats[q].elem_bytes;
+        T.n++;
     }
     hipStream_t s = as_stream(stream);
     const int items = g_tuning.sort_items;
@@ -438,8 +448,8 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
     if (n_max > 0) {
         uint32_t *kin = sc.ka, *vin = sc.va, *kout = sc.kb, *vout = sc.vb;
         const uint32_t kc = (uint32_t)n_cells;
-        // only the permutation is wanted: the last digit pass writes it straight into perm_out
-        const bool direct = n_dats == 0 && perm_out != nullptr;
+        // no dat left to gather: the last digit pass writes the permutation straight into perm_out
+        const bool direct = T.n == 0 && perm_out != nullptr;
         for (int pass = 0; pass < passes; pass++) {
             const int shift = 8 * pass;
             const bool first = pass == 0;
@@ -458,7 +468,7 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
                 scan_apply<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
                 OPPX_LAUNCH_CHECK("sort_scan");
 #define OPPX_SCATTER(I, LST, F) sort_scatter<I, LST, F><<<(unsigned)sc.nblocks, kB, 0, s>>>( \
-    kin, vin, p2c, kc, counters, n_max, shift, sc.hist, sc.nblocks, kout, vdst)
+    kin, vin, p2c, kc, counters, n_max, shift, sc.hist, sc.nblocks, kout, vdst, p2c_out)
 #define OPPX_SCATTER_LF(I)                                                     \
     do {                                                                       \
         if (last) { if (first) OPPX_SCATTER(I, true, true); else OPPX_SCATTER(I, true, false); } \
@@ -474,7 +484,7 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
             uint32_t* t = kin; kin = kout; kout = t;
             t = vin; vin = vout; vout = t;
         }
-        if (!direct) {
+        if (!direct && (T.n > 0 || perm_out)) {
             sort_gather<<<grid_for(n_max, kB), kB, 0, s>>>(vin, counters, n_max, T, perm_out);
             OPPX_LAUNCH_CHECK("sort_gather");
         }

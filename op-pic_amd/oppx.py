@@ -454,21 +454,27 @@ class FempicDevice:
             self.perm = torch.zeros(self.cap, dtype=torch.int32, device=self.dev)
         return self._swap
 
-    def _descs(self, tensors):
-        dims = [(3, 8), (3, 8), (4, 8), (1, 4)]
-        return (DatDesc * 4)(*[DatDesc(t.data_ptr(), d, e) for t, (d, e) in zip(tensors, dims)])
+    _DAT_SHAPES = {"pos": (3, 8), "vel": (3, 8), "lc": (4, 8), "p2c": (1, 4)}
+
+    def _descs(self, tensors, names=("pos", "vel", "lc", "p2c")):
+        shapes = [self._DAT_SHAPES[n] for n in names]
+        assert len(tensors) == len(shapes)
+        return (DatDesc * len(tensors))(*[DatDesc(t.data_ptr(), d, e) for t, (d, e) in zip(tensors, shapes)])
 
     def sort(self, drop_removed=False, perm=None, lazy=False):
         """Stable sort of the particle dats by cell (removed last); drop_removed applies the finalize.
-        lazy=True computes only the permutation and leaves the record gather pending: the next fused
-        step reads its inputs through it (pass_fused), or materialize() applies it."""
+        lazy=True computes only the permutation (and the sorted p2c, written by the sort's last pass into
+        the swap buffer) and leaves the rest of the record gather pending: the next fused step reads its
+        inputs through it (pass_fused), or materialize() applies it."""
         self.materialize()
         swap = self._swap_buffers()
         cur = [self.pos, self.vel, self.lc, self.p2c]
         out_perm = perm if perm is not None else (self.perm if lazy else None)
+        names = ("p2c",) if lazy else ("pos", "vel", "lc", "p2c")
+        ins, outs = (cur[3:], swap[3:]) if lazy else (cur, swap)
         check(lib().oppx_sort_by_cell(self.n_max, self.cap, self.counters.data_ptr(), self.nc, self.p2c.data_ptr(),
-                                      self._descs(cur) if not lazy else None, self._descs(swap) if not lazy else None,
-                                      0 if lazy else 4, out_perm.data_ptr() if out_perm is not None else None,
+                                      self._descs(ins, names), self._descs(outs, names), len(ins),
+                                      out_perm.data_ptr() if out_perm is not None else None,
                                       1 if drop_removed else 0, self.sort_scratch.data_ptr(), self.sort_scratch_bytes,
                                       self.stream()), "sort_by_cell")
         if lazy:
@@ -484,8 +490,10 @@ class FempicDevice:
             return
         swap = self._swap_buffers()
         cur = [self.pos, self.vel, self.lc, self.p2c]
+        # the sort already wrote the sorted p2c into the swap buffer
         check(lib().oppx_gather_dats(self.n_max, self.cap, self.counters.data_ptr(), self.perm.data_ptr(),
-                                     self._descs(cur), self._descs(swap), 4, self.stream()), "gather_dats")
+                                     self._descs(cur[:3], ("pos", "vel", "lc")), self._descs(swap[:3], ("pos", "vel", "lc")),
+                                     3, self.stream()), "gather_dats")
         self.pos, self.vel, self.lc, self.p2c = swap
         self._swap = cur
         self._rebind()
@@ -531,8 +539,8 @@ class FempicDevice:
         k = charge / mass * dt
         if getattr(self, "pending", False):  # the step reads its inputs through the pending sort
             swap = self._swap_buffers()
-            src = FempicParts(self.cap, self.pos.data_ptr(), self.vel.data_ptr(), self.lc.data_ptr(),
-                              self.p2c.data_ptr())
+            # p2c None: the sort wrote the sorted p2c into the target already
+            src = FempicParts(self.cap, self.pos.data_ptr(), self.vel.data_ptr(), self.lc.data_ptr(), None)
             dst = FempicParts(self.cap, swap[0].data_ptr(), swap[1].data_ptr(), swap[2].data_ptr(),
                               swap[3].data_ptr())
             check(lib().oppx_fempic_step_fused_gather(
