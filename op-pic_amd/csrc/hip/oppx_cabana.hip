@@ -4,11 +4,11 @@
 // current weighted into the 12 accumulator entries of the cell it crosses and the particle handed to
 // the face neighbour, until its track ends inside a cell.
 //
-// One lane per particle on a resident grid that strides over the particle range. Deposits go into
-// a workgroup LDS table keyed by cell (ds_add_f64), flushed with one device-scope atomic per
-// (cell, entry) after every tile: particles of one cell share a block, so this removes most of the
-// global atomics; a wave whose first evaluations all deposit into one cell sums them with shuffles
-// first. The arithmetic is the reference GCC build's, FMA for FMA (the file is built with
+// One lane per particle on a resident grid; each block walks one contiguous run of 256-particle
+// tiles. Deposits go into a workgroup LDS table keyed by cell (ds_add_f64), flushed with one
+// device-scope atomic per (cell, entry) every few tiles: particles of one cell share a block, so this
+// removes most of the global atomics; a wave whose first evaluations all deposit into one cell sums
+// them with shuffles first. The arithmetic is the reference GCC build's, FMA for FMA (the file is built with
 // -ffp-contract=off): the interpolated fields and the Boris cross products are FMA chains, the
 // half-advance products stay unfused (their second use lies past the errno branch GCC places around
 // sqrt), v5 = q*ux*uy*uz/3 is fused into all twelve accumulator terms and v1 = v4*d into both of
@@ -22,6 +22,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kCabTable = 64;  // cells per workgroup table (a 256-particle tile spans a few)
 constexpr int kCabProbe = 8;
+constexpr int kCabFlushTiles = 4;  // tiles of one block between table flushes
 
 struct CabArgs {
     int64_t n_cells;
@@ -121,6 +122,46 @@ __device__ __forceinline__ void cab_weight(double q, double ux, double uy, doubl
     cab_calc_j(q, w, uz, dx, dy, v + 8);
 }
 
+enum CabStreak { kCabOn = 0, kCabEnd = 1, kCabFault = 2 };
+
+// One streak of the track (kernels.h:262-353): the part of the remaining streak s up to the first
+// face it reaches (or all of it), its current weighted into out[12] for `cell`, the particle advanced
+// to the face, and -- when the track leaves the cell -- handed to the face neighbour (entering it
+// exactly on the opposite face). kCabEnd: the track ends inside `cell`; kCabFault: the neighbour index
+// is out of range or the walk reached OPPX_MAX_HOPS (`cell` keeps the cell the streak was in).
+__device__ __forceinline__ int cab_streak(const CabArgs& P, int& cell, double& px, double& py, double& pz, double& sx,
+                                          double& sy, double& sz, double q, int h, double* out) {
+    const double dirx = (sx > 0) ? 1.0 : -1.0, diry = (sy > 0) ? 1.0 : -1.0, dirz = (sz > 0) ? 1.0 : -1.0;
+    const double f0 = (sx == 0) ? 3.4e38 : (dirx - px) / sx;
+    const double f1 = (sy == 0) ? 3.4e38 : (diry - py) / sy;
+    const double f2 = (sz == 0) ? 3.4e38 : (dirz - pz) / sz;
+    double f = 2;
+    int axis = 3;
+    if (f0 < f) f = f0, axis = 0;
+    if (f1 < f) f = f1, axis = 1;
+    if (f2 < f) f = f2, axis = 2;
+    f *= 0.5;
+    const double dsx = sx * f, dsy = sy * f, dsz = sz * f;
+    cab_weight(q, dsx, dsy, dsz, px + dsx, py + dsy, pz + dsz, out);
+    sx -= dsx;
+    sy -= dsy;
+    sz -= dsz;
+    px += (dsx + dsx);
+    py += (dsy + dsy);
+    pz += (dsz + dsz);
+    if (axis == 3) return kCabEnd;
+    const double d = axis == 0 ? dirx : (axis == 1 ? diry : dirz);
+    const int face = axis + (d > 0 ? 3 : 0);
+    const int nb = P.c2c6[6 * (int64_t)cell + face];
+    // value selects, not stores through a chosen reference (which would put px/py/pz in scratch)
+    px = axis == 0 ? -d : px;
+    py = axis == 1 ? -d : py;
+    pz = axis == 2 ? -d : pz;
+    if (!((unsigned)nb < (unsigned)P.n_cells) || h + 1 >= OPPX_MAX_HOPS) return kCabFault;
+    cell = nb;
+    return kCabOn;
+}
+
 __global__ void __launch_bounds__(kBlock) cabana_move_deposit_kernel(CabArgs P) {
     __shared__ CabTable T;
     int64_t n = P.ctr->size;
@@ -131,21 +172,33 @@ __global__ void __launch_bounds__(kBlock) cabana_move_deposit_kernel(CabArgs P) 
     cab_table_clear(T);
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
-    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
-        const int64_t i = base + threadIdx.x;
+    // each block walks one contiguous run of tiles, so consecutive tiles share cells and the table is
+    // flushed only every kCabFlushTiles tiles (and after the last)
+    const int64_t tiles = (n + kBlock - 1) / kBlock;
+    const int64_t per = (tiles + gridDim.x - 1) / gridDim.x;
+    const int64_t t_begin = (int64_t)blockIdx.x * per;
+    const int64_t t_end = t_begin + per < tiles ? t_begin + per : tiles;
+    int since_flush = 0;
+    for (int64_t t = t_begin; t < t_end; t++) {
+        const int64_t i = t * kBlock + threadIdx.x;
         int h = 0;
-        // the deposit of a lane's first evaluation waits for the whole wave (see below)
+        // The first evaluation of every lane (push, then the move inside the cell or the first
+        // streak) ends with its deposit in v1st; the wave adds those together before any lane walks
+        // on, so v1st is dead during the later streaks.
         double v1st[12];
         int c1st = -1;
+        bool active = false, fault = false;
+        int cell = kMaxCell;
+        double px = 0, py = 0, pz = 0, sx = 0, sy = 0, sz = 0, q = 0;
         if (i < n) {
-            int cell = ld_nt(P.p2c + i);
+            cell = ld_nt(P.p2c + i);
             if (cell != kMaxCell) {
-                double px = ld_nt(P.pos + i), py = ld_nt(P.pos + S + i), pz = ld_nt(P.pos + 2 * S + i);
-                double sx = ld_nt(P.streak + i), sy = ld_nt(P.streak + S + i), sz = ld_nt(P.streak + 2 * S + i);
-                double ux0 = ld_nt(P.vel + i), uy0 = ld_nt(P.vel + S + i), uz0 = ld_nt(P.vel + 2 * S + i);
-                const double q = ld_nt(P.weight + i) * P.qsp;
-                bool fault = !((unsigned)cell < (unsigned)P.n_cells), done = false;
-                double v[12];
+                active = true;
+                px = ld_nt(P.pos + i); py = ld_nt(P.pos + S + i); pz = ld_nt(P.pos + 2 * S + i);
+                sx = ld_nt(P.streak + i); sy = ld_nt(P.streak + S + i); sz = ld_nt(P.streak + 2 * S + i);
+                const double ux0 = ld_nt(P.vel + i), uy0 = ld_nt(P.vel + S + i), uz0 = ld_nt(P.vel + 2 * S + i);
+                q = ld_nt(P.weight + i) * P.qsp;
+                fault = !((unsigned)cell < (unsigned)P.n_cells);
                 // first evaluation (OPP_DO_ONCE): Boris push, then either a move inside the cell
                 // or the first streak
                 if (!fault) {
@@ -191,76 +244,25 @@ __global__ void __launch_bounds__(kBlock) cabana_move_deposit_kernel(CabArgs P) 
                     v4 = v1 + uy;
                     const double v5 = v2 + uz;
                     h = 1;
+                    c1st = cell;
                     if (v3 <= 1.0 && v4 <= 1.0 && v5 <= 1.0 && -v3 <= 1.0 && -v4 <= 1.0 && -v5 <= 1.0) {
                         px = v3;
                         py = v4;
                         pz = v5;
                         cab_weight(q, ux, uy, uz, v0, v1, v2, v1st);
-                        c1st = cell;
-                        done = true;
+                        active = false;  // the track ends inside the cell
                     } else {
                         sx = ux;
                         sy = uy;
                         sz = uz;
+                        const int st = cab_streak(P, cell, px, py, pz, sx, sy, sz, q, h, v1st);
+                        active = st == kCabOn;
+                        fault = st == kCabFault;
                     }
-                }
-                // streaks: each evaluation deposits one streak and hops across the face it ends on
-                bool first_streak = true;
-                while (!done && !fault) {
-                    if (!first_streak) h++;
-                    const double dirx = (sx > 0) ? 1.0 : -1.0, diry = (sy > 0) ? 1.0 : -1.0, dirz = (sz > 0) ? 1.0 : -1.0;
-                    const double f0 = (sx == 0) ? 3.4e38 : (dirx - px) / sx;
-                    const double f1 = (sy == 0) ? 3.4e38 : (diry - py) / sy;
-                    const double f2 = (sz == 0) ? 3.4e38 : (dirz - pz) / sz;
-                    double f = 2;
-                    int axis = 3;
-                    if (f0 < f) f = f0, axis = 0;
-                    if (f1 < f) f = f1, axis = 1;
-                    if (f2 < f) f = f2, axis = 2;
-                    f *= 0.5;
-                    const double dsx = sx * f, dsy = sy * f, dsz = sz * f;
-                    const double mx = px + dsx, my = py + dsy, mz = pz + dsz;
-                    if (first_streak) {  // still the first evaluation
-                        cab_weight(q, dsx, dsy, dsz, mx, my, mz, v1st);
-                        c1st = cell;
-                    } else {
-                        cab_weight(q, dsx, dsy, dsz, mx, my, mz, v);
-                        cab_table_add(T, P.acc, cell, v);
-                    }
-                    first_streak = false;
-                    sx -= dsx;
-                    sy -= dsy;
-                    sz -= dsz;
-                    px += (dsx + dsx);
-                    py += (dsy + dsy);
-                    pz += (dsz + dsz);
-                    if (axis == 3) break;
-                    const double d = axis == 0 ? dirx : (axis == 1 ? diry : dirz);
-                    const int face = axis + (d > 0 ? 3 : 0);
-                    const int nb = P.c2c6[6 * (int64_t)cell + face];
-                    // the entry point is exactly on the opposite face of the neighbour
-                    if (axis == 0) px = -d;
-                    if (axis == 1) py = -d;
-                    if (axis == 2) pz = -d;
-                    if (!((unsigned)nb < (unsigned)P.n_cells) || h + 1 >= OPPX_MAX_HOPS) {
-                        fault = true;
-                        break;
-                    }
-                    cell = nb;
-                }
-                if (fault) {
-                    nfault++;
                 } else {
-                    st_nt(P.pos + i, px);
-                    st_nt(P.pos + S + i, py);
-                    st_nt(P.pos + 2 * S + i, pz);
-                    st_nt(P.streak + i, sx);
-                    st_nt(P.streak + S + i, sy);
-                    st_nt(P.streak + 2 * S + i, sz);
-                    st_nt(P.p2c + i, cell);
+                    active = false;
                 }
             }
-            if (P.hops) P.hops[i] = h;
         }
         // First-evaluation deposits: in cell order nearly every wave's particles share one cell, and
         // 64 lanes adding to the same 12 LDS words would serialise; such a wave sums its 12 values
@@ -282,11 +284,40 @@ __global__ void __launch_bounds__(kBlock) cabana_move_deposit_kernel(CabArgs P) 
                 cab_table_add(T, P.acc, c1st, v1st);
             }
         }
+        // later streaks: each evaluation deposits one streak and hops across the face it ends on
+        while (active) {
+            h++;
+            double v[12];
+            const int c = cell;
+            const int st = cab_streak(P, cell, px, py, pz, sx, sy, sz, q, h, v);
+            cab_table_add(T, P.acc, c, v);
+            active = st == kCabOn;
+            fault = st == kCabFault;
+        }
+        if (c1st >= 0) {  // a particle that was moved
+            if (fault) {
+                nfault++;
+            } else {
+                st_nt(P.pos + i, px);
+                st_nt(P.pos + S + i, py);
+                st_nt(P.pos + 2 * S + i, pz);
+                st_nt(P.streak + i, sx);
+                st_nt(P.streak + S + i, sy);
+                st_nt(P.streak + 2 * S + i, sz);
+                st_nt(P.p2c + i, cell);
+            }
+        } else if (fault) {
+            nfault++;  // a cell index out of range on entry
+        }
+        if (i < n && P.hops) P.hops[i] = h;
         hsum += h;
         hmax = hmax > h ? hmax : h;
-        __syncthreads();
-        cab_table_flush(T, P.acc);
-        __syncthreads();
+        if (++since_flush == kCabFlushTiles || t + 1 == t_end) {  // uniform over the block
+            __syncthreads();
+            cab_table_flush(T, P.acc);
+            __syncthreads();
+            since_flush = 0;
+        }
     }
     __shared__ long long s_sum[kBlock / kWave];
     __shared__ int s_max[kBlock / kWave], s_flt[kBlock / kWave];
