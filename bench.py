@@ -543,10 +543,57 @@ def bench_cabana(args):
         oracle.cabana_move_deposit(c2c6, interp, hacc, hp, hv, hs, weight[sl].copy(), hc, deck["qsp"],
                                    deck["qdt_2mc"], deck["cdt_d"])
         dt_cpu = time.perf_counter() - t2
-        out["cpu_baseline"] = {"value": m / dt_cpu, "unit": "particles/s", "cores": 1, "kind": "port", "cpu": cpu_model(),
-                               "sample": f"one step of the first {m} particles"}
-        log(f"cpu baseline {m / dt_cpu:.3e} particles/s (1 thread)")
+        port = {"value": m / dt_cpu, "unit": "particles/s", "cores": 1, "kind": "port", "cpu": cpu_model(),
+                "sample": f"the oracle, one step of the first {m} particles"}
+        log(f"cpu oracle {m / dt_cpu:.3e} particles/s (1 thread)")
+        ref = cabana_cpu_reference(c2c6, interp, pos[sl], vel[sl], weight[sl], p2c[sl], deck)
+        if ref is not None:
+            log(f"cpu reference {ref['value']:.3e} particles/s (1 thread)")
+            out["cpu_baseline"] = ref
+            out["cpu_port_baseline"] = port
+        else:
+            out["cpu_baseline"] = port
     return out
+
+
+def cabana_cpu_reference(c2c6, interp, pos, vel, weight, p2c, deck, steps=4):
+    """The reference's own seq build of the CabanaPIC move-deposit loop (oracle/_ref/cabana_ref_seq,
+    compiled from the reference sources where they were available) on the given particles over the
+    full mesh, `steps` steps of accumulator reset + move-deposit; only those loops are timed. None
+    when the build is absent."""
+    import subprocess
+    import tempfile
+
+    binary = os.path.join(ROOT, "oracle", "_ref", "cabana_ref_seq")
+    if not os.path.exists(binary):
+        return None
+    n, nc = int(len(p2c)), int(len(c2c6))
+    with tempfile.TemporaryDirectory() as td:
+        ind, outd = os.path.join(td, "in"), os.path.join(td, "out")
+        os.makedirs(ind)
+        os.makedirs(outd)
+        for a, name, dt_ in ((c2c6, "c2ngc", np.int32), (interp, "c_interp", np.float64), (pos, "p_pos", np.float64),
+                             (vel, "p_vel", np.float64), (weight, "p_weight", np.float64), (p2c, "p2c", np.int32)):
+            np.ascontiguousarray(a, dtype=dt_).tofile(os.path.join(ind, name + ".bin"))
+        lines = [f"INT ncells = {nc}", f"INT nparts = {n}", f"INT num_steps = {steps}", f"REAL qsp = {deck['qsp']!r}",
+                 f"REAL qdt_half_mc = {deck['qdt_2mc']!r}", f"REAL cdt_dx = {deck['cdt_d'][0]!r}",
+                 f"REAL cdt_dy = {deck['cdt_d'][1]!r}", f"REAL cdt_dz = {deck['cdt_d'][2]!r}",
+                 "REAL opp_allocation_multiple = 1.1", "BOOL opp_profile_all = false",
+                 "STRING opp_fill = HoleFill_All", "INT opp_fill_period = 4", "BOOL opp_global_move = false"]
+        par = os.path.join(td, "case.param")
+        with open(par, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        try:
+            r = subprocess.run([binary, par, ind, outd], capture_output=True, text=True, timeout=240)
+        except subprocess.TimeoutExpired:
+            return None
+        if r.returncode != 0:
+            log(f"reference CPU baseline failed (rc {r.returncode}): {r.stderr[-300:]}")
+            return None
+        loop_s = float(open(os.path.join(outd, "loop_seconds.txt")).read())
+    return {"value": n * steps / loop_s, "unit": "particles/s", "cores": 1, "kind": "reference", "cpu": cpu_model(),
+            "sample": f"the reference's seq build, {steps} steps of the first {n} particles over the full mesh, "
+                      f"{loop_s:.2f}s of loops"}
 
 
 def bench_serve(args):

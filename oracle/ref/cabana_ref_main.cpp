@@ -10,6 +10,7 @@
 //
 // usage: cabana_ref <params> <in_dir> <out_dir>
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -92,13 +93,16 @@ int main(int argc, char** argv) {
     }
 
     FILE* stats = fopen((out + "/steps.txt").c_str(), "w");
+    double loop_s = 0.0;  // the step's loops only (accumulator reset + move-deposit), for CPU timing
     for (OPP_main_loop_iter = 0; OPP_main_loop_iter < steps; OPP_main_loop_iter++) {
         oppx_ref_hops.assign((size_t)p_set->set_capacity + 1, 0);
+        const auto t_step = std::chrono::steady_clock::now();
         opp_reset_dat(c_acc, opp_zero_double16);
         opp_particle_move__move_deposit_kernel(p_set, c2ngc_map, p2c_map, opp_arg_dat(vel, OPP_RW),
                                                opp_arg_dat(pos, OPP_RW), opp_arg_dat(streak, OPP_RW),
                                                opp_arg_dat(weight, OPP_READ), opp_arg_dat(c_interp, p2c_map, OPP_READ),
                                                opp_arg_dat(c_acc, p2c_map, OPP_INC));
+        loop_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_step).count();
         long long tot = 0;
         int mx = 0;
         for (int i = 0; i < n_part; i++) { tot += oppx_ref_hops[i]; mx = std::max(mx, oppx_ref_hops[i]); }
@@ -115,6 +119,10 @@ int main(int argc, char** argv) {
         }
     }
     fclose(stats);
+    if (FILE* tf = fopen((out + "/loop_seconds.txt").c_str(), "w")) {
+        fprintf(tf, "%.9f\n", loop_s);
+        fclose(tf);
+    }
     opp_exit();
     return 0;
 }
