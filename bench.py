@@ -382,7 +382,7 @@ def bench_neso(args):
     g_wall, g_ms, _, _ = run(False, min(args.steps, 16), 2)
     alg = 64.0 * n + 32.0 * nx * nx  # R pos 16, vel 16, mdir 8, p2c 4; W pos 16, mdir 8, p2c 4 (B/particle)
     achieved = alg / (ms.mean() * 1e-3) / 1e9
-    return {
+    out = {
         "metric": "NESO-advection particle moves/s (update + multi-hop move)",
         "value": n * args.steps / wall,
         "unit": "particles/s",
@@ -407,6 +407,51 @@ def bench_neso(args):
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": load_traffic("c3", "neso_step_kernel"), "measured_copy_gbs": copy_gbs},
     }
+    if not args.no_cpu:
+        k = min(n, 4_000_000)
+        ref = neso_cpu_reference(m, pos[:k], vel[:k], p2c[:k], nx, cw, dt)
+        if ref is not None:
+            log(f"cpu reference {ref['value']:.3e} particles/s (1 thread)")
+            out["cpu_baseline"] = ref
+    return out
+
+
+def neso_cpu_reference(m, pos, vel, p2c, nx, cw, dt, steps=4):
+    """The reference's own seq build of the NESO-advection mdir loops (oracle/_ref/neso_ref_seq,
+    compiled from the reference sources where they were available) on the given particles over the
+    full mesh, `steps` steps of update_pos + move; only those loops are timed. None when the build
+    is absent."""
+    import subprocess
+    import tempfile
+
+    binary = os.path.join(ROOT, "oracle", "_ref", "neso_ref_seq")
+    if not os.path.exists(binary):
+        return None
+    n = int(len(p2c))
+    with tempfile.TemporaryDirectory() as td:
+        ind, outd = os.path.join(td, "in"), os.path.join(td, "out")
+        os.makedirs(ind)
+        os.makedirs(outd)
+        for a, name, dt_ in ((m["c_pos_ll"], "c_pos_ll", np.float64), (m["c2c"], "c2c", np.int32),
+                             (pos, "p_pos", np.float64), (vel, "p_vel", np.float64), (p2c, "p2c", np.int32)):
+            np.ascontiguousarray(a, dtype=dt_).tofile(os.path.join(ind, name + ".bin"))
+        par = os.path.join(td, "case.param")
+        with open(par, "w") as f:
+            f.write("\n".join([f"INT nx = {nx}", f"INT ny = {nx}", f"INT nparts = {n}", f"INT num_steps = {steps}",
+                               f"REAL dt = {dt!r}", f"REAL cell_width = {cw!r}", "REAL opp_allocation_multiple = 1.1",
+                               "BOOL opp_global_move = false", "STRING opp_fill = HoleFill_All",
+                               "BOOL opp_profile_all = false"]) + "\n")
+        try:
+            r = subprocess.run([binary, par, ind, outd], capture_output=True, text=True, timeout=240)
+        except subprocess.TimeoutExpired:
+            return None
+        if r.returncode != 0:
+            log(f"reference CPU baseline failed (rc {r.returncode}): {r.stderr[-300:]}")
+            return None
+        loop_s = float(open(os.path.join(outd, "loop_seconds.txt")).read())
+    return {"value": n * steps / loop_s, "unit": "particles/s", "cores": 1, "kind": "reference", "cpu": cpu_model(),
+            "sample": f"the reference's seq build, {steps} steps of the first {n} particles over the full mesh, "
+                      f"{loop_s:.2f}s of loops"}
 
 
 def cabana_deck(nx=40, ny=40, nz=60, nppc=750, v0=0.0866025403784439, c=1.0, qsp=-1.0, me=1.0, grid_len=1.0):

@@ -8,6 +8,7 @@
 //
 // usage: neso_ref <params> <in_dir> <out_dir>
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -92,11 +93,14 @@ int main(int argc, char** argv) {
     }
 
     FILE* stats = fopen((out + "/steps.txt").c_str(), "w");
+    double loop_s = 0.0;  // the step's loops only (update_pos + move), for CPU timing
     for (OPP_main_loop_iter = 0; OPP_main_loop_iter < steps; OPP_main_loop_iter++) {
         oppx_ref_hops.assign((size_t)part_set->set_capacity + 1, 0);
+        const auto t_step = std::chrono::steady_clock::now();
         opp_par_loop_all__update_pos_kernel(part_set, opp_arg_dat(p_vel_d, OPP_READ), opp_arg_dat(p_pos_d, OPP_RW));
         opp_particle_move__move_kernel(part_set, c2c_map, p2c_map, opp_arg_dat(p_pos_d, OPP_READ),
                                        opp_arg_dat(p_mdir, OPP_RW), opp_arg_dat(c_pos_ll_d, p2c_map, OPP_READ));
+        loop_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_step).count();
         long long tot = 0;
         int mx = 0;
         for (int i = 0; i < part_set->size; i++) { tot += oppx_ref_hops[i]; mx = std::max(mx, oppx_ref_hops[i]); }
@@ -112,6 +116,10 @@ int main(int argc, char** argv) {
     write_arr<double>(out + "/final_pos.bin", (const double*)p_pos_d->data, 2 * (size_t)part_set->size);
     write_arr<int>(out + "/final_p2c.bin", (const int*)p2c_map->p2c_dat->data, (size_t)part_set->size);
     write_arr<int>(out + "/final_mdir.bin", (const int*)p_mdir->data, 2 * (size_t)part_set->size);
+    if (FILE* tf = fopen((out + "/loop_seconds.txt").c_str(), "w")) {
+        fprintf(tf, "%.9f\n", loop_s);
+        fclose(tf);
+    }
     opp_exit();
     return 0;
 }
