@@ -63,6 +63,45 @@ def name_of(g):
     return "fempic_" + str(g["meta_case"]) + ".npz"
 
 
+def _fempic_api_run(g, m, fill, td):
+    ind, outd = os.path.join(td, "in_" + fill), os.path.join(td, "out_" + fill)
+    os.makedirs(ind)
+    os.makedirs(outd)
+    for k in ["node_pos", "c2n", "c2c", "c_det", "c_vol"]:
+        m[k].tofile(os.path.join(ind, k + ".bin"))
+    g["in_c_ef"].tofile(os.path.join(ind, "c_ef.bin"))
+    g["in_pos"].tofile(os.path.join(ind, "p_pos.bin"))
+    g["in_vel"].tofile(os.path.join(ind, "p_vel.bin"))
+    g["in_p2c"].astype(np.int32).tofile(os.path.join(ind, "p2c.bin"))
+    par = os.path.join(td, fill + ".param")
+    fempic_params(par, g, m, g["in_pos"].shape[0], fill)
+    run_app(app("fempic_api_app"), par, ind, outd)
+    rd = lambda k, t, w: np.fromfile(os.path.join(outd, k + ".bin"), dtype=t).reshape(-1, w)
+    recs = np.hstack([rd("final_p2c", np.int32, 1).astype(np.float64), rd("final_pos", np.float64, 3),
+                      rd("final_vel", np.float64, 3), rd("final_lc", np.float64, 4)])
+    st = np.loadtxt(os.path.join(outd, "steps.txt"), ndmin=2).astype(np.int64)
+    return recs, st
+
+
+@pytest.mark.parametrize("fill", ["Sort_All", "Sort_Periodic"])
+def test_fempic_api_sort_finalize(fill):
+    """The sort finalize through the opp_lib API (the p2c dat written from the sort's keys, the other
+    dats gathered): every particle's record evolves as under hole fill, so the final records are the
+    same multiset bit for bit; Sort_All leaves p2c non-decreasing; counts and hops match the reference."""
+    g = np.load(golden_path("fempic_small_dh_omp1.npz"), allow_pickle=False)
+    n = int(g["meta_nx"])
+    m = oppx.box_mesh(n, n, n)
+    with tempfile.TemporaryDirectory() as td:
+        base, st0 = _fempic_api_run(g, m, "HoleFill_All", td)
+        recs, st = _fempic_api_run(g, m, fill, td)
+    ref = g["out_steps_iter_before_after_hops_maxhops"]
+    assert np.array_equal(st[:, [0, 1, 2, 3]], ref[:, [0, 1, 2, 4]]) and np.array_equal(st, st0)
+    order = lambda a: a[np.lexsort(a.T[::-1])]
+    assert recs.shape == base.shape and np.array_equal(order(recs), order(base))
+    if fill == "Sort_All":
+        assert np.all(np.diff(recs[:, 0]) >= 0)
+
+
 @pytest.mark.parametrize("name,generate", [("fempic_tiny_seq.npz", True), ("fempic_small_dh_omp1.npz", True),
                                            ("fempic_small_dh_omp1.npz", False)])
 def test_fempic_api_app_matches_reference(name, generate):
