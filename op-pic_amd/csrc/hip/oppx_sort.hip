@@ -1,15 +1,17 @@
 // Particle sort by cell on gfx950: stable LSD radix sort of (cell, index) pairs, then one
-// permutation gather per particle dat.
+// permutation gather per particle dat (the p2c dat itself is written from the sorted keys).
 //
 // Reference: opp_particle_sort / the Sort_* fill policies (opp_lib/src/core/opp_particle_organize_core.cpp:98-121,
 // opp_lib/src/omp/opp_particle_mover.cpp:223-265, hip opp_lib/src/hip/opp_particle_sorter.cpp:188-267) order the
 // particles by p2c with removed particles (MAX_CELL_INDEX) last. Their tie order differs between backends
 // (std::sort / thrust radix); this build's contract is stable by (p2c, index).
 //
-// Each pass sorts by one 8-bit digit. A block owns 256 * sort_items consecutive elements (tuning); ranks inside the block
-// come from wave-level digit matching (8 ballots give the lanes holding the same digit), so equal digits
-// keep their index order and no LDS atomic is contended by nearly sorted keys. Per-block digit counts
-// are scanned digit-major across blocks to give every (digit, block) its output offset.
+// Each pass sorts by one 8-bit digit. A block owns 256 * sort_items consecutive elements (tuning),
+// each wave a contiguous part of them; ranks come from wave-level digit matching (8 ballots give the
+// lanes holding the same digit) against the wave's own digit counts, so equal digits keep their index
+// order and no LDS atomic is contended by nearly sorted keys. Per-block digit counts (a histogram
+// pass that needs no ranks) are scanned digit-major across blocks to give every (digit, block) its
+// output offset.
 
 #include "oppx_internal.h"
 
