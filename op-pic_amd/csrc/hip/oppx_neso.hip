@@ -140,10 +140,12 @@ __device__ __forceinline__ int neso_particle(const NesoArgs& P, int flags, int64
     return h;
 }
 
-// Resident grid, grid-stride over the particles, two particles per lane per iteration (both loaded
-// before either is walked, so each lane keeps two particles' loads in flight); the per-block
-// statistics then reach the counters from a few thousand blocks instead of one block per 256
-// particles (same-address atomics serialise).
+// Resident grid, grid-stride over the particles, kNesoPerLane particles per lane per iteration (all
+// loaded before any is walked, so each lane keeps that many particles' loads in flight); the
+// per-block statistics then reach the counters from a few thousand blocks instead of one block per
+// 256 particles (same-address atomics serialise).
+constexpr int kNesoPerLane = 4;
+
 template <bool GRID>
 __global__ void __launch_bounds__(kBlock) neso_step_kernel(NesoArgs P, int flags) {
     int64_t n = P.ctr->size;
@@ -151,23 +153,24 @@ __global__ void __launch_bounds__(kBlock) neso_step_kernel(NesoArgs P, int flags
     long long hsum = 0;
     int hmax = 0, nfault = 0;
     const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += 2 * stride) {
-        const int64_t j = i + stride;
-        const NesoPart a = neso_load(P, flags, i);
-        NesoPart b{};
-        if (j < n) b = neso_load(P, flags, j);
-        bool fa = false, fb = false;
-        const int ha = neso_particle<GRID>(P, flags, i, a, fa);
-        if (P.hops && (flags & OPPX_NESO_MOVE)) P.hops[i] = ha;
-        hsum += ha;
-        hmax = hmax > ha ? hmax : ha;
-        nfault += fa ? 1 : 0;
-        if (j < n) {
-            const int hb = neso_particle<GRID>(P, flags, j, b, fb);
-            if (P.hops && (flags & OPPX_NESO_MOVE)) P.hops[j] = hb;
-            hsum += hb;
-            hmax = hmax > hb ? hmax : hb;
-            nfault += fb ? 1 : 0;
+    for (int64_t i0 = (int64_t)blockIdx.x * kBlock + threadIdx.x; i0 < n; i0 += kNesoPerLane * stride) {
+        NesoPart part[kNesoPerLane];
+#pragma unroll
+        for (int u = 0; u < kNesoPerLane; u++) {
+            const int64_t i = i0 + u * stride;
+            part[u] = i < n ? neso_load(P, flags, i) : NesoPart{};
+        }
+#pragma unroll
+        for (int u = 0; u < kNesoPerLane; u++) {
+            const int64_t i = i0 + u * stride;
+            if (i < n) {
+                bool f = false;
+                const int h = neso_particle<GRID>(P, flags, i, part[u], f);
+                if (P.hops && (flags & OPPX_NESO_MOVE)) P.hops[i] = h;
+                hsum += h;
+                hmax = hmax > h ? hmax : h;
+                nfault += f ? 1 : 0;
+            }
         }
     }
     __shared__ long long s_sum[kBlock / kWave];
