@@ -145,7 +145,8 @@ def cpu_baseline_reference(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, 
     return dict(value=moved / loop_s, unit="particles/s", cores=threads, kind="reference", cpu=cpu_model(),
                 sample=f"the reference's OpenMP build ({threads} threads, OMP_PROC_BIND=close) on one particle in {stride} "
                        f"of the same input ({n} particles, the full {m['n_cells']}-tet mesh, overlay read "
-                       f"from file), {steps} steps of push + move + deposit + hole fill, {loop_s:.2f}s of loops")
+                       f"from file), {steps} steps of push + move + deposit + hole fill, {loop_s:.2f}s of loops",
+                particle_steps=moved, loop_seconds=loop_s)
 
 
 def load_traffic(workload_tag, kernel="fempic_first_kernel"):
@@ -310,7 +311,11 @@ def bench_ensemble(args):
     t_plain = time.perf_counter() - t1
     plain = float(sum(live)) / t_plain
     log(f"plain per-instance launches: {t_plain * 1e3:.1f} ms -> {plain:.3e} particles/s")
-    return {
+    # the move phase's algorithmic bytes: 136 B per live particle (as the fused step) + 176 B per tet
+    # (c_det 128, c_vol 8, c2c 16, c_ef 24; no deposit in this pass)
+    alg = np.array([136.0 * lv + 176.0 * n_tets for lv in live])
+    achieved = float(alg.sum() / (ph[:, 0].sum() * 1e-3) / 1e9)
+    out = {
         "metric": "ensemble particles/s (batched graph launch vs plain per-instance launches)",
         "value": value,
         "unit": "particles/s",
@@ -330,7 +335,35 @@ def bench_ensemble(args):
                             "fp32_deposit": float(np.median(ph[:, 2]))},
         "fp32_deposit_err_vs_fp64": f32_err,
         "autotune_table": [[c, round(ms, 4)] for c, ms in table],
+        "roofline": {"kernel": "fempic_first_kernel (batched move phase, eager steps)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None},
     }
+    if not args.no_cpu:
+        # the reference's OpenMP build on the first instances (about 16 M particles), S steps each
+        moved = secs = 0.0
+        used = parts = 0
+        for d in inst:
+            if used and parts + len(d["p2c"]) > 16_000_000:
+                break
+            q = float(d["charge"])
+            r = cpu_baseline_reference(d["mesh"], d["ef"], d["pos"], d["vel"], d["p2c"], d["overlay"], d["bmin"],
+                                       d["bmax"], d["gs"], q, float(d["mass"]), float(d["dt"]), steps=S, stride=1)
+            if r is None:
+                moved = 0.0
+                break
+            moved += r["particle_steps"]
+            secs += r["loop_seconds"]
+            used += 1
+            parts += len(d["p2c"])
+        if moved > 0:
+            out["cpu_baseline"] = {"value": moved / secs, "unit": "particles/s", "cores": r["cores"], "kind": "reference",
+                                   "cpu": cpu_model(),
+                                   "sample": f"the reference's OpenMP build ({r['cores']} threads) on the first {used} "
+                                             f"instances ({parts} particles), {S} steps each of push + move + FP64 "
+                                             f"deposit + hole fill, {secs:.2f}s of loops"}
+            log(f"cpu reference {moved / secs:.3e} particles/s on {used} instances")
+    return out
 
 
 def bench_neso(args):
