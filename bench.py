@@ -340,30 +340,41 @@ def bench_ensemble(args):
                      "traffic": None},
     }
     if not args.no_cpu:
-        # the reference's OpenMP build on the first instances (about 16 M particles), S steps each
-        moved = secs = 0.0
-        used = parts = 0
-        for d in inst:
-            if used and parts + len(d["p2c"]) > 16_000_000:
-                break
-            q = float(d["charge"])
-            r = cpu_baseline_reference(d["mesh"], d["ef"], d["pos"], d["vel"], d["p2c"], d["overlay"], d["bmin"],
-                                       d["bmax"], d["gs"], q, float(d["mass"]), float(d["dt"]), steps=S, stride=1)
-            if r is None:
-                moved = 0.0
-                break
-            moved += r["particle_steps"]
-            secs += r["loop_seconds"]
-            used += 1
-            parts += len(d["p2c"])
-        if moved > 0:
-            out["cpu_baseline"] = {"value": moved / secs, "unit": "particles/s", "cores": r["cores"], "kind": "reference",
-                                   "cpu": cpu_model(),
-                                   "sample": f"the reference's OpenMP build ({r['cores']} threads) on the first {used} "
-                                             f"instances ({parts} particles), {S} steps each of push + move + FP64 "
-                                             f"deposit + hole fill, {secs:.2f}s of loops"}
-            log(f"cpu reference {moved / secs:.3e} particles/s on {used} instances")
+        ref = reference_on_instances([(d, S) for d in inst], "instances")
+        if ref is not None:
+            out["cpu_baseline"] = ref
     return out
+
+
+def reference_on_instances(runs, what, max_particles=16_000_000, max_runs=64):
+    """CPU baseline of the multi-instance workloads: the reference's OpenMP build
+    (cpu_baseline_reference) stepping (instance, steps) pairs of `runs` in order, one after another,
+    skipping those that would take the sample past max_particles particles (at most max_runs runs);
+    particle steps over loop seconds, summed."""
+    moved = secs = 0.0
+    used = parts = 0
+    r = None
+    for d, steps in runs:
+        if used >= max_runs:
+            break
+        if used and parts + len(d["p2c"]) > max_particles:
+            continue
+        r = cpu_baseline_reference(d["mesh"], d["ef"], d["pos"], d["vel"], d["p2c"], d["overlay"], d["bmin"],
+                                   d["bmax"], d["gs"], float(d["charge"]), float(d["mass"]), float(d["dt"]),
+                                   steps=steps, stride=1)
+        if r is None:
+            return None
+        moved += r["particle_steps"]
+        secs += r["loop_seconds"]
+        used += 1
+        parts += len(d["p2c"])
+    if r is None or secs <= 0:
+        return None
+    log(f"cpu reference {moved / secs:.3e} particles/s on {used} {what}")
+    return {"value": moved / secs, "unit": "particles/s", "cores": r["cores"], "kind": "reference", "cpu": cpu_model(),
+            "sample": f"the reference's OpenMP build ({r['cores']} threads) on {used} {what}, taken in order ({parts} "
+                      f"particles), one after another, each for its steps of push + move + FP64 deposit + hole "
+                      f"fill, {secs:.2f}s of loops"}
 
 
 def bench_neso(args):
@@ -705,7 +716,7 @@ def bench_serve(args):
     reqs_plain = [serve.Request(r.rid, r.arrival, r.tpl, r.steps) for r in reqs]
     _, sp = serve.serve_plain(reqs_plain)
     log(f"plain per-request launches: {sp}")
-    return {
+    out = {
         "metric": "serving particles/s (continuous batching), p50/p99 segment latency",
         "value": st["value"],
         "unit": "particles/s",
@@ -726,6 +737,11 @@ def bench_serve(args):
         "host_s": {k: st[k] for k in ("admit_s", "step_s", "retire_s")},
         "speedup_vs_plain": st["value"] / sp["value"],
     }
+    if not args.no_cpu:
+        ref = reference_on_instances([(r.tpl.inst, r.steps) for r in reqs], "requests")
+        if ref is not None:
+            out["cpu_baseline"] = ref
+    return out
 
 
 def main():
