@@ -5,7 +5,8 @@ have arrived and their state fits both the arena's free ranges and the HBM budge
 iteration then runs ONE batched step over all resident requests (batched push/move, Sort_All
 finalize, FP32 compensated deposit: a fixed number of launches whatever the mix), and requests that
 have done their steps are handed back and their ranges freed at once, so later arrivals join the
-running batch between iterations instead of waiting for it to drain.
+running batch between iterations instead of waiting for it to drain. The bulk of a retired request's
+state goes back to the host on a copy stream while the next iteration's step runs.
 
 Arena layout (device): cells, nodes and overlay words are ranges allocated first-fit per request;
 every cell, node and p2c index is stored concatenated (offset by the request's base); particles of
@@ -158,6 +159,8 @@ class FempicArena:
         self.cells, self.nodes, self.ovl = RangeAlloc(C), RangeAlloc(N), RangeAlloc(max(1, max_overlay))
         self.slots_free = list(range(max_insts - 1, -1, -1))
         self.res = {}  # slot -> (template, cb, nb, ob)
+        # results of retired requests go to the host on their own stream, beside the next step
+        self.copy_stream = torch.cuda.Stream(device=self.dev)
         self.cap, self.size = P, 0
 
     def stream(self):
@@ -267,13 +270,20 @@ class FempicArena:
         self.size = int(self.counters[0].item())  # the live count after the finalize (host sync)
 
     def retire(self, slots) -> list:
-        """Hand back requests' states (each contiguous after the sort) and free their ranges. All range
-        lookups happen before any particle is marked removed, so the set is still sorted for them."""
+        """Hand back requests' states (each contiguous after the sort) and free their ranges."""
+        return self.retire_async(slots).wait()
+
+    def retire_async(self, slots) -> "PendingResults":
+        """retire() with the bulk of the copy left running: p2c and the node charge come back on the
+        arena's stream (their ranges are reused at once), pos/vel/lc on the copy stream. Those stay
+        untouched until the second step from now, whose sort writes their buffer: call wait() before
+        that step. All range lookups happen before any particle is marked removed, so the set is
+        still sorted for them."""
         L, s = lib(), self.stream()
         torch = self.torch
         slots = list(slots)
         if not slots:
-            return []
+            return PendingResults([], None)
         keys = []
         for slot in slots:
             tpl, cb, nb, ob = self.res[slot]
@@ -282,7 +292,9 @@ class FempicArena:
         bounds = torch.zeros(len(keys), dtype=torch.int64, device=self.dev)
         check(L.oppx_lower_bound(self.size, self.counters.data_ptr(), self.p2c.data_ptr(), k.data_ptr(), len(keys),
                                  bounds.data_ptr(), s), "lower_bound")
-        b = bounds.cpu().numpy()
+        b = bounds.cpu().numpy()  # the step is complete from here on
+        main = torch.cuda.current_stream(self.dev)
+        self.copy_stream.wait_stream(main)
         staged = []
         pat = np.array([MAX_CELL_INDEX], np.int32)
         for j, slot in enumerate(slots):
@@ -292,11 +304,12 @@ class FempicArena:
             pin = lambda shape, dt: torch.empty(shape, dtype=dt, pin_memory=True)
             out = dict(pos=pin((3, k), torch.float64), vel=pin((3, k), torch.float64), lc=pin((4, k), torch.float64),
                        p2c=pin((k,), torch.int32), charge_f32=pin((tpl.n_nodes,), torch.float32))
-            out["pos"].copy_(self.pos[:, lo:hi], non_blocking=True)
-            out["vel"].copy_(self.vel[:, lo:hi], non_blocking=True)
-            out["lc"].copy_(self.lc[:, lo:hi], non_blocking=True)
             out["p2c"].copy_(self.p2c[lo:hi], non_blocking=True)
             out["charge_f32"].copy_(self.charge_f32[nb:nb + tpl.n_nodes], non_blocking=True)
+            with torch.cuda.stream(self.copy_stream):
+                out["pos"].copy_(self.pos[:, lo:hi], non_blocking=True)
+                out["vel"].copy_(self.vel[:, lo:hi], non_blocking=True)
+                out["lc"].copy_(self.lc[:, lo:hi], non_blocking=True)
             staged.append((out, k, cb))
             if k:  # leaves the particle set at the next sort
                 check(L.oppx_fill_pattern(self.p2c.data_ptr() + 4 * lo, k, 1, 4, pat.ctypes.data, 0, s),
@@ -311,9 +324,10 @@ class FempicArena:
             if ob >= 0:
                 self.ovl.release(ob)
             self.slots_free.append(slot)
-        torch.cuda.current_stream(self.dev).synchronize()
-        return [dict(n=k, pos=o["pos"].numpy().T, vel=o["vel"].numpy().T, lc=o["lc"].numpy().T,
-                     p2c=o["p2c"].numpy() - cb, charge_f32=o["charge_f32"].numpy()) for o, k, cb in staged]
+        done = (torch.cuda.Event(), torch.cuda.Event())
+        done[0].record(main)
+        done[1].record(self.copy_stream)
+        return PendingResults(staged, done)
 
     @staticmethod
     def for_budget(arena_bytes: float, device="cuda", max_insts: int = 4096) -> "FempicArena":
@@ -325,6 +339,22 @@ class FempicArena:
                            max_overlay=2 * cells, max_insts=max_insts, device=device)
 
 
+class PendingResults:
+    """Results of retire_async(): counts at once, the arrays after wait()."""
+
+    def __init__(self, staged, events):
+        self.staged, self.events = staged, events
+        self.counts = [k for _, k, _ in staged]
+
+    def wait(self) -> list:
+        if self.events is not None:
+            for e in self.events:
+                e.synchronize()
+            self.events = None
+        return [dict(n=k, pos=o["pos"].numpy().T, vel=o["vel"].numpy().T, lc=o["lc"].numpy().T,
+                     p2c=o["p2c"].numpy() - cb, charge_f32=o["charge_f32"].numpy()) for o, k, cb in self.staged]
+
+
 @dataclass
 class Request:
     rid: int
@@ -334,7 +364,8 @@ class Request:
     slot: int = -1
     left: int = 0
     t_admit: float = 0.0
-    t_done: float = 0.0
+    t_retire: float = 0.0  # ranges and budget freed
+    t_done: float = 0.0    # results on the host
     result: dict = field(default_factory=dict)
 
 
@@ -348,6 +379,16 @@ def serve(arena: FempicArena, requests, budget_bytes: float, keep_results: bool 
     peak_resident, iters, part_steps = 0.0, 0, 0
     dropping = 0  # particles of retired requests still in the set until the next sort
     spent = dict(admit_s=0.0, step_s=0.0, retire_s=0.0)
+    prev = None  # (results still copying, their requests): resolved after the next step
+
+    def finish(pr):
+        pend, reqs = pr
+        for r, res in zip(reqs, pend.wait()):
+            r.t_done = clock() - t0
+            if keep_results:
+                r.result = res
+            done.append(r)
+
     while pending or active:
         now = clock() - t0
         ta = clock()
@@ -364,6 +405,9 @@ def serve(arena: FempicArena, requests, budget_bytes: float, keep_results: bool 
             pending.popleft()
         spent["admit_s"] += clock() - ta
         if not active:
+            if prev is not None:
+                finish(prev)
+                prev = None
             if pending:
                 time.sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.01)))
             continue
@@ -375,17 +419,21 @@ def serve(arena: FempicArena, requests, budget_bytes: float, keep_results: bool 
         iters += 1
         finished = [k for k, r in active.items() if r.left == 1]
         tr = clock()
-        for slot, res in zip(finished, arena.retire(finished)):
-            r = active.pop(slot)
-            r.t_done = clock() - t0
-            if keep_results:
-                r.result = res
-            dropping += res["n"]
+        if prev is not None:  # their copies ran beside this step; the next step's sort reuses the buffer
+            finish(prev)
+            prev = None
+        pend = arena.retire_async(finished)
+        reqs_f = [active.pop(slot) for slot in finished]
+        for r, n in zip(reqs_f, pend.counts):
+            r.t_retire = clock() - t0  # its device ranges and budget are free from here on
+            dropping += n
             resident -= r.tpl.device_bytes()
-            done.append(r)
+        prev = (pend, reqs_f) if reqs_f else None
         spent["retire_s"] += clock() - tr
         for r in active.values():
             r.left -= 1
+    if prev is not None:
+        finish(prev)
     wall = clock() - t0
     lat = np.array([r.t_done - r.arrival for r in done]) * 1e3
     return done, dict(wall_s=wall, iterations=iters, particle_steps=part_steps, value=part_steps / wall,

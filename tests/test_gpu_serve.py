@@ -82,8 +82,10 @@ def test_admission_waits_for_budget():
     arena = serve.FempicArena(max_cells=30000, max_nodes=8000, max_particles=200000, max_overlay=40000, max_insts=8)
     done, stats = serve.serve(arena, reqs, budget_bytes=2.5 * need, keep_results=True)
     assert stats["requests"] == 4 and stats["peak_resident_bytes"] <= 2.5 * need
-    # two fit at once: the third is admitted only after the first pair retires
+    # two fit at once: the third is admitted only after the first pair retires (frees its ranges and
+    # budget; its results reach the host while the next step runs)
     by_id = {r.rid: r for r in done}
-    assert by_id[2].t_admit >= min(by_id[0].t_done, by_id[1].t_done)
+    assert by_id[2].t_admit >= min(by_id[0].t_retire, by_id[1].t_retire)
+    assert all(r.t_done >= r.t_retire for r in done)
     for r in done:
         assert same(r.result, alone(r.tpl, r.steps)), f"request {r.rid}"
