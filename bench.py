@@ -708,12 +708,16 @@ def bench_serve(args):
         reqs.append(serve.Request(len(reqs), t, tpl, 10))
     budget = args.serve_budget_gb * 1e9
     arena = serve.FempicArena.for_budget(min(budget, args.serve_arena_gb * 1e9))
+    # untimed warm-up on the same arena (first launches, the pinned host blocks results come back in)
+    warm = [serve.Request(i, 0.0, small[i % len(small)], 10) for i in range(min(32, len(small)))]
+    serve.serve(arena, warm, budget)
     _, st = serve.serve(arena, reqs, budget, log=log)
     log(f"continuous batching: {st}")
     del arena
     import torch
     torch.cuda.empty_cache()
     reqs_plain = [serve.Request(r.rid, r.arrival, r.tpl, r.steps) for r in reqs]
+    serve.serve_plain([serve.Request(i, 0.0, small[i % len(small)], 10) for i in range(min(4, len(small)))])
     _, sp = serve.serve_plain(reqs_plain)
     log(f"plain per-request launches: {sp}")
     out = {

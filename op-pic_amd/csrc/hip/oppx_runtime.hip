@@ -68,13 +68,13 @@ struct Pattern {
 // element i, component d = pattern[d]; AoS when stride == 0, else SoA with the given stride
 template <typename T>
 __global__ void fill_pattern_kernel(T* __restrict__ dst, int64_t n, int dim, int64_t stride, Pattern pat) {
+    // one record per thread and step, its components in an inner loop: no 64-bit division per element
     const T* pv = reinterpret_cast<const T*>(pat.b);
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * dim; t += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (stride == 0) {
-            dst[t] = pv[t % dim];
+            for (int d = 0; d < dim; d++) dst[i * dim + d] = pv[d];  // AoS: record i is dim consecutive values
         } else {
-            const int64_t d = t / n, i = t % n;
-            dst[d * stride + i] = pv[d];
+            for (int d = 0; d < dim; d++) dst[d * stride + i] = pv[d];  // SoA: component d at d * stride + i
         }
     }
 }
@@ -82,11 +82,13 @@ __global__ void fill_pattern_kernel(T* __restrict__ dst, int64_t n, int dim, int
 // entries [first, first + count) of every `rec`-int record: v -> v + offset unless v < 0 or v == MAX
 __global__ void offset_indices_kernel(int32_t* __restrict__ base, int64_t n, int rec, int first, int count,
                                       int32_t offset) {
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * count; t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = t / count, k = t % count;
-        int32_t* p = base + i * rec + first + k;
-        const int32_t v = *p;
-        if (v >= 0 && v != kMaxCell) *p = v + offset;
+    // one record per thread and step (no 64-bit division per entry)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t* p = base + i * rec + first;
+        for (int k = 0; k < count; k++) {
+            const int32_t v = p[k];
+            if (v >= 0 && v != kMaxCell) p[k] = v + offset;
+        }
     }
 }
 
@@ -197,7 +199,7 @@ int oppx_fill_pattern(void* dst, int64_t n, int dim, int elem_bytes, const void*
     OPPX_ARG_CHECK(dst && pattern, "oppx_fill_pattern: null pointer");
     Pattern p{};
     memcpy(p.b, pattern, (size_t)dim * elem_bytes);
-    int64_t g = grid_for(n * dim, 256);
+    int64_t g = grid_for(n, 256);
     if (g > 8192) g = 8192;
     if (elem_bytes == 8)
         fill_pattern_kernel<double><<<(unsigned)g, 256, 0, as_stream(stream)>>>((double*)dst, n, dim, stride, p);
@@ -213,7 +215,7 @@ int oppx_offset_indices(int32_t* base, int64_t n_records, int rec_ints, int firs
                    "oppx_offset_indices: bad record layout");
     if (n_records == 0 || offset == 0) return OPPX_OK;
     OPPX_ARG_CHECK(base != nullptr, "oppx_offset_indices: null pointer");
-    int64_t g = grid_for(n_records * count, 256);
+    int64_t g = grid_for(n_records, 256);
     if (g > 8192) g = 8192;
     offset_indices_kernel<<<(unsigned)g, 256, 0, as_stream(stream)>>>(base, n_records, rec_ints, first, count, offset);
     OPPX_LAUNCH_CHECK("offset_indices");

@@ -159,6 +159,7 @@ class FempicArena:
         self.cells, self.nodes, self.ovl = RangeAlloc(C), RangeAlloc(N), RangeAlloc(max(1, max_overlay))
         self.slots_free = list(range(max_insts - 1, -1, -1))
         self.res = {}  # slot -> (template, cb, nb, ob)
+        self.dropping = 0  # particles of retired requests still in the set (marked removed) until the next sort
         # results of retired requests go to the host on their own stream, beside the next step
         self.copy_stream = torch.cuda.Stream(device=self.dev)
         self.cap, self.size = P, 0
@@ -268,6 +269,7 @@ class FempicArena:
                                                self.dep_scratch.data_ptr(), self.dep_bytes, s),
               "deposit_f32")
         self.size = int(self.counters[0].item())  # the live count after the finalize (host sync)
+        self.dropping = 0  # the finalize dropped the retired particles
 
     def retire(self, slots) -> list:
         """Hand back requests' states (each contiguous after the sort) and free their ranges."""
@@ -311,6 +313,7 @@ class FempicArena:
                 out["vel"].copy_(self.vel[:, lo:hi], non_blocking=True)
                 out["lc"].copy_(self.lc[:, lo:hi], non_blocking=True)
             staged.append((out, k, cb))
+            self.dropping += k
             if k:  # leaves the particle set at the next sort
                 check(L.oppx_fill_pattern(self.p2c.data_ptr() + 4 * lo, k, 1, 4, pat.ctypes.data, 0, s),
                       "fill_pattern")
@@ -377,7 +380,6 @@ def serve(arena: FempicArena, requests, budget_bytes: float, keep_results: bool 
     pending, active, done = deque(requests), {}, []
     resident = 0.0
     peak_resident, iters, part_steps = 0.0, 0, 0
-    dropping = 0  # particles of retired requests still in the set until the next sort
     spent = dict(admit_s=0.0, step_s=0.0, retire_s=0.0)
     prev = None  # (results still copying, their requests): resolved after the next step
 
@@ -411,11 +413,10 @@ def serve(arena: FempicArena, requests, budget_bytes: float, keep_results: bool 
             if pending:
                 time.sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.01)))
             continue
-        part_steps += arena.size - dropping
+        part_steps += arena.size - arena.dropping  # retired particles still in the set are not moved
         ts = clock()
         arena.step()
         spent["step_s"] += clock() - ts
-        dropping = 0
         iters += 1
         finished = [k for k, r in active.items() if r.left == 1]
         tr = clock()
@@ -424,9 +425,8 @@ def serve(arena: FempicArena, requests, budget_bytes: float, keep_results: bool 
             prev = None
         pend = arena.retire_async(finished)
         reqs_f = [active.pop(slot) for slot in finished]
-        for r, n in zip(reqs_f, pend.counts):
+        for r in reqs_f:
             r.t_retire = clock() - t0  # its device ranges and budget are free from here on
-            dropping += n
             resident -= r.tpl.device_bytes()
         prev = (pend, reqs_f) if reqs_f else None
         spent["retire_s"] += clock() - tr
