@@ -7,7 +7,11 @@
 // by cell (oppx_sort_by_cell), so:
 //   pass 1: the heads of the runs of equal cells are compacted (block counts, scan, scatter), then
 //           one lane per run sums lc[0..3] of its run in index order (FP32 Kahan) -- every lane of a
-//           wave owns a run instead of one lane in ~(run length) holding it;
+//           wave owns a run instead of one lane in ~(run length) holding it. A run longer than
+//           kLongRun particles is summed by a whole wave instead: lane l takes particles
+//           head + l, head + l + 64, ... (Kahan), and the 64 lane sums are added in lane order, so a
+//           long run no longer holds its wave for its whole length. Either way the order depends only
+//           on the run itself (not on where it sits in the particle set);
 //   pass 2: one lane per node sums its (cell, vertex) partials in the fixed node->cell order
 //           (FP32 Kahan) and stores the result.
 // The order of every addition is fixed by the data, so the result does not depend on scheduling
@@ -42,9 +46,26 @@ __device__ __forceinline__ int64_t live_of(const oppx_move_counters* ctr, int64_
     return n < 0 ? 0 : (n > n_max ? n_max : n);
 }
 
-__device__ __forceinline__ bool run_head(const int32_t* __restrict__ p2c, int64_t j, int64_t n_cells) {
-    const int32_t c = p2c[j];
-    return (unsigned)c < (unsigned)n_cells && (j == 0 || p2c[j - 1] != c);
+// A block's p2c tile (its kRunElems entries and the one before) staged in LDS with coalesced loads,
+// one padding word per 16 so that thread t's 16 consecutive entries sit in distinct banks.
+struct RunTile {
+    int32_t v[kRunElems + 1 + (kRunElems + 1) / 16 + 1];
+};
+
+__device__ __forceinline__ int tile_at(int e) { return e + (e >> 4); }
+
+__device__ __forceinline__ void load_run_tile(RunTile& T, const int32_t* __restrict__ p2c, int64_t base0, int64_t n) {
+    for (int t = threadIdx.x; t < kRunElems + 1; t += kB) {
+        const int64_t j = base0 - 1 + t;
+        T.v[tile_at(t)] = (j >= 0 && j < n) ? p2c[j] : -1;  // -1: never a valid cell
+    }
+    __syncthreads();
+}
+
+// element base0 + e of the tile (e in [0, kRunElems)) starts a run of a valid cell
+__device__ __forceinline__ bool run_head(const RunTile& T, int e, int64_t n_cells) {
+    const int32_t c = T.v[tile_at(e + 1)];
+    return (unsigned)c < (unsigned)n_cells && T.v[tile_at(e)] != c;
 }
 
 // exclusive block scan of one uint32 per thread (kB threads); total in *tot
@@ -71,18 +92,23 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* tot) {
 
 __global__ void __launch_bounds__(kB) run_count(const int32_t* __restrict__ p2c, const oppx_move_counters* ctr,
                                                 int64_t n_max, int64_t n_cells, uint32_t* __restrict__ bsum) {
+    __shared__ RunTile T;
     const int64_t n = live_of(ctr, n_max);
-    const int64_t base = (int64_t)blockIdx.x * kRunElems + (int64_t)threadIdx.x * kRunItems;
+    const int64_t base0 = (int64_t)blockIdx.x * kRunElems;
+    load_run_tile(T, p2c, base0, n);
+    const int e0 = (int)threadIdx.x * kRunItems;
     uint32_t c = 0;
     for (int r = 0; r < kRunItems; r++)
-        if (base + r < n && run_head(p2c, base + r, n_cells)) c++;
+        if (base0 + e0 + r < n && run_head(T, e0 + r, n_cells)) c++;
     uint32_t tot;
     block_scan(c, &tot);
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// single block: exclusive scan of nb block counts in place; the total goes to *n_runs
+// single block: exclusive scan of nb block counts in place; the total goes to n_runs[0], and the
+// long-run count n_runs[1] starts at zero
 __global__ void __launch_bounds__(kB) run_scan(uint32_t* __restrict__ bsum, int64_t nb, uint32_t* __restrict__ n_runs) {
+    if (threadIdx.x == 0) n_runs[1] = 0u;
     uint32_t carry = 0;
     for (int64_t b0 = 0; b0 < nb; b0 += kB) {
         const int64_t j = b0 + threadIdx.x;
@@ -98,28 +124,49 @@ __global__ void __launch_bounds__(kB) run_scan(uint32_t* __restrict__ bsum, int6
 __global__ void __launch_bounds__(kB) run_compact(const int32_t* __restrict__ p2c, const oppx_move_counters* ctr,
                                                   int64_t n_max, int64_t n_cells, const uint32_t* __restrict__ bsum,
                                                   int32_t* __restrict__ heads) {
+    __shared__ RunTile T;
     const int64_t n = live_of(ctr, n_max);
-    const int64_t base = (int64_t)blockIdx.x * kRunElems + (int64_t)threadIdx.x * kRunItems;
+    const int64_t base0 = (int64_t)blockIdx.x * kRunElems;
+    load_run_tile(T, p2c, base0, n);
+    const int e0 = (int)threadIdx.x * kRunItems;
     uint32_t c = 0;
     for (int r = 0; r < kRunItems; r++)
-        if (base + r < n && run_head(p2c, base + r, n_cells)) c++;
+        if (base0 + e0 + r < n && run_head(T, e0 + r, n_cells)) c++;
     uint32_t tot;
     uint32_t pos = bsum[blockIdx.x] + block_scan(c, &tot);
     for (int r = 0; r < kRunItems; r++)
-        if (base + r < n && run_head(p2c, base + r, n_cells)) heads[pos++] = (int32_t)(base + r);
+        if (base0 + e0 + r < n && run_head(T, e0 + r, n_cells)) heads[pos++] = (int32_t)(base0 + e0 + r);
+}
+
+constexpr int64_t kLongRun = 64;  // runs longer than this are summed by a whole wave
+
+// End of run r (one past its last particle): the next head (valid cells sort before every invalid or
+// removed entry, so runs are back to back), or for the last run the first index holding another cell.
+__device__ __forceinline__ int64_t run_end(const int32_t* __restrict__ p2c, const int32_t* __restrict__ heads,
+                                           int64_t r, int64_t n_runs, int64_t n, int32_t c) {
+    if (r + 1 < n_runs) return heads[r + 1];
+    int64_t e = heads[r];
+    while (e < n && p2c[e] == c) e++;
+    return e;
 }
 
 __global__ void __launch_bounds__(kB) cell_runs(const int32_t* __restrict__ p2c, const double* __restrict__ lc,
                                                 int64_t stride, const oppx_move_counters* ctr, int64_t n_max,
-                                                const int32_t* __restrict__ heads, const uint32_t* __restrict__ n_runs,
-                                                float* __restrict__ part) {
+                                                const int32_t* __restrict__ heads, uint32_t* __restrict__ n_runs,
+                                                int32_t* __restrict__ long_runs, float* __restrict__ part) {
     const int64_t r = (int64_t)blockIdx.x * kB + threadIdx.x;
-    if (r >= (int64_t)*n_runs) return;
+    const int64_t nr = (int64_t)n_runs[0];
+    if (r >= nr) return;
     const int64_t n = live_of(ctr, n_max);
     const int64_t j = heads[r];
     const int32_t c = p2c[j];
+    const int64_t e = run_end(p2c, heads, r, nr, n, c);
+    if (e - j > kLongRun) {  // summed by cell_runs_long
+        long_runs[atomicAdd(n_runs + 1, 1u)] = (int32_t)r;
+        return;
+    }
     Kahan k0, k1, k2, k3;
-    for (int64_t q = j; q < n && p2c[q] == c; q++) {
+    for (int64_t q = j; q < e; q++) {
         k0.add((float)lc[q]);
         k1.add((float)lc[stride + q]);
         k2.add((float)lc[2 * stride + q]);
@@ -130,10 +177,50 @@ __global__ void __launch_bounds__(kB) cell_runs(const int32_t* __restrict__ p2c,
     *reinterpret_cast<float4*>(part + 4 * (int64_t)c) = v;
 }
 
+// One wave per long run (waves stride over the list): lane l sums particles head + l + 64 t in index
+// order (Kahan), then lane 0 adds the 64 lane sums in lane order (Kahan).
+__global__ void __launch_bounds__(kB) cell_runs_long(const int32_t* __restrict__ p2c, const double* __restrict__ lc,
+                                                     int64_t stride, const oppx_move_counters* ctr, int64_t n_max,
+                                                     const int32_t* __restrict__ heads,
+                                                     const uint32_t* __restrict__ n_runs,
+                                                     const int32_t* __restrict__ long_runs, float* __restrict__ part) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wave = ((int64_t)blockIdx.x * kB + threadIdx.x) / kWave;
+    const int64_t n_waves = (int64_t)gridDim.x * (kB / kWave);
+    const int64_t nr = (int64_t)n_runs[0], nl = (int64_t)n_runs[1];
+    const int64_t n = live_of(ctr, n_max);
+    for (int64_t k = wave; k < nl; k += n_waves) {
+        const int64_t r = long_runs[k];
+        const int64_t j = heads[r];
+        const int32_t c = p2c[j];
+        const int64_t e = run_end(p2c, heads, r, nr, n, c);
+        Kahan k0, k1, k2, k3;
+        for (int64_t q = j + lane; q < e; q += kWave) {
+            k0.add((float)lc[q]);
+            k1.add((float)lc[stride + q]);
+            k2.add((float)lc[2 * stride + q]);
+            k3.add((float)lc[3 * stride + q]);
+        }
+        Kahan t0, t1, t2, t3;
+        for (int l = 0; l < kWave; l++) {  // the same on every lane: lane l's sums, in lane order
+            t0.add(__shfl(k0.s, l));
+            t1.add(__shfl(k1.s, l));
+            t2.add(__shfl(k2.s, l));
+            t3.add(__shfl(k3.s, l));
+        }
+        if (lane == 0) {
+            float4 v;
+            v.x = t0.s; v.y = t1.s; v.z = t2.s; v.w = t3.s;
+            *reinterpret_cast<float4*>(part + 4 * (int64_t)c) = v;
+        }
+    }
+}
+
 struct RunScratch {
     uint32_t* bsum;
-    uint32_t* n_runs;
+    uint32_t* n_runs;  // [0] runs, [1] long runs
     int32_t* heads;
+    int32_t* long_runs;
     int64_t nb;
 };
 
@@ -141,7 +228,7 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 size_t run_scratch_need(int64_t n_max) {
     const int64_t nb = n_max > 0 ? (n_max + kRunElems - 1) / kRunElems : 1;
-    return align256((size_t)nb * 4) + 256 + align256((size_t)(n_max > 0 ? n_max : 1) * 4);
+    return align256((size_t)nb * 4) + 256 + 2 * align256((size_t)(n_max > 0 ? n_max : 1) * 4);
 }
 
 RunScratch run_carve(void* base, int64_t n_max) {
@@ -153,6 +240,8 @@ RunScratch run_carve(void* base, int64_t n_max) {
     r.n_runs = reinterpret_cast<uint32_t*>(p);
     p += 256;
     r.heads = reinterpret_cast<int32_t*>(p);
+    p += align256((size_t)(n_max > 0 ? n_max : 1) * 4);
+    r.long_runs = reinterpret_cast<int32_t*>(p);
     return r;
 }
 
@@ -165,7 +254,16 @@ __global__ void __launch_bounds__(kB) node_gather(const int32_t* __restrict__ no
     if (v >= n_nodes) return;
     Kahan k;
     const int32_t e1 = node_end ? node_end[v] : node_start[v + 1];
-    for (int32_t e = node_start[v]; e < e1; e++) k.add(part[node_ref[e]]);
+    int32_t e = node_start[v];
+    for (; e + 4 <= e1; e += 4) {  // four loads in flight, then the four adds in order
+        const int32_t r0 = node_ref[e], r1 = node_ref[e + 1], r2 = node_ref[e + 2], r3 = node_ref[e + 3];
+        const float p0 = part[r0], p1 = part[r1], p2 = part[r2], p3 = part[r3];
+        k.add(p0);
+        k.add(p1);
+        k.add(p2);
+        k.add(p3);
+    }
+    for (; e < e1; e++) k.add(part[node_ref[e]]);
     if (charge) charge[v] = (double)k.s;
     if (charge_f32) charge_f32[v] = k.s;
 }
@@ -202,8 +300,13 @@ int oppx_fempic_deposit_sorted_f32(int64_t n_max, int64_t stride, oppx_move_coun
         run_scan<<<1, kB, 0, s>>>(rs.bsum, rs.nb, rs.n_runs);
         run_compact<<<(unsigned)rs.nb, kB, 0, s>>>(p2c, counters, n_max, n_cells, rs.bsum, rs.heads);
         cell_runs<<<grid_for(n_max, kB), kB, 0, s>>>(p2c, lc, stride, counters, n_max, rs.heads, rs.n_runs,
-                                                     cell_partials);
+                                                     rs.long_runs, cell_partials);
         OPPX_LAUNCH_CHECK("cell_runs");
+        int64_t gl = grid_for(n_max, kLongRun * (kB / kWave));  // at most one long run per kLongRun particles
+        if (gl > 2048) gl = 2048;
+        cell_runs_long<<<(unsigned)gl, kB, 0, s>>>(p2c, lc, stride, counters, n_max, rs.heads, rs.n_runs,
+                                                    rs.long_runs, cell_partials);
+        OPPX_LAUNCH_CHECK("cell_runs_long");
     }
     node_gather<<<grid_for(n_nodes, kB), kB, 0, s>>>(node_start, node_end, node_ref, n_nodes, cell_partials,
                                                      node_charge, node_charge_f32);
