@@ -92,56 +92,70 @@ def cpu_baseline(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, budget_s):
                        f"{t_tot:.1f}s")
 
 
+def run_reference_build(name, arrays, params, prepare=None, env=None):
+    """Runs oracle/_ref/<name> (built from the reference's own sources where they were available) on
+    `arrays` ((array, file stem, dtype) written as <in>/<stem>.bin) and the `.param` lines
+    params(tmpdir); prepare(tmpdir) may write further inputs. Returns (seconds the driver timed around
+    the step loops, rows of its steps.txt), or None when the build is absent or fails."""
+    import subprocess
+    import tempfile
+
+    binary = os.path.join(ROOT, "oracle", "_ref", name)
+    if not os.path.exists(binary):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        ind, outd = os.path.join(td, "in"), os.path.join(td, "out")
+        os.makedirs(ind)
+        os.makedirs(outd)
+        for a, stem, dt_ in arrays:
+            np.ascontiguousarray(a, dtype=dt_).tofile(os.path.join(ind, stem + ".bin"))
+        if prepare is not None:
+            prepare(td)
+        par = os.path.join(td, "case.param")
+        with open(par, "w") as f:
+            f.write("\n".join(params(td)) + "\n")
+        try:
+            r = subprocess.run([binary, par, ind, outd], env=env, capture_output=True, text=True, timeout=240)
+        except subprocess.TimeoutExpired:
+            return None
+        if r.returncode != 0:
+            log(f"reference CPU baseline {name} failed (rc {r.returncode}): {r.stderr[-300:]}")
+            return None
+        loop_s = float(open(os.path.join(outd, "loop_seconds.txt")).read())
+        st = np.loadtxt(os.path.join(outd, "steps.txt"), ndmin=2)
+    return loop_s, st
+
+
 def cpu_baseline_reference(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, steps=10, stride=10):
     """The reference's own OpenMP build (oracle/_ref/fempic_ref_omp, compiled from the reference
     sources where they were available) on the host's cores: the full mesh, one particle in `stride`
     of the same input, `steps` steps of push + move (direct hop, overlay read from a file in the
     reference's format, as opp_dh_data_generate = false does) + deposit + hole fill; only the
     steps' loops are timed. None when the build is absent."""
-    import subprocess
-    import tempfile
-
     import oppx
 
-    binary = os.path.join(ROOT, "oracle", "_ref", "fempic_ref_omp")
-    if not os.path.exists(binary):
-        return None
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     sl = slice(0, None, stride)
-    with tempfile.TemporaryDirectory() as td:
-        ind, outd = os.path.join(td, "in"), os.path.join(td, "out")
-        os.makedirs(ind)
-        os.makedirs(outd)
-        for k in ("node_pos", "c2n", "c2c", "c_det", "c_vol"):
-            np.ascontiguousarray(m[k]).tofile(os.path.join(ind, k + ".bin"))
-        np.ascontiguousarray(ef).tofile(os.path.join(ind, "c_ef.bin"))
-        np.ascontiguousarray(pos[sl]).tofile(os.path.join(ind, "p_pos.bin"))
-        np.ascontiguousarray(vel[sl]).tofile(os.path.join(ind, "p_vel.bin"))
-        np.ascontiguousarray(p2c[sl], np.int32).tofile(os.path.join(ind, "p2c.bin"))
-        oppx.dh_overlay_save(os.path.join(td, oppx.dh_overlay_filename(m["n_cells"], gs)), ov)
-        n = int(p2c[sl].shape[0])
-        lines = [f"INT nnodes = {m['n_nodes']}", f"INT ncells = {m['n_cells']}", f"INT nparts = {n}",
-                 f"INT num_steps = {steps}", f"REAL dt = {dt!r}", f"REAL mass = {ms!r}", f"REAL charge = {q!r}",
-                 f"REAL grid_spacing = {gs!r}", f"REAL bminx = {float(bmin[0])!r}", f"REAL bminy = {float(bmin[1])!r}",
-                 f"REAL bminz = {float(bmin[2])!r}", f"REAL bmaxx = {float(bmax[0])!r}",
-                 f"REAL bmaxy = {float(bmax[1])!r}", f"REAL bmaxz = {float(bmax[2])!r}",
-                 "BOOL opp_global_move = true", "BOOL opp_dh_data_generate = false",
-                 f"STRING opp_dh_mesh_folder = {td}/", "REAL opp_allocation_multiple = 1.1",
-                 "BOOL opp_profile_all = false", "STRING opp_fill = HoleFill_All", "INT opp_fill_period = 4"]
-        par = os.path.join(td, "case.param")
-        with open(par, "w") as f:
-            f.write("\n".join(lines) + "\n")
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
-        try:
-            r = subprocess.run([binary, par, ind, outd], env=env, capture_output=True, text=True, timeout=240)
-        except subprocess.TimeoutExpired:
-            return None
-        if r.returncode != 0:
-            log(f"reference CPU baseline failed (rc {r.returncode}): {r.stderr[-300:]}")
-            return None
-        loop_s = float(open(os.path.join(outd, "loop_seconds.txt")).read())
-        st = np.loadtxt(os.path.join(outd, "steps.txt"), ndmin=2)
-        moved = float(st[:, 1].sum())
+    n = int(p2c[sl].shape[0])
+    arrays = [(m[k], k, None) for k in ("node_pos", "c2n", "c2c", "c_det", "c_vol")] + \
+             [(ef, "c_ef", None), (pos[sl], "p_pos", np.float64), (vel[sl], "p_vel", np.float64),
+              (p2c[sl], "p2c", np.int32)]
+    params = lambda td: [
+        f"INT nnodes = {m['n_nodes']}", f"INT ncells = {m['n_cells']}", f"INT nparts = {n}",
+        f"INT num_steps = {steps}", f"REAL dt = {dt!r}", f"REAL mass = {ms!r}", f"REAL charge = {q!r}",
+        f"REAL grid_spacing = {gs!r}", f"REAL bminx = {float(bmin[0])!r}", f"REAL bminy = {float(bmin[1])!r}",
+        f"REAL bminz = {float(bmin[2])!r}", f"REAL bmaxx = {float(bmax[0])!r}", f"REAL bmaxy = {float(bmax[1])!r}",
+        f"REAL bmaxz = {float(bmax[2])!r}", "BOOL opp_global_move = true", "BOOL opp_dh_data_generate = false",
+        f"STRING opp_dh_mesh_folder = {td}/", "REAL opp_allocation_multiple = 1.1", "BOOL opp_profile_all = false",
+        "STRING opp_fill = HoleFill_All", "INT opp_fill_period = 4"]
+    # the overlay as the reference's own file, so its O(boxes x tets) build is skipped
+    prepare = lambda td: oppx.dh_overlay_save(os.path.join(td, oppx.dh_overlay_filename(m["n_cells"], gs)), ov)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+    out = run_reference_build("fempic_ref_omp", arrays, params, prepare, env)
+    if out is None:
+        return None
+    loop_s, st = out
+    moved = float(st[:, 1].sum())
     return dict(value=moved / loop_s, unit="particles/s", cores=threads, kind="reference", cpu=cpu_model(),
                 sample=f"the reference's OpenMP build ({threads} threads, OMP_PROC_BIND=close) on one particle in {stride} "
                        f"of the same input ({n} particles, the full {m['n_cells']}-tet mesh, overlay read "
@@ -465,34 +479,17 @@ def neso_cpu_reference(m, pos, vel, p2c, nx, cw, dt, steps=4):
     compiled from the reference sources where they were available) on the given particles over the
     full mesh, `steps` steps of update_pos + move; only those loops are timed. None when the build
     is absent."""
-    import subprocess
-    import tempfile
-
-    binary = os.path.join(ROOT, "oracle", "_ref", "neso_ref_seq")
-    if not os.path.exists(binary):
-        return None
     n = int(len(p2c))
-    with tempfile.TemporaryDirectory() as td:
-        ind, outd = os.path.join(td, "in"), os.path.join(td, "out")
-        os.makedirs(ind)
-        os.makedirs(outd)
-        for a, name, dt_ in ((m["c_pos_ll"], "c_pos_ll", np.float64), (m["c2c"], "c2c", np.int32),
-                             (pos, "p_pos", np.float64), (vel, "p_vel", np.float64), (p2c, "p2c", np.int32)):
-            np.ascontiguousarray(a, dtype=dt_).tofile(os.path.join(ind, name + ".bin"))
-        par = os.path.join(td, "case.param")
-        with open(par, "w") as f:
-            f.write("\n".join([f"INT nx = {nx}", f"INT ny = {nx}", f"INT nparts = {n}", f"INT num_steps = {steps}",
-                               f"REAL dt = {dt!r}", f"REAL cell_width = {cw!r}", "REAL opp_allocation_multiple = 1.1",
-                               "BOOL opp_global_move = false", "STRING opp_fill = HoleFill_All",
-                               "BOOL opp_profile_all = false"]) + "\n")
-        try:
-            r = subprocess.run([binary, par, ind, outd], capture_output=True, text=True, timeout=240)
-        except subprocess.TimeoutExpired:
-            return None
-        if r.returncode != 0:
-            log(f"reference CPU baseline failed (rc {r.returncode}): {r.stderr[-300:]}")
-            return None
-        loop_s = float(open(os.path.join(outd, "loop_seconds.txt")).read())
+    arrays = [(m["c_pos_ll"], "c_pos_ll", np.float64), (m["c2c"], "c2c", np.int32), (pos, "p_pos", np.float64),
+              (vel, "p_vel", np.float64), (p2c, "p2c", np.int32)]
+    params = lambda td: [
+        f"INT nx = {nx}", f"INT ny = {nx}", f"INT nparts = {n}", f"INT num_steps = {steps}", f"REAL dt = {dt!r}",
+        f"REAL cell_width = {cw!r}", "REAL opp_allocation_multiple = 1.1", "BOOL opp_global_move = false",
+        "STRING opp_fill = HoleFill_All", "BOOL opp_profile_all = false"]
+    out = run_reference_build("neso_ref_seq", arrays, params)
+    if out is None:
+        return None
+    loop_s = out[0]
     return {"value": n * steps / loop_s, "unit": "particles/s", "cores": 1, "kind": "reference", "cpu": cpu_model(),
             "sample": f"the reference's seq build, {steps} steps of the first {n} particles over the full mesh, "
                       f"{loop_s:.2f}s of loops"}
@@ -650,36 +647,19 @@ def cabana_cpu_reference(c2c6, interp, pos, vel, weight, p2c, deck, steps=4):
     compiled from the reference sources where they were available) on the given particles over the
     full mesh, `steps` steps of accumulator reset + move-deposit; only those loops are timed. None
     when the build is absent."""
-    import subprocess
-    import tempfile
-
-    binary = os.path.join(ROOT, "oracle", "_ref", "cabana_ref_seq")
-    if not os.path.exists(binary):
-        return None
     n, nc = int(len(p2c)), int(len(c2c6))
-    with tempfile.TemporaryDirectory() as td:
-        ind, outd = os.path.join(td, "in"), os.path.join(td, "out")
-        os.makedirs(ind)
-        os.makedirs(outd)
-        for a, name, dt_ in ((c2c6, "c2ngc", np.int32), (interp, "c_interp", np.float64), (pos, "p_pos", np.float64),
-                             (vel, "p_vel", np.float64), (weight, "p_weight", np.float64), (p2c, "p2c", np.int32)):
-            np.ascontiguousarray(a, dtype=dt_).tofile(os.path.join(ind, name + ".bin"))
-        lines = [f"INT ncells = {nc}", f"INT nparts = {n}", f"INT num_steps = {steps}", f"REAL qsp = {deck['qsp']!r}",
-                 f"REAL qdt_half_mc = {deck['qdt_2mc']!r}", f"REAL cdt_dx = {deck['cdt_d'][0]!r}",
-                 f"REAL cdt_dy = {deck['cdt_d'][1]!r}", f"REAL cdt_dz = {deck['cdt_d'][2]!r}",
-                 "REAL opp_allocation_multiple = 1.1", "BOOL opp_profile_all = false",
-                 "STRING opp_fill = HoleFill_All", "INT opp_fill_period = 4", "BOOL opp_global_move = false"]
-        par = os.path.join(td, "case.param")
-        with open(par, "w") as f:
-            f.write("\n".join(lines) + "\n")
-        try:
-            r = subprocess.run([binary, par, ind, outd], capture_output=True, text=True, timeout=240)
-        except subprocess.TimeoutExpired:
-            return None
-        if r.returncode != 0:
-            log(f"reference CPU baseline failed (rc {r.returncode}): {r.stderr[-300:]}")
-            return None
-        loop_s = float(open(os.path.join(outd, "loop_seconds.txt")).read())
+    arrays = [(c2c6, "c2ngc", np.int32), (interp, "c_interp", np.float64), (pos, "p_pos", np.float64),
+              (vel, "p_vel", np.float64), (weight, "p_weight", np.float64), (p2c, "p2c", np.int32)]
+    params = lambda td: [
+        f"INT ncells = {nc}", f"INT nparts = {n}", f"INT num_steps = {steps}", f"REAL qsp = {deck['qsp']!r}",
+        f"REAL qdt_half_mc = {deck['qdt_2mc']!r}", f"REAL cdt_dx = {deck['cdt_d'][0]!r}",
+        f"REAL cdt_dy = {deck['cdt_d'][1]!r}", f"REAL cdt_dz = {deck['cdt_d'][2]!r}",
+        "REAL opp_allocation_multiple = 1.1", "BOOL opp_profile_all = false", "STRING opp_fill = HoleFill_All",
+        "INT opp_fill_period = 4", "BOOL opp_global_move = false"]
+    out = run_reference_build("cabana_ref_seq", arrays, params)
+    if out is None:
+        return None
+    loop_s = out[0]
     return {"value": n * steps / loop_s, "unit": "particles/s", "cores": 1, "kind": "reference", "cpu": cpu_model(),
             "sample": f"the reference's seq build, {steps} steps of the first {n} particles over the full mesh, "
                       f"{loop_s:.2f}s of loops"}
