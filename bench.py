@@ -126,6 +126,17 @@ def run_reference_build(name, arrays, params, prepare=None, env=None):
     return loop_s, st
 
 
+def reference_build_variant(app):
+    """(binary name, environment, threads, description) of the reference build to time for `app`
+    ("neso" or "cabana"): the OpenMP build on the host's cores when it was built, else the seq
+    build on one thread."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    if os.path.exists(os.path.join(ROOT, "oracle", "_ref", f"{app}_ref_omp")):
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+        return f"{app}_ref_omp", env, threads, f"the reference's OpenMP build ({threads} threads, OMP_PROC_BIND=close)"
+    return f"{app}_ref_seq", None, 1, "the reference's seq build"
+
+
 def cpu_baseline_reference(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, steps=10, stride=10):
     """The reference's own OpenMP build (oracle/_ref/fempic_ref_omp, compiled from the reference
     sources where they were available) on the host's cores: the full mesh, one particle in `stride`
@@ -466,19 +477,19 @@ def bench_neso(args):
                      "traffic": load_traffic("c3", "neso_step_kernel"), "measured_copy_gbs": copy_gbs},
     }
     if not args.no_cpu:
-        k = min(n, 4_000_000)
+        k = min(n, 16_000_000)
         ref = neso_cpu_reference(m, pos[:k], vel[:k], p2c[:k], nx, cw, dt)
         if ref is not None:
-            log(f"cpu reference {ref['value']:.3e} particles/s (1 thread)")
+            log(f"cpu reference {ref['value']:.3e} particles/s ({ref['cores']} threads)")
             out["cpu_baseline"] = ref
     return out
 
 
-def neso_cpu_reference(m, pos, vel, p2c, nx, cw, dt, steps=4):
-    """The reference's own seq build of the NESO-advection mdir loops (oracle/_ref/neso_ref_seq,
-    compiled from the reference sources where they were available) on the given particles over the
-    full mesh, `steps` steps of update_pos + move; only those loops are timed. None when the build
-    is absent."""
+def neso_cpu_reference(m, pos, vel, p2c, nx, cw, dt, steps=10):
+    """The reference's own build of the NESO-advection mdir loops (oracle/_ref/neso_ref_omp on the
+    host's cores, else neso_ref_seq; compiled from the reference sources where they were available)
+    on the given particles over the full mesh, `steps` steps of update_pos + move; only those loops
+    are timed. None when no build is present."""
     n = int(len(p2c))
     arrays = [(m["c_pos_ll"], "c_pos_ll", np.float64), (m["c2c"], "c2c", np.int32), (pos, "p_pos", np.float64),
               (vel, "p_vel", np.float64), (p2c, "p2c", np.int32)]
@@ -486,13 +497,14 @@ def neso_cpu_reference(m, pos, vel, p2c, nx, cw, dt, steps=4):
         f"INT nx = {nx}", f"INT ny = {nx}", f"INT nparts = {n}", f"INT num_steps = {steps}", f"REAL dt = {dt!r}",
         f"REAL cell_width = {cw!r}", "REAL opp_allocation_multiple = 1.1", "BOOL opp_global_move = false",
         "STRING opp_fill = HoleFill_All", "BOOL opp_profile_all = false"]
-    out = run_reference_build("neso_ref_seq", arrays, params)
+    name, env, threads, what = reference_build_variant("neso")
+    out = run_reference_build(name, arrays, params, env=env)
     if out is None:
         return None
     loop_s = out[0]
-    return {"value": n * steps / loop_s, "unit": "particles/s", "cores": 1, "kind": "reference", "cpu": cpu_model(),
-            "sample": f"the reference's seq build, {steps} steps of the first {n} particles over the full mesh, "
-                      f"{loop_s:.2f}s of loops"}
+    return {"value": n * steps / loop_s, "unit": "particles/s", "cores": threads, "kind": "reference",
+            "cpu": cpu_model(),
+            "sample": f"{what}, {steps} steps of the first {n} particles over the full mesh, {loop_s:.2f}s of loops"}
 
 
 def cabana_deck(nx=40, ny=40, nz=60, nppc=750, v0=0.0866025403784439, c=1.0, qsp=-1.0, me=1.0, grid_len=1.0):
@@ -632,9 +644,10 @@ def bench_cabana(args):
         port = {"value": m / dt_cpu, "unit": "particles/s", "cores": 1, "kind": "port", "cpu": cpu_model(),
                 "sample": f"the oracle, one step of the first {m} particles"}
         log(f"cpu oracle {m / dt_cpu:.3e} particles/s (1 thread)")
-        ref = cabana_cpu_reference(c2c6, interp, pos[sl], vel[sl], weight[sl], p2c[sl], deck)
+        rl = slice(0, min(n, 16_000_000))
+        ref = cabana_cpu_reference(c2c6, interp, pos[rl], vel[rl], weight[rl], p2c[rl], deck)
         if ref is not None:
-            log(f"cpu reference {ref['value']:.3e} particles/s (1 thread)")
+            log(f"cpu reference {ref['value']:.3e} particles/s ({ref['cores']} threads)")
             out["cpu_baseline"] = ref
             out["cpu_port_baseline"] = port
         else:
@@ -642,11 +655,11 @@ def bench_cabana(args):
     return out
 
 
-def cabana_cpu_reference(c2c6, interp, pos, vel, weight, p2c, deck, steps=4):
-    """The reference's own seq build of the CabanaPIC move-deposit loop (oracle/_ref/cabana_ref_seq,
-    compiled from the reference sources where they were available) on the given particles over the
-    full mesh, `steps` steps of accumulator reset + move-deposit; only those loops are timed. None
-    when the build is absent."""
+def cabana_cpu_reference(c2c6, interp, pos, vel, weight, p2c, deck, steps=10):
+    """The reference's own build of the CabanaPIC move-deposit loop (oracle/_ref/cabana_ref_omp on
+    the host's cores, else cabana_ref_seq; compiled from the reference sources where they were
+    available) on the given particles over the full mesh, `steps` steps of accumulator reset +
+    move-deposit; only those loops are timed. None when no build is present."""
     n, nc = int(len(p2c)), int(len(c2c6))
     arrays = [(c2c6, "c2ngc", np.int32), (interp, "c_interp", np.float64), (pos, "p_pos", np.float64),
               (vel, "p_vel", np.float64), (weight, "p_weight", np.float64), (p2c, "p2c", np.int32)]
@@ -656,13 +669,14 @@ def cabana_cpu_reference(c2c6, interp, pos, vel, weight, p2c, deck, steps=4):
         f"REAL cdt_dy = {deck['cdt_d'][1]!r}", f"REAL cdt_dz = {deck['cdt_d'][2]!r}",
         "REAL opp_allocation_multiple = 1.1", "BOOL opp_profile_all = false", "STRING opp_fill = HoleFill_All",
         "INT opp_fill_period = 4", "BOOL opp_global_move = false"]
-    out = run_reference_build("cabana_ref_seq", arrays, params)
+    name, env, threads, what = reference_build_variant("cabana")
+    out = run_reference_build(name, arrays, params, env=env)
     if out is None:
         return None
     loop_s = out[0]
-    return {"value": n * steps / loop_s, "unit": "particles/s", "cores": 1, "kind": "reference", "cpu": cpu_model(),
-            "sample": f"the reference's seq build, {steps} steps of the first {n} particles over the full mesh, "
-                      f"{loop_s:.2f}s of loops"}
+    return {"value": n * steps / loop_s, "unit": "particles/s", "cores": threads, "kind": "reference",
+            "cpu": cpu_model(),
+            "sample": f"{what}, {steps} steps of the first {n} particles over the full mesh, {loop_s:.2f}s of loops"}
 
 
 def bench_serve(args):

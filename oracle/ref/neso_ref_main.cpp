@@ -18,9 +18,15 @@
 
 static std::vector<int> oppx_ref_hops;
 
+#if defined(OPPX_REF_OMP)
+#include "opp_omp.h"
+#define opp_check_part_move_status(f, i1, m, n, t) (oppx_ref_hops[(n)]++, opp_check_part_move_status(f, i1, m, n, t))
+#include OPPX_REF_KERNELS_OMP
+#else
 #include "opp_seq.h"
 #define opp_check_part_move_status(m, n, rc) (oppx_ref_hops[(n)]++, opp_check_part_move_status(m, n, rc))
 #include OPPX_REF_KERNELS_SEQ
+#endif
 
 namespace {
 
