@@ -11,7 +11,7 @@ for sub in ("op-pic_amd", "oracle"):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (run on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU")
     config.addinivalue_line("markers", "slow: takes more than a few seconds on the CPU")
 
 
