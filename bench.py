@@ -346,7 +346,12 @@ def bench_ensemble(args):
         "unit": "particles/s",
         "n_gpus": 1,
         "steps": S,
+        # untimed before the graph replay: the autotuner's trials and the same S steps run eagerly
+        "warmup": S,
+        "ms_per_step": 1e3 * t_graph / S,
         "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
         "dtype": "f64 move, f32 compensated deposit",
         "data": "synthetic",
         "config": {"workload": f"{len(inst)} box instances, {n_tets} tets, {n_part} particles (seed 4)",
@@ -704,7 +709,7 @@ def bench_serve(args):
     arena = serve.FempicArena.for_budget(min(budget, args.serve_arena_gb * 1e9))
     # untimed warm-up on the same arena (first launches, the pinned host blocks results come back in)
     warm = [serve.Request(i, 0.0, small[i % len(small)], 10) for i in range(min(32, len(small)))]
-    serve.serve(arena, warm, budget)
+    _, wst = serve.serve(arena, warm, budget)
     _, st = serve.serve(arena, reqs, budget, log=log)
     log(f"continuous batching: {st}")
     del arena
@@ -719,7 +724,13 @@ def bench_serve(args):
         "value": st["value"],
         "unit": "particles/s",
         "n_gpus": 1,
+        # a step is one continuous-batching iteration (one batched launch sequence over the arena)
+        "steps": st["iterations"],
+        "warmup": wst["iterations"],
+        "ms_per_step": 1e3 * st["wall_s"] / max(st["iterations"], 1),
         "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
         "dtype": "f64 move, f32 compensated deposit",
         "data": "synthetic",
         "config": {"workload": f"Poisson {args.serve_rate}/s for {args.serve_seconds}s, {len(reqs)} requests of 10 "
@@ -734,6 +745,9 @@ def bench_serve(args):
         "plain_p99_latency_ms": sp["p99_latency_ms"],
         "host_s": {k: st[k] for k in ("admit_s", "step_s", "retire_s")},
         "speedup_vs_plain": st["value"] / sp["value"],
+        # the stream's kernels are the ensemble's (batched move, sort, FP32 deposit) at varying batch
+        # sizes; their roofline is the one --workload ensemble reports
+        "roofline": None,
     }
     if not args.no_cpu:
         ref = reference_on_instances([(r.tpl.inst, r.steps) for r in reqs], "requests")
