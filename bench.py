@@ -584,10 +584,30 @@ def bench_cabana(args):
     stream = torch.cuda.current_stream().cuda_stream
     L = oppx.lib()
 
+    every = args.cabana_sort_every
+    cur = [d_pos, d_vel, d_streak, d_w, d_p2c]
+    shapes = [(3, 8), (3, 8), (3, 8), (1, 8), (1, 4)]
+    swap = [torch.empty_like(x) for x in cur] if every > 0 else None
+    sort_bytes = L.oppx_sort_scratch_bytes(n) if every > 0 else 0
+    sort_scratch = torch.empty(max(sort_bytes, 1), dtype=torch.uint8, device="cuda")
+    descs = lambda ts: (oppx.DatDesc * len(ts))(*[oppx.DatDesc(x.data_ptr(), d, e) for x, (d, e) in zip(ts, shapes)])
+    state = {"parts": parts, "k": 0}
+
+    def sort():
+        # Sort_Periodic: a stable sort of every particle dat by cell into the swap buffers, then swap
+        nonlocal cur, swap
+        oppx.check(L.oppx_sort_by_cell(n, n, ctr.data_ptr(), nc, cur[4].data_ptr(), descs(cur), descs(swap), len(cur),
+                                       None, 0, sort_scratch.data_ptr(), sort_bytes, stream), "cabana sort")
+        cur, swap = swap, cur
+        state["parts"] = oppx.CabanaParts(n, *[x.data_ptr() for x in cur])
+
     def step():
         acc.zero_()
-        oppx.check(L.oppx_cabana_move_deposit(n, ctypes.byref(parts), ctypes.byref(mesh), acc.data_ptr(), deck["qsp"],
-                                              deck["qdt_2mc"], cd, ctr.data_ptr(), None, stream), "cabana")
+        oppx.check(L.oppx_cabana_move_deposit(n, ctypes.byref(state["parts"]), ctypes.byref(mesh), acc.data_ptr(),
+                                              deck["qsp"], deck["qdt_2mc"], cd, ctr.data_ptr(), None, stream), "cabana")
+        state["k"] += 1
+        if every > 0 and state["k"] % every == 0:
+            sort()
 
     copy_gbs = copy_peak_gbs(torch)
     log(f"device copy rate {copy_gbs:.0f} GB/s")
@@ -625,7 +645,9 @@ def bench_cabana(args):
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"CabanaPIC two-stream deck, {nx}x{ny}x{nz} periodic cells, {deck['nppc']} "
-                               f"particles/cell ({n}), fixed synthetic interpolated fields"},
+                               f"particles/cell ({n}), fixed synthetic interpolated fields",
+                   "sort_every": every},
+        "faults": int(ctr[6].item()),
         "p50_step_ms": float(np.percentile(ms, 50)),
         "p99_step_ms": float(np.percentile(ms, 99)),
         "roofline": {"kernel": "cabana_move_deposit_kernel", "bound": "hbm", "achieved": achieved,
@@ -785,6 +807,8 @@ def main():
                          "serve: the serving stream; neso: the NESO-advection pure move")
     ap.add_argument("--neso-nx", type=int, default=1024)
     ap.add_argument("--neso-particles", type=int, default=64_000_000)
+    ap.add_argument("--cabana-sort-every", type=int, default=0,
+                    help="CabanaPIC: sort the particles by cell after every K-th step (0: never)")
     ap.add_argument("--neso-sort-every", type=int, default=0,
                     help="NESO: sort particles by cell at set-up and every K steps inside the timed loop (0: never)")
     ap.add_argument("--serve-rate", type=float, default=400.0)
