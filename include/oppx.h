@@ -406,6 +406,16 @@ int oppx_neso_step(int64_t n_max, const oppx_neso_parts* parts, const oppx_quad_
                    double dt, double cell_width, double ext_x, double ext_y, int flags,
                    oppx_move_counters* counters, int32_t* hops, void* stream);
 
+/* opp_par_loop_all__verify_kernel (apps/app_neso_advection/kernels.h:115-150;
+ * apps_codegen/app_neso_advection_mdir_cg/hip/verify_kernel_loop.hpp): adds to *count (device int) the
+ * number of the counters->size particles that are not inside their cell: the cell's global index
+ * c_gbl_idx[p2c] (device [n_cells]) gives (ix, iy) = (g mod nx, g div nx), and a particle counts when
+ * ix or iy is negative or a coordinate lies outside [i * cell_width, i * cell_width + cell_width].
+ * A p2c outside [0, n_cells) is counted in counters->faults instead. pos is SoA with `stride`. */
+int oppx_neso_verify(int64_t n_max, int64_t stride, const double* pos, const int32_t* p2c, const int32_t* c_gbl_idx,
+                     int64_t n_cells, int nx, double cell_width, oppx_move_counters* counters, int32_t* count,
+                     void* stream);
+
 /* *mismatches (device) = the number of quad records that differ, bit for bit, from those of the
  * nx x ny periodic grid of apps/app_neso_advection/advec_misc.h:115-143 (corner (x, y) * cell_width,
  * neighbours wrapped on both axes; cell = x + nx * y). Zero lets oppx_neso_step_grid replace the

@@ -30,6 +30,7 @@ struct QuadCache {
 };
 static TetAuxCache g_tet;
 static QuadCache g_quad;
+static int32_t* g_verify_count = nullptr;  // device int of the NESO verify loop
 
 void free_field_loop_buffers();
 
@@ -37,6 +38,8 @@ void free_loop_caches() {
     free_field_loop_buffers();
     dev_free(g_tet.aux);
     dev_free(g_quad.cells);
+    dev_free(g_verify_count);
+    g_verify_count = nullptr;
     g_tet = TetAuxCache();
     g_quad = QuadCache();
 }
@@ -480,6 +483,41 @@ void opp_particle_move__move_kernel(opp_set set, opp_map c2c_map, opp_map p2c_ma
     OPP_move_max_hops = c.max_hops;
     set->particle_remove_count = 0;
     opp_profiler->end("move_kernel");
+}
+
+// apps_codegen/app_neso_advection_mdir_cg/hip/verify_kernel_loop.hpp; elemental apps/app_neso_advection/kernels.h:115-150.
+// arg0 p_pos (OPP_READ), arg1 the cells' global index through p2c (OPP_READ), arg2 the int OPP_INC global
+// that receives the number of particles outside their cell.
+void opp_par_loop_all__verify_kernel(opp_set set, opp_arg arg0, opp_arg arg1, opp_arg arg2) {
+    opp_profiler->start("verify_kernel");
+    if (arg2.argtype != OPP_ARG_GBL || !arg2.data || arg2.acc != OPP_INC)
+        opp_abort("verify_kernel: expects an int OPP_INC global");
+    opp_arg args[2] = {arg0, arg1};
+    args_to_device(2, args);
+    if (set->mesh_relation_dat->dirty_hd == Dirty::Device) opp_upload_dat(set->mesh_relation_dat);
+    sync_size_to_device(set);
+    int bytes = 0;
+    const char* nd = opp_const_data("CONST_ndimcells", &bytes);
+    if (!nd || bytes < (int)sizeof(int)) opp_abort("constant not declared: CONST_ndimcells");
+    int nx = 0;
+    memcpy(&nx, nd, sizeof(int));
+    if (!g_verify_count) g_verify_count = (int32_t*)dev_alloc(sizeof(int32_t));
+    check(oppx_memset_zero(g_verify_count, sizeof(int32_t), nullptr), "verify_kernel");
+    check(oppx_neso_verify(set->size, set->set_capacity, (const double*)arg0.dat->data_d,
+                           (const int32_t*)set->mesh_relation_dat->data_d, (const int32_t*)arg1.dat->data_d,
+                           arg1.dat->set->size, nx, const_real("CONST_cell_width"), pdev(set).counters, g_verify_count,
+                           nullptr),
+          "verify_kernel");
+    int32_t count = 0;
+    check(oppx_memcpy_dtoh(&count, g_verify_count, sizeof(int32_t), nullptr), "verify_kernel");
+    oppx_move_counters c;
+    read_counters(set, &c);
+    if (c.faults) opp_abort("verify_kernel: " + std::to_string(c.faults) + " particles hold a cell index outside the mesh");
+    int total = 0;
+    memcpy(&total, arg2.data, sizeof(int));
+    total += count;
+    memcpy(arg2.data, &total, sizeof(int));
+    opp_profiler->end("verify_kernel");
 }
 
 // =============================================================================================

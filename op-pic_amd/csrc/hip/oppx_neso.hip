@@ -272,6 +272,53 @@ int neso_step_impl(int64_t n_max, const oppx_neso_parts* parts, const oppx_quad_
     OPPX_LAUNCH_CHECK("neso_step");
     return OPPX_OK;
 }
+// verify_kernel (apps/app_neso_advection/kernels.h:115-150) over the counters->size particles: a
+// particle counts as incorrect when its cell's global index c_gbl_idx[p2c] splits into a negative
+// (ix, iy) = (g mod nx, g div nx), or when a coordinate lies outside [ll, ll + cell_width] with
+// ll = ix * cell_width (a product then a sum: the reference build forms them apart, no FMA). Block
+// sums of the count are added to *count.
+__global__ void __launch_bounds__(kBlock) neso_verify_kernel(int64_t n_max, int64_t stride, const double* __restrict__ pos,
+                                                             const int32_t* __restrict__ p2c,
+                                                             const int32_t* __restrict__ c_gbl_idx, int64_t n_cells,
+                                                             int nx, double cw, const oppx_move_counters* __restrict__ ctr,
+                                                             int32_t* __restrict__ count, int32_t* __restrict__ faults) {
+    int64_t n = ctr->size;
+    n = n < 0 ? 0 : (n > n_max ? n_max : n);
+    int bad = 0, flt = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const int c = p2c[i];
+        if (!((unsigned)c < (unsigned)n_cells)) {  // the reference would read outside c_idx
+            flt++;
+            continue;
+        }
+        const int g = c_gbl_idx[c];
+        const int iy = g / nx, ix = g - iy * nx;
+        if (ix < 0 || iy < 0) {
+            bad++;
+            continue;
+        }
+        const double llx = (double)ix * cw, x = pos[i];
+        if (x < llx || x > llx + cw) {
+            bad++;
+            continue;
+        }
+        const double lly = (double)iy * cw, y = pos[stride + i];
+        if (y < lly || y > lly + cw) bad++;
+    }
+    __shared__ int s_bad[kBlock / kWave], s_flt[kBlock / kWave];
+    bad = wave_sum_i32(bad);
+    flt = wave_sum_i32(flt);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0) { s_bad[w] = bad; s_flt[w] = flt; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int b = 0, f = 0;
+        for (int q = 0; q < kBlock / kWave; q++) { b += s_bad[q]; f += s_flt[q]; }
+        if (b) atomicAdd(count, b);
+        if (f && faults) atomicAdd(faults, f);
+    }
+}
+
 }  // namespace
 }  // namespace oppx
 
@@ -312,6 +359,22 @@ int oppx_neso_step_grid(int64_t n_max, const oppx_neso_parts* parts, int nx, int
     OPPX_ARG_CHECK(nx > 0 && ny > 0 && (int64_t)nx * ny < kMaxCell, "oppx_neso_step_grid: nx, ny out of range");
     return neso_step_impl(n_max, parts, nullptr, (int64_t)nx * ny, nx, ny, dt, cell_width, ext_x, ext_y, flags,
                           counters, hops, stream);
+}
+
+int oppx_neso_verify(int64_t n_max, int64_t stride, const double* pos, const int32_t* p2c, const int32_t* c_gbl_idx,
+                     int64_t n_cells, int nx, double cell_width, oppx_move_counters* counters, int32_t* count,
+                     void* stream) {
+    OPPX_ARG_CHECK(n_max >= 0 && stride >= n_max, "oppx_neso_verify: need 0 <= n_max <= stride");
+    OPPX_ARG_CHECK(counters && count, "oppx_neso_verify: null counters or count");
+    OPPX_ARG_CHECK(nx > 0 && n_cells > 0 && n_cells < kMaxCell, "oppx_neso_verify: nx or n_cells out of range");
+    if (n_max == 0) return OPPX_OK;
+    OPPX_ARG_CHECK(pos && p2c && c_gbl_idx, "oppx_neso_verify: null array");
+    unsigned g = grid_for(n_max, kBlock);
+    if (g > 4096) g = 4096;
+    neso_verify_kernel<<<g, kBlock, 0, as_stream(stream)>>>(n_max, stride, pos, p2c, c_gbl_idx, n_cells, nx, cell_width,
+                                                            counters, count, &counters->faults);
+    OPPX_LAUNCH_CHECK("neso_verify");
+    return OPPX_OK;
 }
 
 }  // extern "C"

@@ -11,17 +11,34 @@
 // drift + move for NESO-advection) run as gfx950 kernels through include/oppx.h.
 #pragma once
 
+// The standard headers the reference's opp_defs.h and opp_profiler.h include
+// (opp_lib/include/opp_defs.h:35-56, opp_lib/include/opp_profiler.h:35-42):
+// generated code and application headers use std::mt19937, PRId64, std::ostringstream, ... without
+// including them.
+#include <inttypes.h>
+#include <sys/time.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <array>
 #include <cfloat>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <functional>
+#include <iomanip>
+#include <iostream>
 #include <limits>
 #include <map>
 #include <memory>
+#include <random>
+#include <sstream>
+#include <stack>
 #include <string>
 #include <vector>
 
@@ -70,6 +87,22 @@ constexpr int MIN_INT = std::numeric_limits<int>::min();
 #define OPP_DEFAULT_GPU_THREADS_PER_BLOCK 32
 #define LOG_STR_LEN 100000
 #define OPP_IS_VALID_PROCESS (OPP_rank >= 0)
+#define OPP_RETURN_IF_INVALID_PROCESS if (OPP_rank < 0) return;
+#define OPP_RETURN_NULL_IF_INVALID_PROCESS if (OPP_rank < 0) return nullptr;
+#define OPP_RETURN_ZERO_IF_INVALID_PROCESS if (OPP_rank < 0) return 0;
+#define OPP_RETURN_EMPTY_IF_INVALID_PROCESS if (OPP_rank < 0) return "";
+#define OPP_RETURN_ARG_IF_INVALID_PROCESS if (OPP_rank < 0) return opp_arg();
+#define UNUSED(expr) do { (void)(expr); } while (0)
+#define ZERO_double 0.0
+#define ZERO_float 0.0f
+#define ZERO_int 0
+#define ZERO_uint 0
+#define ZERO_ll 0
+#define ZERO_ull 0
+#define ZERO_bool 0
+#define OPP_REAL_ZERO 0.0
+#define OPP_INT_ZERO 0
+#define OPP_BOOL_ZERO 0
 
 #ifndef MIN
 #define MIN(a, b) ((a < b) ? (a) : (b))
@@ -78,6 +111,8 @@ constexpr int MIN_INT = std::numeric_limits<int>::min();
 #define MAX(a, b) ((a > b) ? (a) : (b))
 #endif
 #define FMOD(a, b) ((a) - (b) * trunc((a) / (b)))
+
+#include "opp_util.h"
 
 enum opp_iterate_type { OPP_ITERATE_ALL = 1, OPP_ITERATE_INJECTED };
 enum opp_move_status { OPP_MOVE_DONE = 0, OPP_NEED_MOVE, OPP_NEED_REMOVE };
@@ -460,5 +495,6 @@ void opp_particle_move__move_deposit_kernel(opp_set, opp_map, opp_map, opp_arg, 
                                             opp_arg);
 // NESO-advection
 void opp_par_loop_all__update_pos_kernel(opp_set, opp_arg, opp_arg);
+void opp_par_loop_all__verify_kernel(opp_set, opp_arg, opp_arg, opp_arg);
 void opp_particle_move__move_kernel(opp_set, opp_map, opp_map, opp_arg, opp_arg, opp_arg);
 void opp_init_direct_hop_cg(double, const opp_dat, const opp::BoundingBox&, opp_map, opp_map, opp_arg, opp_arg, opp_arg);

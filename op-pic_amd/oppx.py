@@ -140,6 +140,8 @@ _SIGS = {
     "oppx_neso_step": (c_int, [c_int64, POINTER(NesoParts), c_void_p, c_int64, c_double, c_double, c_double, c_double,
                                c_int, c_void_p, c_void_p, c_void_p]),
     "oppx_quad_grid_check": (c_int, [c_void_p, c_int64, c_int, c_int, c_double, c_void_p, c_void_p]),
+    "oppx_neso_verify": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_int, c_double, c_void_p,
+                                 c_void_p, c_void_p]),
     "oppx_fempic_inject": (c_int, [c_int64, c_int64, POINTER(FempicParts), POINTER(Inlet), c_int64, c_void_p, c_void_p,
                                    c_int64, c_double, c_double, c_double, c_double, c_void_p, c_void_p]),
     "oppx_fempic_node_charge_density": (c_int, [c_int64, c_void_p, c_void_p, c_double, c_void_p]),

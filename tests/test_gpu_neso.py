@@ -91,3 +91,45 @@ def test_gpu_quad_grid_check_rejects_other_meshes():
     c = m2["c2c"].reshape(-1, 4)
     c[5, 0], c[5, 1] = c[5, 1], c[5, 0]
     assert not oppx.NesoDevice(m2, pos, vel, p2c).grid
+
+
+@pytest.mark.parametrize("n,seed", [(0, 1), (1, 2), (5000, 3), (200_003, 4)])
+def test_gpu_neso_verify_counts_misplaced(n, seed):
+    """oppx_neso_verify (the verify loop of apps/app_neso_advection/kernels.h:115-150) on particles of
+    which some sit outside their cell, some exactly on a cell edge (inside: the test is inclusive), and
+    some hold a cell whose global index is negative; the count equals the closed form computed here."""
+    import torch
+
+    nx, ny, cw = 37, 23, 0.25
+    rng = np.random.default_rng(seed)
+    n_cells = nx * ny
+    gidx = np.arange(n_cells, dtype=np.int32)
+    gidx[rng.random(n_cells) < 0.05] = -3  # a corrupt global index: ix, iy negative
+    p2c = rng.integers(0, n_cells, n).astype(np.int32)
+    g = gidx[p2c].astype(np.int64)
+    ix, iy = g % nx, g // nx
+    # C division: (-3) / nx == 0 and -3 - 0 == -3, so ix < 0 for the corrupt entries
+    ix = np.where(g < 0, g - np.trunc(g / nx).astype(np.int64) * nx, ix)
+    iy = np.where(g < 0, np.trunc(g / nx).astype(np.int64), iy)
+    u = rng.random((n, 2)) * 1.2 - 0.1  # about a third of the particles leave their box
+    edge = rng.random(n) < 0.05
+    u[edge, 0] = 1.0  # exactly on the upper x edge: inside
+    pos = np.stack([(ix * cw) + u[:, 0] * cw, (iy * cw) + u[:, 1] * cw], axis=1)
+    pos[edge, 0] = ix[edge] * cw + cw
+    llx, lly = ix * cw, iy * cw
+    outside = (g < 0) | (pos[:, 0] < llx) | (pos[:, 0] > llx + cw) | (pos[:, 1] < lly) | (pos[:, 1] > lly + cw)
+    expect = int(outside.sum())
+    cap = n + 5
+    pos_d = torch.zeros((2, cap), dtype=torch.float64, device="cuda")
+    pos_d[:, :n] = torch.from_numpy(pos.T.copy())
+    p2c_d = torch.zeros(cap, dtype=torch.int32, device="cuda")
+    p2c_d[:n] = torch.from_numpy(p2c)
+    gidx_d = torch.from_numpy(gidx).to("cuda")
+    ctr = torch.zeros(oppx.COUNTER_WORDS, dtype=torch.int32, device="cuda")
+    ctr[0] = n
+    count = torch.full((1,), 7, dtype=torch.int32, device="cuda")  # the kernel adds to what is there
+    oppx.check(oppx.lib().oppx_neso_verify(n, cap, pos_d.data_ptr(), p2c_d.data_ptr(), gidx_d.data_ptr(), n_cells, nx,
+                                           cw, ctr.data_ptr(), count.data_ptr(),
+                                           torch.cuda.current_stream().cuda_stream), "neso_verify")
+    assert int(count.item()) == 7 + expect
+    assert int(ctr[6].item()) == 0  # faults
