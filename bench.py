@@ -592,6 +592,11 @@ def bench_cabana(args):
     sort_scratch = torch.empty(max(sort_bytes, 1), dtype=torch.uint8, device="cuda")
     descs = lambda ts: (oppx.DatDesc * len(ts))(*[oppx.DatDesc(x.data_ptr(), d, e) for x, (d, e) in zip(ts, shapes)])
     state = {"parts": parts, "k": 0}
+    # the kernel resets its fault and hop counters at every launch: each step's values are kept on the
+    # device (no host round trip inside the timed loop) and summed afterwards
+    n_log = args.warmup + args.steps
+    fault_log = torch.zeros(n_log, dtype=torch.int32, device="cuda")
+    hops_log = torch.zeros(n_log, dtype=torch.int64, device="cuda")
 
     def sort():
         # Sort_Periodic: a stable sort of every particle dat by cell into the swap buffers, then swap
@@ -605,6 +610,8 @@ def bench_cabana(args):
         acc.zero_()
         oppx.check(L.oppx_cabana_move_deposit(n, ctypes.byref(state["parts"]), ctypes.byref(mesh), acc.data_ptr(),
                                               deck["qsp"], deck["qdt_2mc"], cd, ctr.data_ptr(), None, stream), "cabana")
+        fault_log[state["k"]] = ctr[6]
+        hops_log[state["k"]] = ctr[4:6].view(torch.int64)[0]
         state["k"] += 1
         if every > 0 and state["k"] % every == 0:
             sort()
@@ -623,10 +630,12 @@ def bench_cabana(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t1
     ms = np.array([a.elapsed_time(b) for a, b in evs])
-    c = ctr.cpu().numpy()
-    hops = int(c[4:6].view(np.int64)[0])
+    faults = fault_log.cpu().numpy()
+    hops_pp = hops_log.cpu().numpy()[args.warmup:] / n
     log(f"cabana: {1e3 * wall / args.steps:.3f} ms/step, p50 {np.percentile(ms, 50):.3f} ms, "
-        f"{hops / n:.3f} evaluations/particle in the last step, faults {int(c[6])}")
+        f"{hops_pp[0]:.4f} -> {hops_pp[-1]:.4f} evaluations/particle over the timed steps, faults {int(faults.sum())}")
+    if faults.sum():
+        raise RuntimeError(f"cabana walk faults in {int((faults > 0).sum())} steps: {faults.tolist()}")
     # R pos 24, vel 24, streak 24, weight 8, p2c 4; W pos 24, vel 24, streak 24, p2c 4 per particle;
     # per cell: interp 144 R, acc 96 RW (zeroing included), c2c 24 R
     alg = 160.0 * n + (144 + 96 + 96 + 24) * nc
@@ -647,7 +656,9 @@ def bench_cabana(args):
         "config": {"workload": f"CabanaPIC two-stream deck, {nx}x{ny}x{nz} periodic cells, {deck['nppc']} "
                                f"particles/cell ({n}), fixed synthetic interpolated fields",
                    "sort_every": every},
-        "faults": int(ctr[6].item()),
+        "faults": int(faults.sum()),
+        "evaluations_per_particle_by_step": [round(float(h), 5) for h in hops_pp],
+        "step_ms_by_step": [round(float(x), 4) for x in ms],
         "p50_step_ms": float(np.percentile(ms, 50)),
         "p99_step_ms": float(np.percentile(ms, 99)),
         "roofline": {"kernel": "cabana_move_deposit_kernel", "bound": "hbm", "achieved": achieved,

@@ -113,7 +113,8 @@ typedef struct oppx_tuning {
     int32_t sort_items;
     int32_t inline_hops;
     int32_t first_block;       /* threads per block of the first-hop pass: 256 or 1024 */
-    int32_t reserved;
+    int32_t lanes;             /* lanes per particle in the first-hop pass: 1, or 4 (a quad walks one
+                                  particle, lane r owning tet vertex r) */
 } oppx_tuning;
 
 /* One instance of a batched FEM-PIC ensemble. The instances' meshes are concatenated into one
@@ -190,6 +191,30 @@ int oppx_box_particles_uniform(int64_t n_part, uint64_t seed, double sigma, int 
                                double lx, double ly, double lz, int order, double* pos, double* vel, int* p2c);
 /* node -> (cell, vertex) incidence: node_start [n_nodes+1], node_ref [4*n_cells] = c*4+k ascending per node. */
 int oppx_node_cell_csr(int64_t n_cells, int64_t n_nodes, const int* c2n, int* node_start, int* node_ref);
+
+/* FEM-PIC text mesh files (mesh.dat: node count, element count, "index x y z" in mm, "index 304 n1 n2 n3 n4"
+ * tets with 1-based nodes; inlet.dat: inlet nodes, then "index 203 n1 n2 n3" triangles; wall.dat: fixed nodes).
+ * oppx_box12_write writes the 12-tet box mesh of nx x ny x nz cubes of edge cube_length mm into dir in that
+ * format (node and tet numbering, face rows and node lists of scripts/batch/fempic/box_mesh_gen.py:19-178):
+ * corners numbered x fastest, then one centre node per cube; tets 12q .. 12q+11 belong to cube q.
+ * oppx_mesh_file_load reads the files as apps/app_fempic/minifempic_funcs.h:230-588 (LoadVolumeMesh,
+ * LoadSurfaceMesh) does: positions in m, volume sign flip, c_det [16 n_cells] (alpha, beta, gamma, delta per
+ * vertex, packed as apps_codegen/app_fempic_cg/fempic_misc_mesh_loader.h:119-127), c_vol, c2c (-1 on the
+ * boundary), node volumes (centroid weights x tet volume, scattered in tet order), node types (0 normal,
+ * 1 open, 2 inlet, 3 fixed) and the inlet faces (owning tet, nodes, u, v, unit normal, area). inlet_path and
+ * wall_path may be NULL. Host arrays sized by oppx_mesh_file_sizes. */
+int oppx_box12_write(int nx, int ny, int nz, double cube_length, const char* dir);
+/* n_part particles uniform in [0,lx]x[0,ly]x[0,lz] (the streams of oppx_box_particles_uniform) on a 12-tet box
+ * mesh read back from oppx_box12_write's files, each tagged with the first of its cube's 12 tets that the
+ * mover's inside test accepts (OPPX_ERR_STATE if one is in none). */
+int oppx_box12_particles(int64_t n_part, uint64_t seed, double sigma, int nx, int ny, int nz, double lx, double ly,
+                         double lz, const double* c_det, const double* c_vol, double* pos, double* vel, int* p2c);
+int oppx_mesh_file_sizes(const char* mesh_path, const char* inlet_path, int64_t* n_nodes, int64_t* n_cells,
+                         int64_t* n_ifaces);
+int oppx_mesh_file_load(const char* mesh_path, const char* inlet_path, const char* wall_path, double* node_pos,
+                        int32_t* node_type, double* node_vol, int32_t* c2n, int32_t* c2c, double* c_det,
+                        double* c_vol, int32_t* if2c, int32_t* if2n, double* if_u, double* if_v, double* if_normal,
+                        double* if_area);
 /* Overlay dims of CellMapper::init_host (opp_lib/include/opp_direct_hop_core.h:317-336). */
 int oppx_dh_grid_dims(const double bmin[3], const double bmax[3], double gs, int64_t dims[3]);
 /* Overlay contents of CellMapper::generateStructuredMesh (opp_lib/src/seq/opp_direct_hop_seq.cpp:134-265),
@@ -460,7 +485,9 @@ int oppx_holefill(int64_t n_max, int64_t stride, oppx_move_counters* counters, c
  * node_charge (double, nullable) and/or node_charge_f32 (nullable). The OPP_INC deposit of
  * apps/app_fempic/kernels.h:164-175 in the segmented-reduction form of
  * apps_codegen/app_fempic_cg/hip/deposit_charge_on_nodes_kernel_loop.hpp:98-158; every addition order
- * is fixed by the data (deterministic). scratch: oppx_fempic_deposit_scratch_bytes(n_max) bytes. */
+ * is fixed by the data (deterministic). scratch: oppx_fempic_deposit_scratch_bytes(n_max) bytes. A p2c that
+ * breaks the precondition (a valid cell right after a larger one, or after a removed particle) is reported:
+ * each such run start adds one to counters->faults, and the charge is then not the deposit's. */
 size_t oppx_fempic_deposit_scratch_bytes(int64_t n_max);
 int oppx_fempic_deposit_sorted_f32(int64_t n_max, int64_t stride, oppx_move_counters* counters, const int32_t* p2c,
                                    const double* lc, int64_t n_cells, const int32_t* node_start, const int32_t* node_end,

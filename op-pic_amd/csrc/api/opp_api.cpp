@@ -736,14 +736,16 @@ void opp_upload_particle_set(opp_set set, bool realloc) {
 
 // ---------------------------------------------------------------------------------------------
 // particle count growth (opp_lib/src/core/opp_increase_part_count_core.cpp:35-110): capacity grows to
-// size + insert * alloc_mult; device arrays grow by device-to-device copies of the live range;
-// new mesh-relation slots hold MAX_CELL_INDEX.
+// size + insert * alloc_mult, and never below the new size (an opp_allocation_multiple below 1 would
+// otherwise leave the new particles past the allocation); device arrays grow by device-to-device copies
+// of the live range; new mesh-relation slots hold MAX_CELL_INDEX.
 void opp_increase_particle_count(opp_set set, int insert_count) {
     if (!set->is_particle) opp_abort("opp_increase_particle_count: not a particle set");
     if (insert_count <= 0) return;
     const int new_size = set->size + insert_count;
     if (set->set_capacity < new_size) {
-        const size_t new_cap = (size_t)set->size + (size_t)((size_t)insert_count * OPP_part_alloc_mult);
+        size_t new_cap = (size_t)set->size + (size_t)((size_t)insert_count * OPP_part_alloc_mult);
+        if (new_cap < (size_t)new_size) new_cap = (size_t)new_size;
         if (new_cap > (size_t)INT_MAX - 1) opp_abort("opp_increase_particle_count: capacity beyond int range");
         for (opp_dat d : *set->particle_dats) {
             char* h = (char*)realloc(d->data, new_cap * d->size);

@@ -90,7 +90,10 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* tot) {
     return off + inc - v;
 }
 
-__global__ void __launch_bounds__(kB) run_count(const int32_t* __restrict__ p2c, const oppx_move_counters* ctr,
+// Also checks the precondition: a valid cell right after a larger cell or after an entry that is not a
+// valid cell (a removed particle) means p2c is not sorted, and two runs of one cell would overwrite each
+// other's partials; every such entry is counted in ctr->faults.
+__global__ void __launch_bounds__(kB) run_count(const int32_t* __restrict__ p2c, oppx_move_counters* ctr,
                                                 int64_t n_max, int64_t n_cells, uint32_t* __restrict__ bsum) {
     __shared__ RunTile T;
     const int64_t n = live_of(ctr, n_max);
@@ -98,11 +101,21 @@ __global__ void __launch_bounds__(kB) run_count(const int32_t* __restrict__ p2c,
     load_run_tile(T, p2c, base0, n);
     const int e0 = (int)threadIdx.x * kRunItems;
     uint32_t c = 0;
-    for (int r = 0; r < kRunItems; r++)
-        if (base0 + e0 + r < n && run_head(T, e0 + r, n_cells)) c++;
+    int unsorted = 0;
+    for (int r = 0; r < kRunItems; r++) {
+        const int64_t j = base0 + e0 + r;
+        if (j >= n) break;
+        if (run_head(T, e0 + r, n_cells)) {
+            c++;
+            const int32_t prev = T.v[tile_at(e0 + r)], cur = T.v[tile_at(e0 + r + 1)];
+            if (j > 0 && (!((unsigned)prev < (unsigned)n_cells) || prev > cur)) unsorted++;
+        }
+    }
     uint32_t tot;
     block_scan(c, &tot);
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+    unsorted = wave_sum_i32(unsorted);
+    if ((threadIdx.x & (kWave - 1)) == 0 && unsorted) atomicAdd(&ctr->faults, unsorted);
 }
 
 // single block: exclusive scan of nb block counts in place; the total goes to n_runs[0], and the

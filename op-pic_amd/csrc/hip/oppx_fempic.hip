@@ -501,6 +501,206 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
     block_stats(P, done ? h : 0, done ? h : 0, (live && removed) ? 1 : 0, fault ? 1 : 0);
 }
 
+// ---------------------------------------------------------------------------------------------
+// Quad variant of the first pass: four lanes walk one particle. Lane r of a quad owns vertex r of the
+// current tet: it loads that vertex's four plane coefficients (32 B of the tet's 128 B c_det line) and
+// one 8 B word of the 32 B aux record, and computes lc_r; the quad combines the inside test and the
+// first-minimum neighbour choice through quad DPP moves, so every branch is the same in all four lanes.
+// Per hop a lane issues 3 loads instead of the one-lane walk's 10, the four lanes of a quad touch one
+// line per load, and a wave's longest walk is the longest of 16 particles instead of 64. The push is
+// split the same way (lane d < 3 owns component d), so are the deposit's four node adds and the lc stores.
+// Same results as fempic_first_kernel.
+constexpr int kQuadBlock = 256;                  // threads per block
+constexpr int kQuadParts = kQuadBlock / 4;       // particles per block (one 64-bit removal word)
+
+// Value of lane SRC of the quad (DPP quad_perm broadcast); every lane of the quad must be active.
+template <int SRC>
+__device__ __forceinline__ int quad_bcast_i32(int v) {
+    return __builtin_amdgcn_mov_dpp(v, SRC * 0x55, 0xF, 0xF, false);
+}
+template <int SRC>
+__device__ __forceinline__ double quad_bcast(double v) {
+    const int2 h = __builtin_bit_cast(int2, v);
+    return __builtin_bit_cast(double, make_int2(quad_bcast_i32<SRC>(h.x), quad_bcast_i32<SRC>(h.y)));
+}
+// AND over the quad (quad_perm [1,0,3,2] then [2,3,0,1]).
+__device__ __forceinline__ int quad_and(int v) {
+    v &= __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);
+    v &= __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);
+    return v;
+}
+// Bits 0, 4, 8, ..., 60 of m packed into bits 0..15.
+__device__ __forceinline__ uint32_t every_fourth_bit(unsigned long long m) {
+    m &= 0x1111111111111111ull;
+    m = (m | (m >> 3)) & 0x0303030303030303ull;
+    m = (m | (m >> 6)) & 0x000F000F000F000Full;
+    m = (m | (m >> 12)) & 0x000000FF000000FFull;
+    m = (m | (m >> 24)) & 0xFFFFull;
+    return (uint32_t)m;
+}
+
+// One evaluation of apps/app_fempic/kernels.h:106-161 by the quad at `cell` (quad-uniform): lane r
+// returns lc_r in l; the result code and the new cell are the same in all four lanes.
+__device__ __forceinline__ int quad_hop(const StepArgs& P, int r, double x, double y, double z, int& cell, double& l) {
+    const double2* __restrict__ d = reinterpret_cast<const double2*>(P.det + 16 * (int64_t)cell + 4 * r);
+    const double2 ab = d[0], gd = d[1];  // alpha, beta ; gamma, delta of vertex r
+    const double w = reinterpret_cast<const double*>(P.aux + cell)[r];  // c2 | c2c[0..1] | c2c[2..3] | pad
+    const double c2 = quad_bcast<0>(w);
+    l = c2 * fma(-gd.y, z, fma(gd.x, y, fma(-ab.y, x, ab.x)));
+    const int in = (l > -1e-10) && ((l - 1.0) < 1e-10);
+    if (quad_and(in)) return kHopInside;
+    const double l0 = quad_bcast<0>(l), l1 = quad_bcast<1>(l), l2 = quad_bcast<2>(l), l3 = quad_bcast<3>(l);
+    const int2 n01 = __builtin_bit_cast(int2, quad_bcast<1>(w)), n23 = __builtin_bit_cast(int2, quad_bcast<2>(w));
+    int nb = n01.x;
+    double ml = l0;
+    if (l1 < ml) { ml = l1; nb = n01.y; }
+    if (l2 < ml) { ml = l2; nb = n23.x; }
+    if (l3 < ml) { ml = l3; nb = n23.y; }
+    if (nb < 0) {
+        cell = kMaxCell;
+        return kHopExit;
+    }
+    if (nb >= P.n_cells) return kHopBad;
+    cell = nb;
+    return kHopMove;
+}
+
+template <bool PUSH, bool DH, bool DEPOSIT, bool BATCH, bool GATHER>
+__global__ void __launch_bounds__(kQuadBlock) fempic_quad_kernel(StepArgs P, uint32_t nblocks, int inline_hops, WorkList W) {
+    const uint32_t b = xcd_block(blockIdx.x, nblocks);
+    const int r = threadIdx.x & 3;
+    const int64_t i = (int64_t)b * kQuadParts + (threadIdx.x >> 2);
+    int64_t n = P.ctr->size;
+    n = n < 0 ? 0 : (n > P.n_max ? P.n_max : n);
+    if ((int64_t)b * kQuadParts >= n) return;  // whole block past the live count
+    const bool live = i < n;                     // quad-uniform
+    const int64_t S = P.stride;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const double* __restrict__ spos = GATHER ? P.src_pos : P.pos;
+    const double* __restrict__ svel = GATHER ? P.src_vel : P.vel;
+    const int32_t* __restrict__ sp2c = GATHER ? P.src_p2c : P.p2c;
+    auto ld = [](const auto* p) { return GATHER ? *p : ld_stream(p); };
+    int64_t src = i;
+    bool fault = false;
+    if (GATHER && live) {
+        src = P.perm[i];
+        if (!((uint64_t)src < (uint64_t)S)) {
+            fault = true;
+            src = i;
+        }
+    }
+    const bool p2c_in_place = GATHER && P.src_p2c == nullptr;
+    int cell = (live && !fault) ? (p2c_in_place ? ld_stream(P.p2c + i) : ld(sp2c + src)) : kMaxCell;
+    const bool skip = live && !fault && cell == kMaxCell;
+    fault = fault || (live && !skip && !((unsigned)cell < (unsigned)P.n_cells));
+    bool removed = false, pending = false, pushed = false, p2c_done = false, lc_done = false;
+    int h = 0;
+    double l = 0.0;
+    __shared__ DepositTable s_table;
+    __shared__ uint32_t s_bits[kQuadBlock / kWave];
+    if (DEPOSIT) table_clear(s_table);
+
+    if (live && !skip && !fault) {
+        double kk = P.k, dt = P.dt;
+        DhView D = P.dh;
+        if (BATCH) {
+            const int in = P.cell_inst[cell];
+            if ((unsigned)in < (unsigned)P.n_insts) {
+                const oppx_batch_inst* I = P.insts + in;
+                kk = I->k;
+                dt = I->dt;
+                D.cell = I->dh_cell;
+                D.dx = I->dims[0]; D.dy = I->dims[1]; D.dz = I->dims[2];
+                D.mx = I->min[0]; D.my = I->min[1]; D.mz = I->min[2];
+                D.inv_gs = I->inv_gs;
+            } else {
+                fault = true;
+            }
+        }
+        if (!fault) {
+            // lane d < 3 owns component d: x_d (and, pushing, v_d = fma(k, E_d, v_d); x_d = fma(v_d, dt, x_d))
+            const int d = r < 3 ? r : 2;
+            double xd = ld(spos + d * S + src);
+            if (PUSH) {
+                const double e = P.ef[3 * (int64_t)cell + d];
+                double vd = ld(svel + d * S + src);
+                vd = fma(kk, e, vd);
+                xd = fma(vd, dt, xd);
+                if (r < 3) {
+                    st_stream(P.vel + d * S + i, vd);
+                    st_stream(P.pos + d * S + i, xd);
+                }
+                pushed = true;
+            }
+            const double x = quad_bcast<0>(xd), y = quad_bcast<1>(xd), z = quad_bcast<2>(xd);
+            if (DH && (!BATCH || D.cell != nullptr)) {
+                cell = dh_lookup(D, x, y, z);
+                if (cell == kMaxCell) removed = true;
+                else if (!((unsigned)cell < (unsigned)P.n_cells)) fault = true;
+            }
+            while (!removed && !fault) {
+                const int res = quad_hop(P, r, x, y, z, cell, l);
+                h++;
+                if (res == kHopInside) break;
+                if (res == kHopExit) { removed = true; break; }
+                if (res == kHopBad || h >= OPPX_MAX_HOPS) { fault = true; break; }
+                if (h >= inline_hops) { pending = true; break; }
+            }
+        }
+        if (!fault) {
+            if (!pending && !removed && h > 0) {
+                st_stream(P.lc + r * S + i, l);
+                lc_done = true;
+            }
+            if (r == 0) st_stream(P.p2c + i, removed ? kMaxCell : cell);
+            p2c_done = true;
+        }
+        if (P.hops && r == 0) st_stream(P.hops + i, h);
+    }
+    if (GATHER && live) {
+        if (!pushed && r < 3) {
+            st_stream(P.pos + r * S + i, spos[r * S + src]);
+            st_stream(P.vel + r * S + i, svel[r * S + src]);
+        }
+        if (!p2c_done && !p2c_in_place && r == 0) st_stream(P.p2c + i, sp2c[src]);
+        if (!lc_done) st_stream(P.lc + r * S + i, P.src_lc[r * S + src]);
+        if (P.hops && r == 0 && !(live && !skip && !fault)) P.hops[i] = h;
+    }
+
+    // still-moving particles go to the worklist (one entry per quad, from its lane 0)
+    const bool lead = r == 0;
+    const unsigned long long pmask = __ballot(pending && lead);
+    if (pmask) {
+        const uint32_t q = blockIdx.x % kSlots;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&P.qcount[q], __popcll(pmask));
+        base = __shfl(base, 0);
+        if (pending && lead) {
+            const int64_t slot = base + __popcll(pmask & ((1ull << lane) - 1ull));
+            if (slot < W.cap) W.entry[q * W.cap + slot] = make_int2((int)i, h);
+        }
+    }
+    if (DEPOSIT) {
+        __syncthreads();  // table cleared
+        if (live && !skip && !fault && !removed && !pending) {
+            const int node = P.c2n[4 * (int64_t)cell + r];
+            table_add(s_table, P.charge, node, l);
+        }
+    }
+    // removal bitmap: the block's 64 particles are one word, 16 bits per wave
+    const uint32_t bits16 = every_fourth_bit(__ballot(live && removed && lead));
+    if (lane == 0) s_bits[wave] = bits16;
+    __syncthreads();
+    if (DEPOSIT) table_flush(s_table, P.charge);
+    if (threadIdx.x == 0) {
+        unsigned long long word = 0ull;
+        for (int w = 0; w < kQuadBlock / kWave; w++) word |= (unsigned long long)s_bits[w] << (16 * w);
+        P.hole_bits[b] = word;
+    }
+    const bool done = live && !skip && !pending && lead;
+    block_stats(P, done ? h : 0, done ? h : 0, (live && removed && lead) ? 1 : 0, (fault && lead) ? 1 : 0);
+}
+
 template <bool DEPOSIT>
 __global__ void __launch_bounds__(kBlock) fempic_walk_kernel(StepArgs P, WorkList W) {
     const uint32_t q = blockIdx.x % kSlots;
@@ -900,7 +1100,10 @@ template <bool PUSH, bool DH, bool DEPOSIT, bool BATCH, bool GATHER = false>
 bool launch_two_pass(const StepArgs& P, int64_t n_max, void* work, size_t work_bytes, hipStream_t s) {
     if (!P.slots || g_tuning.inline_hops < 1 || work_bytes < work_need(n_max)) return false;
     WorkList W{reinterpret_cast<int2*>(reinterpret_cast<char*>(work) + kWorkHeader), work_cap(n_max)};
-    if (g_tuning.first_block == kMaxBlock) {
+    if (g_tuning.lanes == 4) {
+        const unsigned g = grid_for(n_max, kQuadParts);
+        fempic_quad_kernel<PUSH, DH, DEPOSIT, BATCH, GATHER><<<g, kQuadBlock, 0, s>>>(P, g, g_tuning.inline_hops, W);
+    } else if (g_tuning.first_block == kMaxBlock) {
         const unsigned g = grid_for(n_max, kMaxBlock);
         fempic_first_kernel<kMaxBlock, PUSH, DH, DEPOSIT, BATCH, GATHER><<<g, kMaxBlock, 0, s>>>(P, g, g_tuning.inline_hops, W);
     } else {
@@ -1131,6 +1334,7 @@ int oppx_set_tuning(const oppx_tuning* t) {
                    "oppx_set_tuning: sort_items must be 4, 8 or 16");
     OPPX_ARG_CHECK(t->inline_hops >= 0 && t->inline_hops <= 64, "oppx_set_tuning: inline_hops in [0, 64]");
     OPPX_ARG_CHECK(t->first_block == kBlock || t->first_block == kMaxBlock, "oppx_set_tuning: first_block 256 or 1024");
+    OPPX_ARG_CHECK(t->lanes == 1 || t->lanes == 4, "oppx_set_tuning: lanes (per particle in the first pass) 1 or 4");
     g_tuning = *t;
     return OPPX_OK;
 }

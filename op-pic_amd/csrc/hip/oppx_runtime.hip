@@ -10,7 +10,7 @@ namespace oppx {
 
 static thread_local std::string g_last_error;
 
-oppx_tuning g_tuning = {256, 16, 0, 0, 16, 64, 256, 0};
+oppx_tuning g_tuning = {256, 16, 0, 0, 16, 64, 256, 1};
 
 unsigned resident_blocks(const void* kernel, int block) {
     static int cus = 0;

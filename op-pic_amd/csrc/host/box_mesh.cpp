@@ -23,27 +23,12 @@
 #include <vector>
 
 #include "oppx.h"
+#include "tet_geometry.h"
+
+using oppx_host::det3;
+using oppx_host::tet_contains;
 
 namespace {
-
-// ---------------------------------------------------------------------------------------
-// Determinants as written in minifempic_funcs.h:69-103 (cofactor expansion).
-double det3(const double M[3][3]) {
-    return M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
-           M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
-           M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
-}
-
-double det4(const double M[4][4]) {
-    double M0[3][3], M1[3][3], M2[3][3], M3[3][3];
-    for (int i = 0; i < 3; i++) {
-        M0[i][0] = M[i + 1][1]; M0[i][1] = M[i + 1][2]; M0[i][2] = M[i + 1][3];
-        M1[i][0] = M[i + 1][0]; M1[i][1] = M[i + 1][2]; M1[i][2] = M[i + 1][3];
-        M2[i][0] = M[i + 1][0]; M2[i][1] = M[i + 1][1]; M2[i][2] = M[i + 1][3];
-        M3[i][0] = M[i + 1][0]; M3[i][1] = M[i + 1][1]; M3[i][2] = M[i + 1][2];
-    }
-    return M[0][0] * det3(M0) - M[0][1] * det3(M1) + M[0][2] * det3(M2) - M[0][3] * det3(M3);
-}
 
 // The 6 tets of a unit cube that share the diagonal (0,0,0)-(1,1,1). Tet t walks the
 // cube corners along axis order kKuhnPerm[t]: v0=(0,0,0), v1=e_a, v2=e_a+e_b, v3=(1,1,1).
@@ -73,18 +58,6 @@ inline double rng_normal(uint64_t seed, uint64_t ctr) {
     const double u1 = 1.0 - rng_u01(seed, 2 * ctr);
     const double u2 = rng_u01(seed, 2 * ctr + 1);
     return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-}
-
-// Inside test of the mover's elemental kernel (apps/app_fempic/kernels.h:106-127), with the
-// FMA contraction the reference's GCC -O3 build applies: lc = c2*fnma(d3,z,fma(d2,y,fnma(d1,x,d0))).
-inline bool tet_contains(const double* det, double vol, double x, double y, double z) {
-    const double c2 = (1.0 / 6.0) / vol;
-    for (int i = 0; i < 4; i++) {
-        const double* d = det + 4 * i;
-        const double lc = c2 * std::fma(-d[3], z, std::fma(d[2], y, std::fma(-d[1], x, d[0])));
-        if (!(lc > -1e-10 && (lc - 1.0) < 1e-10)) return false;
-    }
-    return true;
 }
 
 // Numbering of a structured n0 x n1 x n2 grid: rank[i + n0*(j + n1*k)]. Order 0 is that linear
@@ -168,90 +141,12 @@ int oppx_box_mesh_build(int nx, int ny, int nz, double lx, double ly, double lz,
                         off[kKuhnPerm[t][s]] = 1;
                         con[s + 1] = (int)nid(ci + off[0], cj + off[1], ck + off[2]);
                     }
-                    // volume (1/6) det4 and the ABCD -> ADBC flip of minifempic_funcs.h:284-313
-                    double M[4][4];
-                    for (int v = 0; v < 4; v++) {
-                        M[v][0] = 1.0;
-                        for (int d = 0; d < 3; d++) M[v][d + 1] = node_pos[3 * (int64_t)con[v] + d];
-                    }
-                    double vol = (1.0 / 6.0) * det4(M);
-                    if (vol < 0) {
-                        std::swap(con[1], con[3]);
-                        vol = -vol;
-                    }
-                    c_vol[c] = vol;
+                    oppx_host::tet_geometry(node_pos, con, c_det + 16 * c, c_vol + c);
                     for (int v = 0; v < 4; v++) c2n[4 * c + v] = con[v];
-                    // 3x3 determinants per vertex (minifempic_funcs.h:316-380)
-                    for (int v = 0; v < 4; v++) {
-                        int v2, v3, v4;
-                        switch (v) {
-                            case 0: v2 = 1; v3 = 2; v4 = 3; break;
-                            case 1: v2 = 3; v3 = 2; v4 = 0; break;
-                            case 2: v2 = 3; v3 = 0; v4 = 1; break;
-                            default: v2 = 1; v3 = 0; v4 = 2; break;
-                        }
-                        const double* p2 = node_pos + 3 * (int64_t)con[v2];
-                        const double* p3 = node_pos + 3 * (int64_t)con[v3];
-                        const double* p4 = node_pos + 3 * (int64_t)con[v4];
-                        const double A[3][3] = {{p2[0], p2[1], p2[2]}, {p3[0], p3[1], p3[2]}, {p4[0], p4[1], p4[2]}};
-                        const double B[3][3] = {{1, p2[1], p2[2]}, {1, p3[1], p3[2]}, {1, p4[1], p4[2]}};
-                        const double G[3][3] = {{1, p2[0], p2[2]}, {1, p3[0], p3[2]}, {1, p4[0], p4[2]}};
-                        const double D[3][3] = {{1, p2[0], p2[1]}, {1, p3[0], p3[1]}, {1, p4[0], p4[1]}};
-                        c_det[(4 * c + v) * 4 + 0] = det3(A);
-                        c_det[(4 * c + v) * 4 + 1] = det3(B);
-                        c_det[(4 * c + v) * 4 + 2] = det3(G);
-                        c_det[(4 * c + v) * 4 + 3] = det3(D);
-                    }
                 }
             }
 
-    // Face neighbours: c2c[4c+v] = the tet sharing the face opposite local vertex v, -1 on
-    // the boundary (minifempic_funcs.h:388-443). Faces are bucketed by their smallest node.
-    std::vector<int64_t> start(nn + 1, 0);
-    for (int64_t c = 0; c < nc; c++)
-        for (int v = 0; v < 4; v++) {
-            int a = INT_MAX;
-            for (int w = 0; w < 4; w++)
-                if (w != v) a = std::min(a, c2n[4 * c + w]);
-            start[a + 1]++;
-        }
-    for (int64_t n = 0; n < nn; n++) start[n + 1] += start[n];
-    std::vector<int64_t> fill(start.begin(), start.end() - 1);
-    std::vector<int64_t> face_of(4 * nc);  // entries (c*4+v), grouped by smallest node
-    for (int64_t c = 0; c < nc; c++)
-        for (int v = 0; v < 4; v++) {
-            int a = INT_MAX;
-            for (int w = 0; w < 4; w++)
-                if (w != v) a = std::min(a, c2n[4 * c + w]);
-            face_of[fill[a]++] = 4 * c + v;
-        }
-    auto face_key = [&](int64_t fv, int out[3]) {
-        const int64_t c = fv / 4;
-        const int v = (int)(fv % 4);
-        int m = 0;
-        for (int w = 0; w < 4; w++)
-            if (w != v) out[m++] = c2n[4 * c + w];
-        std::sort(out, out + 3);
-    };
-#pragma omp parallel for schedule(dynamic, 4096)
-    for (int64_t n = 0; n < nn; n++) {
-        for (int64_t p = start[n]; p < start[n + 1]; p++) {
-            const int64_t f = face_of[p];
-            int kf[3];
-            face_key(f, kf);
-            int nb = -1;
-            for (int64_t q = start[n]; q < start[n + 1]; q++) {
-                if (q == p) continue;
-                int kg[3];
-                face_key(face_of[q], kg);
-                if (kg[0] == kf[0] && kg[1] == kf[1] && kg[2] == kf[2]) {
-                    nb = (int)(face_of[q] / 4);
-                    break;
-                }
-            }
-            c2c[f] = nb;
-        }
-    }
+    oppx_host::face_neighbours(nn, nc, c2n, c2c);
     return OPPX_OK;
 }
 
@@ -330,6 +225,37 @@ int oppx_box_particles_uniform(int64_t n_part, uint64_t seed, double sigma, int 
         p2c[p] = (int)(6 * cube + t);
     }
     return OPPX_OK;
+}
+
+int oppx_box12_particles(int64_t n_part, uint64_t seed, double sigma, int nx, int ny, int nz, double lx, double ly,
+                         double lz, const double* c_det, const double* c_vol, double* pos, double* vel, int* p2c) {
+    // Uniform positions in the box of a 12-tet box mesh (tets 12q .. 12q+11 fill cube q, cubes x fastest),
+    // N(0, sigma^2) velocity components (the streams of oppx_box_particles_uniform), and the containing
+    // tet: the first of the cube's 12 that the mover's inside test accepts.
+    if (n_part < 0 || !pos || !vel || !p2c || !c_det || !c_vol || nx <= 0 || ny <= 0 || nz <= 0) return OPPX_ERR_ARG;
+    const int n[3] = {nx, ny, nz};
+    const double L[3] = {lx, ly, lz};
+    int64_t lost = 0;
+#pragma omp parallel for schedule(static) reduction(+ : lost)
+    for (int64_t p = 0; p < n_part; p++) {
+        int ci[3];
+        for (int d = 0; d < 3; d++) {
+            const double u = rng_u01(seed, (uint64_t)p * 8 + d);
+            pos[3 * p + d] = L[d] * u;
+            int c = (int)(u * (double)n[d]);
+            ci[d] = c >= n[d] ? n[d] - 1 : c;
+            vel[3 * p + d] = sigma * rng_normal(seed ^ 0x5851F42D4C957F2DULL, (uint64_t)p * 4 + d);
+        }
+        const int64_t cube = ci[0] + (int64_t)nx * (ci[1] + (int64_t)ny * ci[2]);
+        int t = -1;
+        for (int k = 0; k < 12 && t < 0; k++) {
+            const int64_t c = 12 * cube + k;
+            if (tet_contains(c_det + 16 * c, c_vol[c], pos[3 * p], pos[3 * p + 1], pos[3 * p + 2])) t = (int)c;
+        }
+        if (t < 0) lost++;
+        p2c[p] = t < 0 ? (int)(12 * cube) : t;
+    }
+    return lost ? OPPX_ERR_STATE : OPPX_OK;
 }
 
 int oppx_dh_grid_dims(const double bmin[3], const double bmax[3], double gs, int64_t dims[3]) {
@@ -529,3 +455,59 @@ int oppx_dh_build_overlay_geom(int64_t n_cells, const double* c_det, const doubl
 
 
 }  // extern "C"
+
+namespace oppx_host {
+
+// Face neighbours: c2c[4c+v] = the tet sharing the face opposite local vertex v of tet c, -1 on the
+// boundary (apps/app_fempic/minifempic_funcs.h:388-443; in a conforming mesh an interior face belongs
+// to exactly two tets, so the answer does not depend on the search order). Faces are bucketed by
+// their smallest node.
+void face_neighbours(int64_t nn, int64_t nc, const int* c2n, int* c2c) {
+    std::vector<int64_t> start(nn + 1, 0);
+    for (int64_t c = 0; c < nc; c++)
+        for (int v = 0; v < 4; v++) {
+            int a = INT_MAX;
+            for (int w = 0; w < 4; w++)
+                if (w != v) a = std::min(a, c2n[4 * c + w]);
+            start[a + 1]++;
+        }
+    for (int64_t n = 0; n < nn; n++) start[n + 1] += start[n];
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    std::vector<int64_t> face_of(4 * nc);  // entries (c*4+v), grouped by smallest node
+    for (int64_t c = 0; c < nc; c++)
+        for (int v = 0; v < 4; v++) {
+            int a = INT_MAX;
+            for (int w = 0; w < 4; w++)
+                if (w != v) a = std::min(a, c2n[4 * c + w]);
+            face_of[fill[a]++] = 4 * c + v;
+        }
+    auto face_key = [&](int64_t fv, int out[3]) {
+        const int64_t c = fv / 4;
+        const int v = (int)(fv % 4);
+        int m = 0;
+        for (int w = 0; w < 4; w++)
+            if (w != v) out[m++] = c2n[4 * c + w];
+        std::sort(out, out + 3);
+    };
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t n = 0; n < nn; n++) {
+        for (int64_t p = start[n]; p < start[n + 1]; p++) {
+            const int64_t f = face_of[p];
+            int kf[3];
+            face_key(f, kf);
+            int nb = -1;
+            for (int64_t q = start[n]; q < start[n + 1]; q++) {
+                if (q == p) continue;
+                int kg[3];
+                face_key(face_of[q], kg);
+                if (kg[0] == kf[0] && kg[1] == kf[1] && kg[2] == kf[2]) {
+                    nb = (int)(face_of[q] / 4);
+                    break;
+                }
+            }
+            c2c[f] = nb;
+        }
+    }
+}
+
+}  // namespace oppx_host

@@ -55,7 +55,7 @@ class NesoParts(Structure):
 
 class Tuning(Structure):
     _fields_ = [("tile", c_int32), ("refill_min", c_int32), ("split", c_int32), ("persistent", c_int32),
-                ("sort_items", c_int32), ("inline_hops", c_int32), ("first_block", c_int32), ("reserved", c_int32)]
+                ("sort_items", c_int32), ("inline_hops", c_int32), ("first_block", c_int32), ("lanes", c_int32)]
 
 
 class BatchInst(Structure):
@@ -167,6 +167,11 @@ _SIGS = {
                                   c_int, c_void_p, c_int, c_void_p, c_size_t, c_void_p]),
     "oppx_set_tuning": (c_int, [POINTER(Tuning)]),
     "oppx_get_tuning": (c_int, [POINTER(Tuning)]),
+    "oppx_box12_write": (c_int, [c_int, c_int, c_int, c_double, ctypes.c_char_p]),
+    "oppx_box12_particles": (c_int, [c_int64, ctypes.c_uint64, c_double, c_int, c_int, c_int, c_double, c_double,
+                                     c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "oppx_mesh_file_sizes": (c_int, [ctypes.c_char_p, ctypes.c_char_p, c_void_p, c_void_p, c_void_p]),
+    "oppx_mesh_file_load": (c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p] + [c_void_p] * 13),
     "oppx_holefill_scratch_bytes": (c_size_t, [c_int64]),
     "oppx_holefill": (c_int, [c_int64, c_int64, c_void_p, c_void_p, POINTER(DatDesc), c_int, c_void_p,
                               c_size_t, c_void_p]),
@@ -238,6 +243,57 @@ def box_mesh(nx: int, ny: int, nz: int, lx=1.0, ly=1.0, lz=1.0, order="x") -> di
     return m
 
 
+def box12_write(nx: int, ny: int, nz: int, cube_length: float, directory: str):
+    """mesh.dat / inlet.dat / wall.dat of the 12-tet box mesh (cube_length in mm) in `directory`."""
+    check(lib().oppx_box12_write(nx, ny, nz, cube_length, directory.encode()), "box12_write")
+
+
+def box12_mesh(nx: int, ny: int, nz: int, lx=1.0, ly=1.0, lz=1.0) -> dict:
+    """The 12-tet box mesh of nx x ny x nz cubes on [0,lx]x[0,ly]x[0,lz] (m), written in the app's text
+    format (cube edges in mm; a cubic cell needs lx/nx == ly/ny == lz/nz) and read back with
+    mesh_from_files: the geometry the reference app's loader computes from those files."""
+    import tempfile
+    cube_mm = 1000.0 * lx / nx
+    if abs(1000.0 * ly / ny - cube_mm) > 1e-12 * cube_mm or abs(1000.0 * lz / nz - cube_mm) > 1e-12 * cube_mm:
+        raise ValueError("the 12-tet box mesh has cubic cells: lx/nx, ly/ny and lz/nz must agree")
+    with tempfile.TemporaryDirectory() as td:
+        box12_write(nx, ny, nz, cube_mm, td)
+        m = mesh_from_files(os.path.join(td, "mesh.dat"), os.path.join(td, "inlet.dat"), os.path.join(td, "wall.dat"))
+    m.update(nx=nx, ny=ny, nz=nz, lx=lx, ly=ly, lz=lz, split=12)
+    return m
+
+
+def box12_particles(n: int, seed: int, sigma: float, m: dict):
+    pos, vel = np.zeros((n, 3)), np.zeros((n, 3))
+    p2c = np.zeros(n, np.int32)
+    check(lib().oppx_box12_particles(n, seed, sigma, m["nx"], m["ny"], m["nz"], m["lx"], m["ly"], m["lz"],
+                                     _ptr(m["c_det"]), _ptr(m["c_vol"]), _ptr(pos), _ptr(vel), _ptr(p2c)),
+          "box12_particles")
+    return pos, vel, p2c
+
+
+def mesh_from_files(mesh_path: str, inlet_path: str | None = None, wall_path: str | None = None) -> dict:
+    """A FEM-PIC mesh read from the app's text files (the box_mesh dict layout, plus node volumes and
+    types and the inlet faces)."""
+    enc = lambda p: p.encode() if p else None
+    sizes = [c_int64(), c_int64(), c_int64()]
+    L = lib()
+    check(L.oppx_mesh_file_sizes(enc(mesh_path), enc(inlet_path), *[ctypes.byref(v) for v in sizes]), "mesh_file_sizes")
+    nn, nc, nf = (int(v.value) for v in sizes)
+    m = dict(n_nodes=nn, n_cells=nc, n_ifaces=nf, node_pos=np.zeros((nn, 3)), node_type=np.zeros(nn, np.int32),
+             node_vol=np.zeros(nn), c2n=np.zeros((nc, 4), np.int32), c2c=np.zeros((nc, 4), np.int32),
+             c_det=np.zeros((nc, 16)), c_vol=np.zeros(nc), if2c=np.zeros(max(nf, 1), np.int32),
+             if2n=np.zeros((max(nf, 1), 3), np.int32), if_u=np.zeros((max(nf, 1), 3)), if_v=np.zeros((max(nf, 1), 3)),
+             if_normal=np.zeros((max(nf, 1), 3)), if_area=np.zeros(max(nf, 1)))
+    keys = ["node_pos", "node_type", "node_vol", "c2n", "c2c", "c_det", "c_vol", "if2c", "if2n", "if_u", "if_v",
+            "if_normal", "if_area"]
+    check(L.oppx_mesh_file_load(enc(mesh_path), enc(inlet_path), enc(wall_path), *[_ptr(m[k]) for k in keys]),
+          "mesh_file_load")
+    for k in keys[7:]:
+        m[k] = m[k][:nf]
+    return m
+
+
 def box_field_sines(m: dict) -> np.ndarray:
     ef = np.zeros((m["n_cells"], 3))
     check(lib().oppx_box_field_sines(m["n_nodes"], m["n_cells"], _ptr(m["node_pos"]), _ptr(m["c2n"]),
@@ -291,12 +347,13 @@ def neso_particles(n: int, seed: int, m: dict, vmax: float):
     return pos, vel, p2c
 
 
-TUNING_FIELDS = ("tile", "refill_min", "split", "persistent", "sort_items", "inline_hops", "first_block")
+TUNING_FIELDS = ("tile", "refill_min", "split", "persistent", "sort_items", "inline_hops", "first_block", "lanes")
 
 
-def set_tuning(tile=256, refill_min=16, split=0, persistent=0, sort_items=16, inline_hops=64, first_block=256):
+def set_tuning(tile=256, refill_min=16, split=0, persistent=0, sort_items=16, inline_hops=64, first_block=256,
+               lanes=1):
     """Process-wide launch shape of the walk and the sort (results do not depend on it)."""
-    t = Tuning(tile, refill_min, split, persistent, sort_items, inline_hops, first_block)
+    t = Tuning(tile, refill_min, split, persistent, sort_items, inline_hops, first_block, lanes)
     check(lib().oppx_set_tuning(ctypes.byref(t)), "set_tuning")
 
 

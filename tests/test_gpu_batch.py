@@ -63,8 +63,8 @@ def alone_step(g, d, deposit):
 
 
 @pytest.fixture(params=[dict(), dict(inline_hops=0, persistent=1), dict(inline_hops=0, tile=64, refill_min=4, sort_items=4),
-                        dict(inline_hops=1, sort_items=8)],
-                ids=["default", "tile-persistent", "tile-small", "two-pass-1"])
+                        dict(inline_hops=1, sort_items=8), dict(lanes=4), dict(lanes=4, inline_hops=2)],
+                ids=["default", "tile-persistent", "tile-small", "two-pass-1", "quad", "quad-two-pass-2"])
 def tuning(request):
     saved = oppx.get_tuning()
     oppx.set_tuning(**dict(saved, **request.param))

@@ -29,7 +29,11 @@ def load(name):
 
 def case(g):
     n = int(g["meta_nx"])
-    m = oppx.box_mesh(n, n, n)
+    if str(g["meta_case"]).startswith("mesh12"):  # 12-tet box read through the mesh-file loader
+        nz = int(g["meta_nz"])
+        m = oppx.box12_mesh(n, int(g["meta_ny"]), nz, 1.0, 1.0, nz / n)
+    else:
+        m = oppx.box_mesh(n, n, n)
     if "in_pos" in g:
         ef, pos, vel, p2c = g["in_c_ef"], g["in_pos"], g["in_vel"], g["in_p2c"]
     else:
@@ -46,9 +50,21 @@ def charge_close(a, b):
     return np.abs(a - b).max() <= DEPOSIT_RTOL * scale
 
 
-@pytest.mark.parametrize("name", ["fempic_tiny_seq.npz", "fempic_small_dh_omp1.npz"])
-@pytest.mark.parametrize("mode", ["unfused", "fused"])
+@pytest.mark.parametrize("name", ["fempic_tiny_seq.npz", "fempic_small_dh_omp1.npz", "fempic_mesh12_dh.npz"])
+@pytest.mark.parametrize("mode", ["unfused", "fused", "fused-quad"])
 def test_gpu_matches_reference_fixture(name, mode):
+    if mode == "fused-quad":
+        saved = oppx.get_tuning()
+        oppx.set_tuning(**dict(saved, lanes=4))
+        try:
+            _fixture_run(name, "fused")
+        finally:
+            oppx.set_tuning(**saved)
+    else:
+        _fixture_run(name, mode)
+
+
+def _fixture_run(name, mode):
     g = load(name)
     m, ef, pos, vel, p2c, kw = case(g)
     gpu = oppx.FempicDevice(m, ef, pos, vel, p2c, capacity=pos.shape[0] + 77, **kw)
@@ -78,15 +94,25 @@ def test_gpu_matches_reference_fixture(name, mode):
     assert charge_close(out["charge"], g["out_final_charge"])
 
 
-@pytest.mark.parametrize("tile,refill,split,persistent", [(64, 64, 0, 0), (128, 8, 0, 0), (512, 32, 0, 0),
-                                                          (256, 16, 1, 0), (512, 1, 1, 0), (256, 16, 0, 1),
-                                                          (64, 4, 1, 1)])
-def test_gpu_walk_launch_shapes_bit_exact(tile, refill, split, persistent):
-    # every launch shape of the cooperative walk gives the same cells, hops, permutation and lc
+SHAPES = [
+    # cooperative tile walk (inline_hops = 0): tile sizes, refill thresholds, split relocation, persistent grid
+    dict(inline_hops=0, tile=64, refill_min=64), dict(inline_hops=0, tile=128, refill_min=8),
+    dict(inline_hops=0, tile=512, refill_min=32), dict(inline_hops=0, split=1),
+    dict(inline_hops=0, tile=512, refill_min=1, split=1), dict(inline_hops=0, persistent=1),
+    dict(inline_hops=0, tile=64, refill_min=4, split=1, persistent=1),
+    # two-pass move: evaluations in the first pass before the worklist walk, block size, quad lanes
+    dict(inline_hops=64), dict(inline_hops=2), dict(inline_hops=1, first_block=1024), dict(lanes=4),
+    dict(lanes=4, inline_hops=1), dict(lanes=4, inline_hops=3),
+]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda d: ",".join(f"{k}={v}" for k, v in d.items()))
+def test_gpu_walk_launch_shapes_bit_exact(shape):
+    # every launch shape of the walk gives the same cells, hops, permutation and lc
     g = load("fempic_small_dh_omp1.npz")
     m, ef, pos, vel, p2c, kw = case(g)
     saved = oppx.get_tuning()
-    oppx.set_tuning(tile, refill, split=split, persistent=persistent)
+    oppx.set_tuning(**dict(saved, **shape))
     try:
         gpu = oppx.FempicDevice(m, ef, pos, vel, p2c, **kw)
         q, ms, dt = float(g["meta_charge"]), float(g["meta_mass"]), float(g["meta_dt"])
@@ -100,6 +126,7 @@ def test_gpu_walk_launch_shapes_bit_exact(tile, refill, split, persistent):
             gpu.sync_size()
         out = gpu.download()
         assert np.array_equal(out["p2c"], g["out_final_p2c"]) and np.array_equal(out["lc"].ravel(), g["out_final_lc"])
+        assert np.array_equal(out["pos"].ravel(), g["out_final_pos"])
         assert charge_close(out["charge"], g["out_final_charge"])
     finally:
         oppx.set_tuning(**saved)

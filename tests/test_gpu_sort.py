@@ -186,3 +186,27 @@ def test_gpu_lazy_sort_steps_equal_eager_sort_and_oracle(every):
             assert np.array_equal(a[key], b[key]), f"step {s} {key}"
             assert np.array_equal(a[key], e[key]), f"step {s} {key} vs oracle"
         assert np.abs(a["charge"] - e["charge"]).max() <= 1e-12 * np.abs(e["charge"]).max()
+
+
+def test_gpu_f32_deposit_reports_unsorted_p2c():
+    """The FP32 deposit's precondition (p2c sorted, removed particles last) is checked: sorted input
+    adds no fault; each run start that follows a larger cell or a removed particle adds one."""
+    m = oppx.box_mesh(3, 3, 3)
+    ef = oppx.box_field_sines(m)
+    pos, vel, p2c = oppx.box_particles(3000, 5, 0.01, m)
+    pos, vel, p2c = oppx.sort_particles_by_cell(pos, vel, p2c)
+    gpu = oppx.FempicDevice(m, ef, pos, vel, p2c)
+    gpu.deposit_sorted_f32()
+    assert gpu.read_counters()["faults"] == 0
+    # three descents planted in the input: a swap of two runs' neighbours, and a removed particle
+    # followed by a valid cell
+    q = p2c.copy()
+    cuts = np.flatnonzero(np.diff(q) > 0)[[3, 20]] + 1  # first index of two runs
+    for c in cuts:
+        q[c - 1], q[c] = q[c], q[c - 1]  # the larger cell now comes first: one descent each
+    q[100] = oppx.MAX_CELL_INDEX
+    expect = int(np.sum((q[1:] < q[:-1]) & (q[1:] != oppx.MAX_CELL_INDEX)))
+    g2 = oppx.FempicDevice(m, ef, pos, vel, np.where(q == oppx.MAX_CELL_INDEX, 0, q).astype(np.int32))
+    g2.p2c[: len(q)] = torch.from_numpy(q.astype(np.int32)).to("cuda")
+    g2.deposit_sorted_f32()
+    assert expect >= 3 and g2.read_counters()["faults"] == expect

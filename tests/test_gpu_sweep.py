@@ -25,8 +25,16 @@ CASES = [
 ]
 
 
+@pytest.fixture(params=[1, 4], ids=["lane-per-particle", "quad-per-particle"])
+def lanes(request):
+    saved = oppx.get_tuning()
+    oppx.set_tuning(**dict(saved, lanes=request.param))
+    yield request.param
+    oppx.set_tuning(**saved)
+
+
 @pytest.mark.parametrize("nx,order,n,sigma,ov_ratio,fill,seed", CASES)
-def test_gpu_fused_step_sweep_matches_oracle(nx, order, n, sigma, ov_ratio, fill, seed):
+def test_gpu_fused_step_sweep_matches_oracle(nx, order, n, sigma, ov_ratio, fill, seed, lanes):
     m = oppx.box_mesh(nx, nx, nx, order=order)
     ef = oppx.box_field_sines(m)
     pos, vel, p2c = oppx.box_particles(n, seed, sigma, m)

@@ -20,6 +20,9 @@ def load(name):
 
 def mesh_of(g):
     n = int(g["meta_nx"])
+    if str(g["meta_case"]).startswith("mesh12"):  # 12-tet box read through the mesh-file loader
+        nz = int(g["meta_nz"])
+        return oppx.box12_mesh(n, int(g["meta_ny"]), nz, 1.0, 1.0, nz / n)
     return oppx.box_mesh(n, n, n)
 
 
@@ -56,7 +59,7 @@ def test_inputs_are_host_independent():
 
 
 @pytest.mark.parametrize("name,threads", [("fempic_tiny_seq.npz", 1), ("fempic_small_dh_omp1.npz", 1),
-                                          ("fempic_small_dh_omp4.npz", 4)])
+                                          ("fempic_small_dh_omp4.npz", 4), ("fempic_mesh12_dh.npz", 1)])
 def test_oracle_matches_reference_bit_exact(name, threads):
     g = load(name)
     m = mesh_of(g)
