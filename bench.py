@@ -175,17 +175,24 @@ def cpu_baseline_reference(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, 
 
 
 def load_traffic(workload_tag, kernel="fempic_first_kernel"):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter summary
-    (profiles/<kernel>_<workload>.md holds a line `traffic_bytes_per_launch: <B>`), or None."""
+    """Measured HBM traffic of the dominant kernel from the committed rocprofv3 counter summary
+    profiles/<kernel>_<workload>.md, which holds `traffic_bytes_per_launch: <B>` (scaled FETCH_SIZE +
+    WRITE_SIZE averaged over the profiled run's launches) and `algorithmic_bytes_per_launch: <B>` (this
+    script's algorithmic figure averaged over the same launches). Returns both, or None."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload_tag}.md")
+    got = {}
     try:
         with open(path) as f:
             for line in f:
-                if line.strip().startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                for key in ("traffic_bytes_per_launch", "algorithmic_bytes_per_launch"):
+                    if line.strip().startswith(key + ":"):
+                        got[key] = float(line.split(":", 1)[1])
     except OSError:
-        pass
-    return None
+        return None
+    if len(got) != 2:
+        return None
+    got["profile"] = os.path.relpath(path, ROOT)
+    return got
 
 
 def copy_peak_gbs(torch, gib=1):
@@ -233,6 +240,29 @@ def f32_deposit_report(torch, gpu):
     }
 
 
+def fp32_vs_oracle(m, final):
+    """Error of the GPU's FP32 compensated deposit of the final state against the oracle's FP64
+    particle-order deposit (apps_codegen/app_fempic_cg/seq/deposit_charge_on_nodes_kernel_loop.hpp:21-60)
+    of the same particles: max |rho32 - rho64| / max |rho64|, and the largest per-node relative error
+    over nodes with |rho64| > 1e-3 max."""
+    import oracle
+
+    keep = (final["p2c"] >= 0) & (final["p2c"] < int(m["n_cells"]))  # removed particles deposit nothing
+    p2c = np.ascontiguousarray(final["p2c"][keep], np.int32)
+    lc = np.ascontiguousarray(final["lc"][keep], np.float64)
+    c2n = np.ascontiguousarray(m["c2n"], np.int32)
+    rho64 = np.zeros(int(m["n_nodes"]))
+    oracle.lib().orc_fempic_deposit(len(p2c), len(rho64), oracle._p(c2n), oracle._p(p2c), oracle._p(lc),
+                                    oracle._p(rho64), 1)
+    rho32 = final["rho32"]
+    scale = max(np.abs(rho64).max(), 1e-300)
+    big = np.abs(rho64) > 1e-3 * scale
+    return {"err_vs_fp64_oracle": float(np.abs(rho32 - rho64).max() / scale),
+            "max_rel_err_vs_fp64_oracle_nodes_above_1e-3_max":
+                float((np.abs(rho32 - rho64)[big] / np.abs(rho64[big])).max()) if big.any() else 0.0,
+            "oracle_particles": int(len(p2c))}
+
+
 def bench_ensemble(args):
     """Batched ensemble: N independent tet-box instances (tets log-uniform 2k-60k, particles log-uniform
     10k-2M, sigma uniform [0.005, 0.02], seed 4) stepped 20 times as one concatenated set: batched walk,
@@ -254,7 +284,7 @@ def bench_ensemble(args):
 
     b = oppx.FempicBatch(inst)
     snap = autotune.StateSnapshot(b)
-    best, table = autotune.autotune(lambda: b.step_batch("f32"), snap, autotune.SHORT_CANDIDATES, steps=2,
+    best, table = autotune.autotune(lambda: b.step_batch("f32"), snap, autotune.DEFAULT_CANDIDATES, steps=2,
                                     use_cache=False, log=log)
     log(f"autotuned launch shape {best}")
 
@@ -363,8 +393,9 @@ def bench_ensemble(args):
         "graph_equals_eager": bool(same),
         "phase_ms_median": {"move": float(np.median(ph[:, 0])), "sort": float(np.median(ph[:, 1])),
                             "fp32_deposit": float(np.median(ph[:, 2]))},
-        "fp32_deposit_err_vs_fp64": f32_err,
+        "fp32_deposit_err_vs_gpu_fp64_atomic": f32_err,
         "autotune_table": [[c, round(ms, 4)] for c, ms in table],
+        "autotune_ms_fastest_slowest": [min(ms for _, ms in table), max(ms for _, ms in table)],
         "roofline": {"kernel": "fempic_first_kernel (batched move phase, eager steps)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None},
@@ -717,6 +748,16 @@ def cabana_cpu_reference(c2c6, interp, pos, vel, weight, p2c, deck, steps=10):
             "sample": f"{what}, {steps} steps of the first {n} particles over the full mesh, {loop_s:.2f}s of loops"}
 
 
+LATENCY_KEYS = tuple(f"p{q}_{w}_ms" for w in ("latency", "queue", "service") for q in (50, 99))
+
+
+def offered_load(reqs, seconds, st):
+    """Particle steps the stream asks for per second of arrivals against the rate the server delivered:
+    a ratio above 1 means requests arrive faster than they can be served and queueing sets the latency."""
+    asked = float(sum(r.tpl.n_part * r.steps for r in reqs)) / seconds
+    return {"particle_steps_per_s": asked, "over_served_rate": asked / max(st["value"], 1e-300)}
+
+
 def bench_serve(args):
     """Serving stream: Poisson arrivals of 10-step segments, sizes as the ensemble's plus 5% large
     requests, FP32 compensated deposit, admission by an HBM budget; continuous batching against the
@@ -769,13 +810,14 @@ def bench_serve(args):
         "config": {"workload": f"Poisson {args.serve_rate}/s for {args.serve_seconds}s, {len(reqs)} requests of 10 "
                                f"steps, {len(small)} small + {len(large)} large instance templates, budget "
                                f"{args.serve_budget_gb} GB"},
-        "p50_latency_ms": st["p50_latency_ms"],
-        "p99_latency_ms": st["p99_latency_ms"],
+        # latency = queueing (arrival to admission) + service (admission to results on the host)
+        **{k: st[k] for k in LATENCY_KEYS},
+        "offered_load": offered_load(reqs, args.serve_seconds, st),
+        "requests": len(reqs),
         "iterations": st["iterations"],
         "peak_resident_gb": st["peak_resident_bytes"] / 1e9,
         "plain_value": sp["value"],
-        "plain_p50_latency_ms": sp["p50_latency_ms"],
-        "plain_p99_latency_ms": sp["p99_latency_ms"],
+        **{f"plain_{k}": sp[k] for k in LATENCY_KEYS},
         "host_s": {k: st[k] for k in ("admit_s", "step_s", "retire_s")},
         "speedup_vs_plain": st["value"] / sp["value"],
         # the stream's kernels are the ensemble's (batched move, sort, FP32 deposit) at varying batch
@@ -832,6 +874,8 @@ def main():
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--particle-scale", type=float, default=1.0)
     ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--no-batched", dest="with_batched", action="store_false",
+                    help="single: leave out the ensemble and serving summaries of the default line")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -852,7 +896,49 @@ def main():
             raise SystemExit("--ensemble-steps must be even (the captured steps end on the starting buffers)")
         print(json.dumps(bench_ensemble(args)), flush=True)
         return
+    out = bench_single(args)
+    if args.with_batched:
+        out.update(batched_summaries(args))
+    print(json.dumps(out), flush=True)
 
+
+ENSEMBLE_KEYS = ("value", "plain_value", "speedup_vs_plain", "ms_per_step", "p50_step_ms", "p99_step_ms",
+                 "graph_equals_eager", "fp32_deposit_err_vs_gpu_fp64_atomic", "phase_ms_median", "autotune_ms_fastest_slowest",
+                 "config")
+SERVE_KEYS = ("value", "plain_value", "speedup_vs_plain", "requests", "iterations", "peak_resident_gb",
+              "offered_load", "config") + LATENCY_KEYS
+
+
+def batched_summaries(args):
+    """The batched ensemble and the serving stream at their default configurations, run after the single
+    instance's state is freed, without their CPU baselines (--workload ensemble / serve give the full
+    lines): particles/s of the batched path and of the plain per-instance launch path on the same work,
+    their ratio, and the step and request latencies."""
+    import copy
+
+    import torch
+
+    torch.cuda.empty_cache()
+    sub = copy.copy(args)
+    sub.no_cpu = True
+    res = {}
+    for name, fn, keys in (("ensemble", bench_ensemble, ENSEMBLE_KEYS), ("serving", bench_serve, SERVE_KEYS)):
+        t0 = time.time()
+        try:
+            r = fn(sub)
+            res[name] = {k: r[k] for k in keys if k in r}
+            res[name]["metric"] = r["metric"]
+        except Exception as e:  # reported in the line, never silently dropped
+            res[name] = {"error": f"{type(e).__name__}: {e}"}
+        log(f"{name} summary in {time.time() - t0:.1f}s: {res[name]}")
+        torch.cuda.empty_cache()
+    return res
+
+
+def bench_single(args):
+    """The large single instance: the 100^3-cube Morton-numbered 6 M-tet box, 50 M particles in cell
+    order, 129^3 overlay; every step is push + direct hop + walk + FP64 atomic deposit, then hole fill
+    or (every --sort-every steps) a Sort_Periodic sort whose record gather runs inside the next pass."""
     import torch
 
     import oppx
@@ -932,14 +1018,24 @@ def main():
             f"({alg[s] / (pass_ms[s] * 1e-3) / 1e9:.0f} GB/s)")
 
     f32 = None
+    final = None
     try:
         f32 = f32_deposit_report(torch, gpu)
         log(f"fp32 compensated deposit: {f32}")
+        if not args.no_cpu:
+            fin = gpu.download()
+            final = dict(p2c=fin["p2c"], lc=fin["lc"], rho32=gpu.charge_f32.cpu().numpy().astype(np.float64))
     except Exception as e:  # reported, never required
         log(f"fp32 deposit report unavailable: {e}")
 
-    cpu = cpu_port = None
+    cpu = cpu_port = cpu_ref_error = None
     if not args.no_cpu:
+        if final is not None and f32 is not None:
+            try:  # the checker: the oracle's FP64 particle-order deposit of the same final state
+                f32.update(fp32_vs_oracle(m, final))
+                log(f"fp32 compensated deposit against the FP64 oracle: {f32['err_vs_fp64_oracle']:.3e}")
+            except Exception as e:
+                log(f"fp32 deposit check against the oracle unavailable: {e}")
         try:
             del gpu
             torch.cuda.empty_cache()
@@ -951,11 +1047,23 @@ def main():
             cpu = cpu_baseline_reference(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt, steps=16, stride=2)
             if cpu:
                 log(f"cpu baseline (reference omp build) {cpu['value']:.3e} particles/s ({cpu['cores']} threads)")
+            else:
+                cpu_ref_error = "the reference build oracle/_ref/fempic_ref_omp is missing or did not run"
         except Exception as e:
-            log(f"reference cpu baseline unavailable: {e}")
-        if cpu is None:
+            cpu_ref_error = f"{type(e).__name__}: {e}"
+        if cpu_ref_error:
+            log(f"reference cpu baseline unavailable: {cpu_ref_error}; cpu_baseline is the oracle's (kind port)")
             cpu, cpu_port = cpu_port, None
 
+    # HBM traffic per launch: the profiled run's measured traffic over its algorithmic bytes, applied to
+    # this run's algorithmic bytes per launch (the two agree when this is the profiled command)
+    prof = load_traffic(args.tag)
+    traffic = traffic_ratio = traffic_src = None
+    if prof is not None:
+        traffic_ratio = prof["traffic_bytes_per_launch"] / prof["algorithmic_bytes_per_launch"]
+        traffic = traffic_ratio * float(alg.mean())
+        traffic_src = (f"{prof['profile']}: {prof['traffic_bytes_per_launch']:.4g} B measured over "
+                       f"{prof['algorithmic_bytes_per_launch']:.4g} algorithmic B per launch")
     fill = "hole-fill" if args.fill == "holefill" or args.sort_every <= 0 else f"hole-fill, sort by cell every {args.sort_every} steps"
     out = {
         "metric": METRIC,
@@ -989,16 +1097,20 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": load_traffic(args.tag),
+            "traffic": traffic,
             "avg_kernel_ms": float(pass_ms.mean()),
             "bytes_per_launch": float(alg.mean()),
+            "traffic_over_algorithmic": traffic_ratio,
+            "traffic_source": traffic_src,
             "measured_copy_gbs": copy_gbs,
         },
         "deposit_fp32_compensated": f32,
         "cpu_baseline": cpu,
         "cpu_port_baseline": cpu_port,
     }
-    print(json.dumps(out), flush=True)
+    if cpu_ref_error:
+        out["cpu_baseline_reference_error"] = cpu_ref_error
+    return out
 
 
 if __name__ == "__main__":

@@ -1,10 +1,16 @@
-"""Autotuner of the walk / sort launch shape (oppx_tuning) for a resident workload.
+"""Autotuner of the move / sort launch shape (oppx_tuning) for a resident workload.
 
-The walk's tile size and refill threshold, the split (coalesced push + relocation pass before the
-walk), the persistent grid and the radix sort's block size change only how the work is scheduled;
-the results are bit-identical for every choice (tests/test_gpu_fempic.py, tests/test_gpu_batch.py,
+Which knobs matter depends on the move path the shape selects:
+  * the two-pass move (inline_hops >= 1, the default): the first pass runs `lanes` lanes per particle
+    (1, or a quad of 4), `first_block` threads per block for one lane per particle (256 or 1024; the quad
+    pass has its own fixed block), and hands a particle to the worklist walk after `inline_hops`
+    evaluations; tile, refill_min, split and persistent are not read;
+  * the cooperative tile walk (inline_hops = 0): `tile`, `refill_min`, `split` and `persistent` apply;
+  * every path: `sort_items`, the keys per thread of the radix sort passes.
+The results are bit-identical for every choice (tests/test_gpu_fempic.py, tests/test_gpu_batch.py,
 tests/test_gpu_sort.py). The tuner times each candidate on the caller's own step, restoring the
-device state before every trial, keeps the fastest, and caches the choice per workload signature.
+device state before every trial, takes the best of `repeats` timings per candidate, keeps the fastest,
+and caches the choice per workload signature.
 """
 
 from __future__ import annotations
@@ -15,20 +21,30 @@ from typing import Callable, Iterable
 
 import oppx
 
-DEFAULT_CANDIDATES = [
-    dict(tile=t, refill_min=r, split=s, persistent=p, sort_items=si)
-    for (t, r) in [(128, 16), (256, 16), (256, 32), (512, 32)]
-    for s in (0,)
-    for p in (0, 1)
-    for si in (8, 16)
-]
+_BASE = dict(tile=256, refill_min=16, split=0, persistent=0)
 
+
+def _shape(**kw):
+    return dict(_BASE, **kw)
+
+
+DEFAULT_CANDIDATES = (
+    [_shape(lanes=1, first_block=b, inline_hops=h, sort_items=si)
+     for b in (256, 1024) for h in (2, 4, 64) for si in (8, 16)]
+    + [_shape(lanes=4, first_block=256, inline_hops=h, sort_items=si) for h in (2, 4, 64) for si in (8, 16)]
+    + [dict(tile=t, refill_min=r, split=sp, persistent=p, lanes=1, first_block=256, inline_hops=0, sort_items=16)
+       for (t, r) in ((256, 16), (512, 32)) for sp in (0, 1) for p in (0, 1)]
+)
+
+# the knobs that move time most on the ensemble, one candidate each way
 SHORT_CANDIDATES = [
-    dict(tile=256, refill_min=16, split=0, persistent=0, sort_items=16),
-    dict(tile=256, refill_min=16, split=0, persistent=1, sort_items=16),
-    dict(tile=128, refill_min=16, split=0, persistent=1, sort_items=8),
-    dict(tile=512, refill_min=32, split=0, persistent=0, sort_items=16),
-    dict(tile=256, refill_min=32, split=1, persistent=0, sort_items=16),
+    _shape(lanes=1, first_block=256, inline_hops=64, sort_items=16),
+    _shape(lanes=1, first_block=1024, inline_hops=64, sort_items=16),
+    _shape(lanes=1, first_block=256, inline_hops=2, sort_items=16),
+    _shape(lanes=1, first_block=256, inline_hops=64, sort_items=8),
+    _shape(lanes=4, first_block=256, inline_hops=64, sort_items=16),
+    _shape(lanes=4, first_block=256, inline_hops=4, sort_items=16),
+    dict(tile=256, refill_min=16, split=0, persistent=1, lanes=1, first_block=256, inline_hops=0, sort_items=16),
 ]
 
 
@@ -76,8 +92,9 @@ class StateSnapshot:
 
 
 def autotune(step: Callable[[], None], snapshot: StateSnapshot, candidates: Iterable[dict] | None = None,
-             steps: int = 2, signature: str | None = None, use_cache: bool = True, log=None):
-    """Time `steps` calls of `step` per candidate tuning from the same state; set and return the best.
+             steps: int = 2, signature: str | None = None, use_cache: bool = True, log=None, repeats: int = 2):
+    """Time `steps` calls of `step` per candidate tuning from the same state, best of `repeats`; set and
+    return the fastest.
 
     Returns (best_tuning, [(tuning, ms_per_step), ...]). The state is restored after the search."""
     import torch
@@ -95,15 +112,17 @@ def autotune(step: Callable[[], None], snapshot: StateSnapshot, candidates: Iter
             oppx.set_tuning(**cand)
             snapshot.restore()
             step()  # first launch of a shape: occupancy query, code load
-            snapshot.restore()
-            torch.cuda.synchronize()
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record()
-            for _ in range(steps):
-                step()
-            b.record()
-            torch.cuda.synchronize()
-            ms = a.elapsed_time(b) / steps
+            ms = float("inf")
+            for _ in range(max(1, repeats)):
+                snapshot.restore()
+                torch.cuda.synchronize()
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                for _ in range(steps):
+                    step()
+                b.record()
+                torch.cuda.synchronize()
+                ms = min(ms, a.elapsed_time(b) / steps)
             table.append((cand, ms))
             if log:
                 log(f"autotune {cand}: {ms:.3f} ms/step")

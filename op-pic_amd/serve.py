@@ -372,6 +372,18 @@ class Request:
     result: dict = field(default_factory=dict)
 
 
+def latency_split(requests) -> dict:
+    """p50/p99 in ms of a request's whole latency (arrival to results on the host) and of its two
+    parts: queueing (arrival to admission) and service (admission to results on the host)."""
+    out = {}
+    for name, f in (("latency", lambda r: r.t_done - r.arrival), ("queue", lambda r: r.t_admit - r.arrival),
+                    ("service", lambda r: r.t_done - r.t_admit)):
+        v = np.array([f(r) for r in requests]) * 1e3
+        out[f"p50_{name}_ms"] = float(np.percentile(v, 50)) if len(v) else 0.0
+        out[f"p99_{name}_ms"] = float(np.percentile(v, 99)) if len(v) else 0.0
+    return out
+
+
 def serve(arena: FempicArena, requests, budget_bytes: float, keep_results: bool = False, log=None):
     """Continuous batching over `requests` (sorted by arrival, seconds from start). Returns
     (requests, stats). Admission is FIFO: a request is admitted once it has arrived and fits."""
@@ -435,10 +447,8 @@ def serve(arena: FempicArena, requests, budget_bytes: float, keep_results: bool 
     if prev is not None:
         finish(prev)
     wall = clock() - t0
-    lat = np.array([r.t_done - r.arrival for r in done]) * 1e3
     return done, dict(wall_s=wall, iterations=iters, particle_steps=part_steps, value=part_steps / wall,
-                      p50_latency_ms=float(np.percentile(lat, 50)), p99_latency_ms=float(np.percentile(lat, 99)),
-                      peak_resident_bytes=peak_resident, requests=len(done), **spent)
+                      **latency_split(done), peak_resident_bytes=peak_resident, requests=len(done), **spent)
 
 
 def serve_plain(requests, keep_results: bool = False):
@@ -451,6 +461,7 @@ def serve_plain(requests, keep_results: bool = False):
     for r in requests:
         while clock() - t0 < r.arrival:
             time.sleep(max(0.0, min(r.arrival - (clock() - t0), 0.01)))
+        r.t_admit = clock() - t0  # this path's admission: the request's turn on the device
         d = r.tpl.inst
         g = oppx.FempicDevice(d["mesh"], d["ef"], d["pos"], d["vel"], d["p2c"], overlay=d.get("overlay"),
                               bmin=d.get("bmin"), bmax=d.get("bmax"), gs=d.get("gs"))
@@ -468,7 +479,5 @@ def serve_plain(requests, keep_results: bool = False):
         r.t_done = clock() - t0
         del g
     wall = clock() - t0
-    lat = np.array([r.t_done - r.arrival for r in requests]) * 1e3
-    return requests, dict(wall_s=wall, particle_steps=part_steps, value=part_steps / wall,
-                          p50_latency_ms=float(np.percentile(lat, 50)), p99_latency_ms=float(np.percentile(lat, 99)),
+    return requests, dict(wall_s=wall, particle_steps=part_steps, value=part_steps / wall, **latency_split(requests),
                           requests=len(requests))
