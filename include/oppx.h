@@ -478,6 +478,17 @@ size_t oppx_holefill_scratch_bytes(int64_t n_max);
 int oppx_holefill(int64_t n_max, int64_t stride, oppx_move_counters* counters, const uint64_t* hole_bits,
                   const oppx_dat_desc* dats, int n_dats, void* scratch, size_t scratch_bytes, void* stream);
 
+/* No reference counterpart: oppx_holefill for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Same arguments and results as
+ * oppx_holefill without the workspace: the scratch is a static device array of
+ * OPPX_HOLEFILL_NOALLOC_SCRATCH_BYTES bytes that every GPU the library runs on holds until the process
+ * ends; n_max above OPPX_HOLEFILL_NOALLOC_MAX_PARTICLES is rejected (OPPX_ERR_ARG). Allocates nothing.
+ * All calls on one GPU share its array: calls that overlap on one GPU (two streams, two threads) race. */
+#define OPPX_HOLEFILL_NOALLOC_MAX_PARTICLES 134217728
+#define OPPX_HOLEFILL_NOALLOC_SCRATCH_BYTES 8454144
+int oppx_holefill_noalloc(int64_t n_max, int64_t stride, oppx_move_counters* counters, const uint64_t* hole_bits,
+                          const oppx_dat_desc* dats, int n_dats, void* stream);
+
 /* FP32 compensated deposit over cell-sorted particles (p2c ascending, e.g. after oppx_sort_by_cell):
  * per-cell Kahan sums of lc in index order into cell_partials [4*n_cells] (scratch, zeroed here), then per
  * node a Kahan sum of its (cell, vertex) partials in node_ref order: entries [node_start[v], node_end[v])

@@ -10,7 +10,14 @@
 // so every live tail record finds its destination hole by a binary search over wp and a
 // select inside one word. Passes: block popcounts -> scan -> word prefix -> tail copy -> finalize.
 
+#include <mutex>
+
 #include "oppx_internal.h"
+
+// Scratch of oppx_holefill_noalloc: a file-scope device array, one copy per GPU, held until the process
+// ends. In host code this name stands for the array but is no device address: kernels receive the
+// address hipGetSymbolAddress returns for the current device (oppx::noalloc_scratch).
+static __device__ __align__(256) unsigned char scratch[OPPX_HOLEFILL_NOALLOC_SCRATCH_BYTES];
 
 namespace oppx {
 namespace {
@@ -151,6 +158,15 @@ struct Scratch {
     int64_t nb;
 };
 
+// bsum (block counts, padded to 64 words), wp (one prefix per bitmap word) and 64 spare words
+constexpr size_t scratch_need(int64_t n_max) {
+    const int64_t W = (n_max + 63) / 64;
+    const int64_t nb = (W + kB - 1) / kB < 1 ? 1 : (W + kB - 1) / kB;
+    return (size_t)(((nb + 63) / 64) * 64 + W + 64) * sizeof(uint32_t);
+}
+static_assert(scratch_need(OPPX_HOLEFILL_NOALLOC_MAX_PARTICLES) <= OPPX_HOLEFILL_NOALLOC_SCRATCH_BYTES,
+              "the static scratch must hold the hole fill of the largest set oppx_holefill_noalloc accepts");
+
 Scratch carve(void* scratch, int64_t n_max) {
     const int64_t W = (n_max + 63) / 64;
     const int64_t nb = (W + kB - 1) / kB;
@@ -161,30 +177,8 @@ Scratch carve(void* scratch, int64_t n_max) {
     return s;
 }
 
-}  // namespace
-}  // namespace oppx
-
-using namespace oppx;
-
-extern "C" {
-
-// a move covers whole tiles: at most ceil(n_max/64) words plus one partial block of the largest tile
-int64_t oppx_hole_bits_words(int64_t n_max) { return (n_max > 0 ? (n_max + 63) / 64 : 0) + 32; }
-
-size_t oppx_holefill_scratch_bytes(int64_t n_max) {
-    if (n_max < 0) return 0;
-    const int64_t W = (n_max + 63) / 64;
-    int64_t nb = (W + kB - 1) / kB;
-    if (nb < 1) nb = 1;
-    return (size_t)(((nb + 63) / 64) * 64 + W + 64) * sizeof(uint32_t);
-}
-
-int oppx_holefill(int64_t n_max, int64_t stride, oppx_move_counters* counters, const uint64_t* hole_bits,
-                          const oppx_dat_desc* dats, int n_dats, void* scratch, size_t scratch_bytes, void* stream) {
-    OPPX_ARG_CHECK(counters && hole_bits && scratch, "oppx_holefill: null pointer");
-    OPPX_ARG_CHECK(n_max >= 0 && stride >= n_max, "oppx_holefill: need 0 <= n_max <= stride");
-    OPPX_ARG_CHECK(n_dats >= 1 && n_dats <= kMaxDats && dats, "oppx_holefill: 1..16 dats required");
-    OPPX_ARG_CHECK(scratch_bytes >= oppx_holefill_scratch_bytes(n_max), "oppx_holefill: scratch too small");
+int holefill_impl(int64_t n_max, int64_t stride, oppx_move_counters* counters, const uint64_t* hole_bits,
+                  const oppx_dat_desc* dats, int n_dats, void* work, hipStream_t s) {
     DatTable T{};
     T.n = n_dats;
     T.stride = stride;
@@ -195,8 +189,8 @@ int oppx_holefill(int64_t n_max, int64_t stride, oppx_move_counters* counters, c
         T.dim[q] = dats[q].dim;
         T.eb[q] = dats[q].elem_bytes;
     }
-    hipStream_t s = as_stream(stream);
-    Scratch sc = carve(scratch, n_max);
+    // every scratch word a pass reads was written by an earlier pass of the same call: nothing to clear
+    Scratch sc = carve(work, n_max);
     hf_block_sums<<<(unsigned)sc.nb, kB, 0, s>>>(hole_bits, counters, sc.bsum, n_max);
     OPPX_LAUNCH_CHECK("hf_block_sums");
     hf_scan_blocks<<<1, kB, 0, s>>>(sc.bsum, sc.nb, counters);
@@ -208,6 +202,63 @@ int oppx_holefill(int64_t n_max, int64_t stride, oppx_move_counters* counters, c
     hf_finalize<<<1, 1, 0, s>>>(hole_bits, counters, sc.wp, n_max);
     OPPX_LAUNCH_CHECK("hf_finalize");
     return OPPX_OK;
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices];  // address of `scratch` on each device, once looked up
+
+}  // namespace
+
+// Device address of the static scratch array on `device`, which must be current: looked up with
+// hipGetSymbolAddress on the first call for the device, then kept.
+int noalloc_scratch(int device, void** out) {
+    OPPX_ARG_CHECK(device >= 0 && device < kMaxDevices, "oppx: device index beyond the static scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[device] == nullptr) {
+        void* p = nullptr;
+        OPPX_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        OPPX_ARG_CHECK(p != nullptr, "oppx: hipGetSymbolAddress returned no address for the static scratch");
+        g_scratch_dev[device] = p;
+    }
+    if (out) *out = g_scratch_dev[device];
+    return OPPX_OK;
+}
+
+}  // namespace oppx
+
+using namespace oppx;
+
+extern "C" {
+
+// a move covers whole tiles: at most ceil(n_max/64) words plus one partial block of the largest tile
+int64_t oppx_hole_bits_words(int64_t n_max) { return (n_max > 0 ? (n_max + 63) / 64 : 0) + 32; }
+
+size_t oppx_holefill_scratch_bytes(int64_t n_max) { return n_max < 0 ? 0 : scratch_need(n_max); }
+
+int oppx_holefill(int64_t n_max, int64_t stride, oppx_move_counters* counters, const uint64_t* hole_bits,
+                  const oppx_dat_desc* dats, int n_dats, void* work, size_t work_bytes, void* stream) {
+    OPPX_ARG_CHECK(counters && hole_bits && work, "oppx_holefill: null pointer");
+    OPPX_ARG_CHECK(n_max >= 0 && stride >= n_max, "oppx_holefill: need 0 <= n_max <= stride");
+    OPPX_ARG_CHECK(n_dats >= 1 && n_dats <= kMaxDats && dats, "oppx_holefill: 1..16 dats required");
+    OPPX_ARG_CHECK(work_bytes >= oppx_holefill_scratch_bytes(n_max), "oppx_holefill: scratch too small");
+    return holefill_impl(n_max, stride, counters, hole_bits, dats, n_dats, work, as_stream(stream));
+}
+
+int oppx_holefill_noalloc(int64_t n_max, int64_t stride, oppx_move_counters* counters, const uint64_t* hole_bits,
+                          const oppx_dat_desc* dats, int n_dats, void* stream) {
+    OPPX_ARG_CHECK(counters && hole_bits, "oppx_holefill_noalloc: null pointer");
+    OPPX_ARG_CHECK(n_max >= 0 && stride >= n_max, "oppx_holefill_noalloc: need 0 <= n_max <= stride");
+    OPPX_ARG_CHECK(n_max <= OPPX_HOLEFILL_NOALLOC_MAX_PARTICLES,
+                   "oppx_holefill_noalloc: n_max exceeds OPPX_HOLEFILL_NOALLOC_MAX_PARTICLES (the static scratch "
+                   "holds the hole fill of at most 134217728 particles); use oppx_holefill with a workspace");
+    OPPX_ARG_CHECK(n_dats >= 1 && n_dats <= kMaxDats && dats, "oppx_holefill_noalloc: 1..16 dats required");
+    int dev = 0;
+    OPPX_HIP_TRY(hipGetDevice(&dev));
+    void* work = nullptr;
+    const int rc = noalloc_scratch(dev, &work);  // looked up by oppx_init, or here on a device's first call
+    if (rc != OPPX_OK) return rc;
+    return holefill_impl(n_max, stride, counters, hole_bits, dats, n_dats, work, as_stream(stream));
 }
 
 }  // extern "C"

@@ -15,6 +15,10 @@ void set_error(const std::string& msg);
 // Process-wide launch shape (oppx_set_tuning), defined in oppx_runtime.hip.
 extern oppx_tuning g_tuning;
 
+// Device address (*out, nullable) of the static scratch of the allocation-free entry point on `device`,
+// which must be current; looked up once per device. Defined in oppx_compact.hip, called by oppx_init.
+int noalloc_scratch(int device, void** out);
+
 // Blocks of `kernel` (with `block` threads) resident on the whole device; the occupancy query is
 // cached per (kernel, block).
 unsigned resident_blocks(const void* kernel, int block);

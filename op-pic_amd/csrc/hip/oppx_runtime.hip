@@ -107,7 +107,7 @@ int oppx_init(int device) {
     OPPX_ARG_CHECK(count > 0, "oppx_init: no HIP device visible");
     OPPX_ARG_CHECK(device >= 0 && device < count, "oppx_init: device index out of range");
     OPPX_HIP_TRY(hipSetDevice(device));
-    return OPPX_OK;
+    return noalloc_scratch(device, nullptr);
 }
 
 int oppx_device_malloc(void** ptr, size_t bytes) {

@@ -175,6 +175,7 @@ _SIGS = {
     "oppx_holefill_scratch_bytes": (c_size_t, [c_int64]),
     "oppx_holefill": (c_int, [c_int64, c_int64, c_void_p, c_void_p, POINTER(DatDesc), c_int, c_void_p,
                               c_size_t, c_void_p]),
+    "oppx_holefill_noalloc": (c_int, [c_int64, c_int64, c_void_p, c_void_p, POINTER(DatDesc), c_int, c_void_p]),
 }
 
 _lib = None
