@@ -245,6 +245,7 @@ def fp32_vs_oracle(m, final):
     particle-order deposit (apps_codegen/app_fempic_cg/seq/deposit_charge_on_nodes_kernel_loop.hpp:21-60)
     of the same particles: max |rho32 - rho64| / max |rho64|, and the largest per-node relative error
     over nodes with |rho64| > 1e-3 max."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))  # oracle/oracle.py, not the oracle/ directory
     import oracle
 
     keep = (final["p2c"] >= 0) & (final["p2c"] < int(m["n_cells"]))  # removed particles deposit nothing
@@ -874,6 +875,8 @@ def main():
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--particle-scale", type=float, default=1.0)
     ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--step-log", default="",
+                    help="single: write every step's live count and algorithmic bytes (JSON) to this file")
     ap.add_argument("--no-batched", dest="with_batched", action="store_false",
                     help="single: leave out the ensemble and serving summaries of the default line")
     args = ap.parse_args()
@@ -1009,6 +1012,10 @@ def bench_single(args):
     pass_ms = np.array([evs[s][1].elapsed_time(evs[s][2]) for s in range(K)])
     fill_ms = np.array([evs[s][2].elapsed_time(evs[s][3]) for s in range(K)])
     alg = np.array([algorithmic_bytes(int(n), nc, nn) for n in timed_sizes], dtype=np.float64)
+    if args.step_log:  # every step's live count and algorithmic bytes, warm-up included (one pass each)
+        with open(args.step_log, "w") as f:
+            json.dump({"live": [int(n) for n in sizes[:W + K]],
+                       "algorithmic_bytes": [algorithmic_bytes(int(n), nc, nn) for n in sizes[:W + K]]}, f)
     achieved = float(alg.sum() / (pass_ms.sum() * 1e-3) / 1e9)
     avg_hops = float(hops[W:].sum() / max(timed_sizes.sum(), 1))
     log(f"timed {K} steps in {wall:.3f}s; live {timed_sizes[0]} -> {sizes[W + K]}; pass avg {pass_ms.mean():.3f} ms;"
