@@ -81,6 +81,7 @@ class StateSnapshot:
 
     def __init__(self, dev):
         self.dev = dev
+        dev.materialize()  # no pending permutation or staged records: the SoA dats are the state
         self.saved = [t.clone() for t in (dev.pos, dev.vel, dev.lc, dev.p2c, dev.counters)]
         self.n_max = dev.n_max
 
@@ -89,6 +90,7 @@ class StateSnapshot:
         for dst, src in zip((d.pos, d.vel, d.lc, d.p2c, d.counters), self.saved):
             dst.copy_(src)
         d.n_max = self.n_max
+        d.pending = d.staged = False
 
 
 def autotune(step: Callable[[], None], snapshot: StateSnapshot, candidates: Iterable[dict] | None = None,

@@ -44,6 +44,11 @@ struct StepArgs {
     const double* __restrict__ src_vel;
     const double* __restrict__ src_lc;
     const int32_t* __restrict__ src_p2c;
+    // 48-byte position/velocity records {x, y, z, vx, vy, vz}: src_rec (gather mode) replaces
+    // src_pos/src_vel as the input, read at perm[i]; rec_out receives the pass's new position and
+    // velocity of particle i at 6i instead of pos/vel (a step before a lazy sort)
+    const double* __restrict__ src_rec;
+    double* __restrict__ rec_out;
     double k, dt;
     int64_t n_max;
     int tile;        // particles per wave tile (multiple of 64, <= kMaxTile)
@@ -79,6 +84,15 @@ template <typename T>
 __device__ __forceinline__ T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
 template <typename T>
 __device__ __forceinline__ void st_stream(T* p, T v) { __builtin_nontemporal_store(v, p); }
+// a 48-byte position/velocity record {x, y, z, vx, vy, vz} as three 16-byte stores; ordinary (cached)
+// stores, so that L2 merges a wave's three partial-line writes into whole lines before they leave
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_record(double* rec, double x, double y, double z, double vx, double vy, double vz) {
+    f64x2* r = reinterpret_cast<f64x2*>(rec);
+    r[0] = f64x2{x, y};
+    r[1] = f64x2{z, vx};
+    r[2] = f64x2{vy, vz};
+}
 
 // Per-block statistics go to one of kSlots slots (blockIdx modulo kSlots) instead of the single
 // counters record: thousands of blocks adding to one address serialise in one L2 channel. The
@@ -394,11 +408,14 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
     int h = 0;
     double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
     __shared__ DepositTable s_table;
-    if (DEPOSIT) table_clear(s_table);
 
-    if (live && !skip && !fault) {
+    // push: the particle's new position and velocity (or, for a particle this pass does not push, the
+    // ones it carries over)
+    const bool go = live && !skip && !fault;
+    double x = 0.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+    DhView D = P.dh;
+    if (go) {
         double kk = P.k, dt = P.dt;
-        DhView D = P.dh;
         if (BATCH) {  // the instance of the particle's cell supplies k, dt and its overlay
             const int in = P.cell_inst[cell];
             if ((unsigned)in < (unsigned)P.n_insts) {
@@ -414,22 +431,82 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
             }
         }
         if (!fault) {
-            double x = ld(spos + src), y = ld(spos + S + src), z = ld(spos + 2 * S + src);
+            if (GATHER && P.src_rec) {  // one 48-byte record instead of six SoA words at permuted addresses
+                const double2* r = reinterpret_cast<const double2*>(P.src_rec + 6 * src);
+                const double2 r0 = r[0], r1 = r[1], r2 = r[2];
+                x = r0.x; y = r0.y; z = r1.x; vx = r1.y; vy = r2.x; vz = r2.y;
+            } else {
+                x = ld(spos + src); y = ld(spos + S + src); z = ld(spos + 2 * S + src);
+                if (PUSH) { vx = ld(svel + src); vy = ld(svel + S + src); vz = ld(svel + 2 * S + src); }
+            }
             if (PUSH) {
                 // calculate_new_pos_vel_kernel: v = fma(k, E, v); x = fma(v, dt, x)
                 const double* e = P.ef + 3 * (int64_t)cell;
                 const double ex = e[0], ey = e[1], ez = e[2];
-                double vx = ld(svel + src), vy = ld(svel + S + src), vz = ld(svel + 2 * S + src);
                 vx = fma(kk, ex, vx);
                 x = fma(vx, dt, x);
                 vy = fma(kk, ey, vy);
                 y = fma(vy, dt, y);
                 vz = fma(kk, ez, vz);
                 z = fma(vz, dt, z);
-                st_stream(P.vel + i, vx); st_stream(P.vel + S + i, vy); st_stream(P.vel + 2 * S + i, vz);
-                st_stream(P.pos + i, x); st_stream(P.pos + S + i, y); st_stream(P.pos + 2 * S + i, z);
+                if (!P.rec_out) {
+                    st_stream(P.vel + i, vx); st_stream(P.vel + S + i, vy); st_stream(P.vel + 2 * S + i, vz);
+                    st_stream(P.pos + i, x); st_stream(P.pos + S + i, y); st_stream(P.pos + 2 * S + i, z);
+                }
                 pushed = true;
             }
+        }
+    }
+    if ((GATHER || P.rec_out) && live && !pushed) {
+        // every field of the output slot is written: what the pass did not produce is carried over
+        double pv[6];
+        if (GATHER && P.src_rec) {
+            for (int d = 0; d < 6; d++) pv[d] = P.src_rec[6 * src + d];
+        } else {
+            for (int d = 0; d < 3; d++) {
+                pv[d] = spos[d * S + src];
+                pv[3 + d] = svel[d * S + src];
+            }
+        }
+        if (P.rec_out) {
+            x = pv[0]; y = pv[1]; z = pv[2]; vx = pv[3]; vy = pv[4]; vz = pv[5];
+        } else {
+            for (int d = 0; d < 3; d++) {
+                st_stream(P.pos + d * S + i, pv[d]);
+                st_stream(P.vel + d * S + i, pv[3 + d]);
+            }
+        }
+    }
+    // records out (block-uniform): every live particle's {x, y, z, vx, vy, vz}. With 256-thread blocks the
+    // wave's 64 records pass through LDS (the deposit table's storage, not yet in use) so that each of
+    // the three 16-byte stores of a lane lands next to its neighbours' (1 KB contiguous per instruction)
+    if (P.rec_out) {
+        if constexpr (BS * 48 <= (int)sizeof(DepositTable)) {
+            const int wave = threadIdx.x / kWave;
+            f64x2* slice = reinterpret_cast<f64x2*>(&s_table) + wave * (3 * kWave);
+            if (live) {
+                slice[3 * lane] = f64x2{x, y};
+                slice[3 * lane + 1] = f64x2{z, vx};
+                slice[3 * lane + 2] = f64x2{vy, vz};
+            }
+            __syncthreads();
+            const int64_t base = (int64_t)b * BS + wave * kWave;  // first particle of the wave
+            const int64_t nw = n - base;                          // live particles of the wave
+            f64x2* dst = reinterpret_cast<f64x2*>(P.rec_out + 6 * base);
+            for (int k = 0; k < 3; k++) {
+                const int q = k * kWave + lane;  // pair q of the wave's 192: record q / 3
+                if (q / 3 < nw) __builtin_nontemporal_store(slice[q], dst + q);
+            }
+            __syncthreads();
+        } else if (live) {
+            st_record(P.rec_out + 6 * i, x, y, z, vx, vy, vz);
+        }
+    }
+    if (DEPOSIT) table_clear(s_table);
+
+    // direct hop and walk
+    if (go) {
+        if (!fault) {
             if (DH && (!BATCH || D.cell != nullptr)) {
                 cell = dh_lookup(D, x, y, z);
                 if (cell == kMaxCell) removed = true;
@@ -456,14 +533,7 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
         if (P.hops) st_stream(P.hops + i, h);
     }
     if (GATHER && live) {
-        // every field of the output slot is written: what the pass did not produce is carried over
         // (a pending particle's lc is final only after the worklist walk, which overwrites this)
-        if (!pushed) {
-            for (int d = 0; d < 3; d++) {
-                st_stream(P.pos + d * S + i, spos[d * S + src]);
-                st_stream(P.vel + d * S + i, svel[d * S + src]);
-            }
-        }
         if (!p2c_done && !p2c_in_place) st_stream(P.p2c + i, sp2c[src]);
         if (!lc_done)
             for (int d = 0; d < 4; d++) st_stream(P.lc + d * S + i, P.src_lc[d * S + src]);
@@ -620,15 +690,21 @@ __global__ void __launch_bounds__(kQuadBlock) fempic_quad_kernel(StepArgs P, uin
         if (!fault) {
             // lane d < 3 owns component d: x_d (and, pushing, v_d = fma(k, E_d, v_d); x_d = fma(v_d, dt, x_d))
             const int d = r < 3 ? r : 2;
-            double xd = ld(spos + d * S + src);
+            const bool from_rec = GATHER && P.src_rec;
+            double xd = from_rec ? P.src_rec[6 * src + d] : ld(spos + d * S + src);
             if (PUSH) {
                 const double e = P.ef[3 * (int64_t)cell + d];
-                double vd = ld(svel + d * S + src);
+                double vd = from_rec ? P.src_rec[6 * src + 3 + d] : ld(svel + d * S + src);
                 vd = fma(kk, e, vd);
                 xd = fma(vd, dt, xd);
                 if (r < 3) {
-                    st_stream(P.vel + d * S + i, vd);
-                    st_stream(P.pos + d * S + i, xd);
+                    if (P.rec_out) {
+                        st_stream(P.rec_out + 6 * i + d, xd);
+                        st_stream(P.rec_out + 6 * i + 3 + d, vd);
+                    } else {
+                        st_stream(P.vel + d * S + i, vd);
+                        st_stream(P.pos + d * S + i, xd);
+                    }
                 }
                 pushed = true;
             }
@@ -657,11 +733,19 @@ __global__ void __launch_bounds__(kQuadBlock) fempic_quad_kernel(StepArgs P, uin
         }
         if (P.hops && r == 0) st_stream(P.hops + i, h);
     }
-    if (GATHER && live) {
-        if (!pushed && r < 3) {
-            st_stream(P.pos + r * S + i, spos[r * S + src]);
-            st_stream(P.vel + r * S + i, svel[r * S + src]);
+    if ((GATHER || P.rec_out) && live && !pushed && r < 3) {  // carry over what the pass did not push
+        const bool from_rec = GATHER && P.src_rec;
+        const double xr = from_rec ? P.src_rec[6 * src + r] : spos[r * S + src];
+        const double vr = from_rec ? P.src_rec[6 * src + 3 + r] : svel[r * S + src];
+        if (P.rec_out) {
+            st_stream(P.rec_out + 6 * i + r, xr);
+            st_stream(P.rec_out + 6 * i + 3 + r, vr);
+        } else {
+            st_stream(P.pos + r * S + i, xr);
+            st_stream(P.vel + r * S + i, vr);
         }
+    }
+    if (GATHER && live) {
         if (!p2c_done && !p2c_in_place && r == 0) st_stream(P.p2c + i, sp2c[src]);
         if (!lc_done) st_stream(P.lc + r * S + i, P.src_lc[r * S + src]);
         if (P.hops && r == 0 && !(live && !skip && !fault)) P.hops[i] = h;
@@ -720,7 +804,10 @@ __global__ void __launch_bounds__(kBlock) fempic_walk_kernel(StepArgs P, WorkLis
         const int64_t i = w.x;
         int h = w.y;
         int cell = P.p2c[i];
-        const double x = P.pos[i], y = P.pos[S + i], z = P.pos[2 * S + i];
+        const bool rec = P.rec_out != nullptr;  // the first pass left the pushed position in a record
+        const double x = rec ? P.rec_out[6 * i] : P.pos[i];
+        const double y = rec ? P.rec_out[6 * i + 1] : P.pos[S + i];
+        const double z = rec ? P.rec_out[6 * i + 2] : P.pos[2 * S + i];
         double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
         bool removed = false, fault = !((unsigned)cell < (unsigned)P.n_cells);
         while (!fault) {
@@ -1017,6 +1104,27 @@ __global__ void tet_aux_kernel(const double* __restrict__ vol, const int32_t* __
     aux[c] = a;
 }
 
+// pos/vel SoA from 48-byte records: slot i takes record perm[i] (perm null: record i)
+__global__ void __launch_bounds__(kBlock) gather_records_kernel(int64_t n_max, int64_t stride,
+                                                                const oppx_move_counters* __restrict__ ctr,
+                                                                const int32_t* __restrict__ perm,
+                                                                const double* __restrict__ rec, double* __restrict__ pos,
+                                                                double* __restrict__ vel, int32_t* __restrict__ faults) {
+    int64_t n = ctr->size;
+    n = n < 0 ? 0 : (n > n_max ? n_max : n);
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const int64_t src = perm ? (int64_t)perm[i] : i;
+        if (!((uint64_t)src < (uint64_t)stride)) {  // a corrupt permutation entry: report, touch nothing
+            atomicAdd(faults, 1);
+            continue;
+        }
+        const f64x2* r = reinterpret_cast<const f64x2*>(rec + 6 * src);
+        const f64x2 a = r[0], b = r[1], c = r[2];
+        st_stream(pos + i, a.x); st_stream(pos + stride + i, a.y); st_stream(pos + 2 * stride + i, b.x);
+        st_stream(vel + i, b.y); st_stream(vel + stride + i, c.x); st_stream(vel + 2 * stride + i, c.y);
+    }
+}
+
 int fill_args(StepArgs& P, const oppx_fempic_parts* parts, const oppx_tet_mesh* mesh, const oppx_dh_grid* dh,
               oppx_move_counters* ctr) {
     OPPX_ARG_CHECK(parts && mesh && ctr, "fempic: null descriptor");
@@ -1287,6 +1395,74 @@ int oppx_fempic_step_fused_gather(int64_t n_max, const int32_t* perm, const oppx
     }
     fold_slots(P, s);
     OPPX_LAUNCH_CHECK("fempic_step_fused_gather");
+    return OPPX_OK;
+}
+
+int oppx_fempic_step_fused_rec(int64_t n_max, const int32_t* perm, const double* src_rec, const double* src_lc,
+                               const oppx_fempic_parts* parts, double* rec_out, const oppx_tet_mesh* mesh,
+                               const oppx_dh_grid* dh, double k, double dt, oppx_move_counters* counters,
+                               uint64_t* hole_bits, void* work, size_t work_bytes, int32_t* hops, double* node_charge,
+                               void* stream) {
+    StepArgs P;
+    int rc = fill_args(P, parts, mesh, dh, counters);
+    if (rc) return rc;
+    OPPX_ARG_CHECK(perm ? (src_rec && src_lc) : (!src_rec && !src_lc),
+                   "oppx_fempic_step_fused_rec: src_rec and src_lc go with perm (all three or none)");
+    OPPX_ARG_CHECK(perm || rec_out, "oppx_fempic_step_fused_rec: neither records in nor records out");
+    OPPX_ARG_CHECK(parts->lc && parts->p2c && (rec_out || (parts->pos && parts->vel)) && (perm || (parts->pos && parts->vel)),
+                   "oppx_fempic_step_fused_rec: particle arrays missing");
+    OPPX_ARG_CHECK(!rec_out || rec_out != src_rec, "oppx_fempic_step_fused_rec: rec_out must not be src_rec");
+    OPPX_ARG_CHECK(!perm || src_lc != parts->lc, "oppx_fempic_step_fused_rec: src_lc and parts->lc must be separate");
+    OPPX_ARG_CHECK(n_max >= 0 && n_max <= parts->stride, "oppx_fempic_step_fused_rec: n_max beyond capacity");
+    OPPX_ARG_CHECK(hole_bits != nullptr, "oppx_fempic_step_fused_rec: hole_bits required");
+    OPPX_ARG_CHECK(g_tuning.inline_hops >= 1 && work && work_bytes >= work_need(n_max),
+                   "oppx_fempic_step_fused_rec: needs the two-pass move (tuning.inline_hops >= 1 and its workspace)");
+    hipStream_t s = as_stream(stream);
+    if (n_max == 0) return OPPX_OK;
+    P.n_max = n_max;
+    P.k = k;
+    P.dt = dt;
+    P.hole_bits = hole_bits;
+    P.hops = hops;
+    P.charge = node_charge;
+    P.rec_out = rec_out;
+    attach_slots(P, work, work_bytes, s);
+    if (perm) {
+        P.perm = perm;
+        P.src_rec = src_rec;
+        P.src_lc = src_lc;
+        P.src_pos = nullptr;  // never read: records replace them
+        P.src_vel = nullptr;
+        P.src_p2c = nullptr;  // the sort wrote the sorted p2c into parts->p2c: read in place
+        if (dh) {
+            if (node_charge) launch_two_pass<true, true, true, false, true>(P, n_max, work, work_bytes, s);
+            else launch_two_pass<true, true, false, false, true>(P, n_max, work, work_bytes, s);
+        } else {
+            if (node_charge) launch_two_pass<true, false, true, false, true>(P, n_max, work, work_bytes, s);
+            else launch_two_pass<true, false, false, false, true>(P, n_max, work, work_bytes, s);
+        }
+    } else if (dh) {
+        if (node_charge) launch_two_pass<true, true, true, false>(P, n_max, work, work_bytes, s);
+        else launch_two_pass<true, true, false, false>(P, n_max, work, work_bytes, s);
+    } else {
+        if (node_charge) launch_two_pass<true, false, true, false>(P, n_max, work, work_bytes, s);
+        else launch_two_pass<true, false, false, false>(P, n_max, work, work_bytes, s);
+    }
+    fold_slots(P, s);
+    OPPX_LAUNCH_CHECK("fempic_step_fused_rec");
+    return OPPX_OK;
+}
+
+int oppx_gather_records(int64_t n_max, int64_t stride, oppx_move_counters* counters, const int32_t* perm,
+                        const double* rec, double* pos, double* vel, void* stream) {
+    OPPX_ARG_CHECK(counters && rec && pos && vel, "oppx_gather_records: null array");
+    OPPX_ARG_CHECK(n_max >= 0 && stride >= n_max, "oppx_gather_records: need 0 <= n_max <= stride");
+    if (n_max == 0) return OPPX_OK;
+    unsigned g = grid_for(n_max, kBlock);
+    if (g > 8192) g = 8192;
+    gather_records_kernel<<<g, kBlock, 0, as_stream(stream)>>>(n_max, stride, counters, perm, rec, pos, vel,
+                                                              &counters->faults);
+    OPPX_LAUNCH_CHECK("gather_records");
     return OPPX_OK;
 }
 

@@ -3,9 +3,11 @@
 * The large single instance exactly as bench.py runs it: the 100^3-cube Morton-numbered 6 M-tet mesh,
   the 129^3 overlay, the 50 M-particle seed-2 input in cell order, and the finalize alternating hole fill
   with a Sort_Periodic sort every 2 steps whose record gather runs inside the next step's pass
-  (fempic_first_kernel with GATHER). A 2 M-particle slice of that input (every 25th particle, still in
-  cell order) runs 6 steps beside the oracle: cells, hop totals and maxima, positions, velocities and
-  lc bit-exact after every step that is not left pending, FP64 deposit within 1e-12 relative.
+  (fempic_first_kernel with GATHER), the pass before each sort leaving positions and velocities as
+  48-byte records that the pass after it reads through the permutation. A 2 M-particle slice of that
+  input (every 25th particle, still in cell order) runs 6 steps beside the oracle: cells, hop totals
+  and maxima, positions, velocities and lc bit-exact after every step that is not left pending, FP64
+  deposit within 1e-12 relative.
 * The 256-instance ensemble of workloads.ensemble_specs(256) (tets 2k-60k, the particle counts scaled
   down for time): every instance's batched result equals the instance run alone, bit for bit, and 8
   instances equal the oracle, FP32 compensated deposit within 1e-6 of the oracle's FP64 charge.
@@ -71,10 +73,11 @@ def _bench_pipeline(m, ef, pos, vel, p2c, ov, bmin, bmax, gs, q, ms, dt):
     sort_every = 2
     for s in range(6):
         print(f"[configs] bench pipeline step {s}", flush=True)
-        gpu.step_begin()
-        gpu.pass_fused(q, ms, dt)  # reads through the pending permutation after a sort step
-        c = gpu.read_counters()
         sorted_step = (s + 1) % sort_every == 0
+        gpu.step_begin()
+        # reads through the pending permutation after a sort step; before one, leaves 48-byte records
+        gpu.pass_fused(q, ms, dt, stage=sorted_step)
+        c = gpu.read_counters()
         if sorted_step:
             gpu.sort(drop_removed=True, lazy=True)  # applied by the next step's pass
             r = orc.step_sorted(q, ms, dt, threads=THREADS)
