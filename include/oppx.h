@@ -304,6 +304,28 @@ int oppx_fempic_step_fused_gather(int64_t n_max, const int32_t* perm, const oppx
                                   double k, double dt, oppx_move_counters* counters, uint64_t* hole_bits, void* work,
                                   size_t work_bytes, int32_t* hops, double* node_charge, void* stream);
 
+/* The fused step with each particle's position and velocity as one 48-byte record {x, y, z, vx, vy, vz}
+ * on either side (no reference counterpart: a layout for the step before a lazy sort and the step
+ * after it). rec_out (nullable): particle i's new position and velocity go to rec_out[6i .. 6i+5]
+ * instead of parts->pos/vel (left as they were); every particle in [0, counters->size) gets a record.
+ * perm (nullable): a pending sort permutation -- particle i then reads its position and velocity at
+ * src_rec[6 perm[i] ..] (one 48-byte read in place of six 8-byte SoA reads at permuted addresses), its
+ * lc at src_lc[d*stride + perm[i]] and its p2c in place (parts->p2c, written sorted by the sort), and
+ * writes every field at i of parts (pos/vel, or rec_out when given). perm NULL: src_rec and src_lc NULL
+ * too, and the step reads parts in place. Per-particle results equal oppx_fempic_step_fused /
+ * oppx_fempic_step_fused_gather. Needs the two-pass move (tuning.inline_hops >= 1 and its workspace). */
+int oppx_fempic_step_fused_rec(int64_t n_max, const int32_t* perm, const double* src_rec, const double* src_lc,
+                               const oppx_fempic_parts* parts, double* rec_out, const oppx_tet_mesh* mesh,
+                               const oppx_dh_grid* dh, double k, double dt, oppx_move_counters* counters,
+                               uint64_t* hole_bits, void* work, size_t work_bytes, int32_t* hops, double* node_charge,
+                               void* stream);
+
+/* pos[d*stride + i] = rec[6 perm[i] + d], vel[d*stride + i] = rec[6 perm[i] + 3 + d] for i < counters->size
+ * (perm NULL: record i): materialises the records oppx_fempic_step_fused_rec left, e.g. before the state is
+ * read back, hole-filled or sorted eagerly. A perm entry outside [0, stride) is counted in counters->faults. */
+int oppx_gather_records(int64_t n_max, int64_t stride, oppx_move_counters* counters, const int32_t* perm,
+                        const double* rec, double* pos, double* vel, void* stream);
+
 /* Batched fused step over an ensemble of independent instances concatenated as described at
  * oppx_batch_inst: one launch walks every instance's particles (no reference counterpart: OP-PIC
  * runs one instance per process). cell_inst (device [mesh->n_cells]) is the instance of every cell,
@@ -534,6 +556,56 @@ int oppx_gather_dats(int64_t n_max, int64_t stride, const oppx_move_counters* co
  * batched set). */
 int oppx_lower_bound(int64_t n_max, const oppx_move_counters* counters, const int32_t* p2c, const int32_t* keys,
                      int64_t k, int64_t* out, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Batched ensembles and continuous-batching serving for C++ hosts (no reference counterpart:
+ * OP-PIC runs one instance per process). Host code over the calls above (op-pic_amd/csrc/host/arena.cpp).
+ * ------------------------------------------------------------------------------------- */
+
+/* One FEM-PIC instance as host arrays, indices local to the instance (read during oppx_arena_admit only). */
+typedef struct oppx_instance {
+    int64_t n_cells, n_nodes, n_part;
+    const double* c_det;       /* [16*n_cells] */
+    const double* c_vol;       /* [n_cells] */
+    const int32_t* c2c;        /* [4*n_cells], -1 = boundary face */
+    const int32_t* c2n;        /* [4*n_cells] */
+    const double* c_ef;        /* [3*n_cells] */
+    const double* pos;         /* [3*n_part], x y z per particle */
+    const double* vel;         /* [3*n_part] */
+    const int32_t* p2c;        /* [n_part] */
+    const int32_t* overlay;    /* nullable: direct-hop overlay [dh_dims[0]*dh_dims[1]*dh_dims[2]] */
+    int64_t dh_dims[3];
+    double dh_min[3];
+    double dh_inv_gs;
+    double charge, mass, dt;   /* push coefficient k = charge / mass * dt */
+} oppx_instance;
+
+/* A resident arena of instances stepped together (opaque). Capacities: concatenated cells, nodes,
+ * particles and overlay words, and instance slots. Device memory is allocated here, once. */
+typedef struct oppx_arena oppx_arena;
+int oppx_arena_create(int64_t max_cells, int64_t max_nodes, int64_t max_particles, int64_t max_overlay,
+                      int32_t max_insts, void* stream, oppx_arena** out);
+int oppx_arena_destroy(oppx_arena* arena);
+
+/* Admit an instance between steps: its mesh and particles are copied into free ranges (indices
+ * rebased) and it joins the next step. *slot identifies it. OPPX_ERR_NOMEM: no room at the moment
+ * (nothing changed; retry after a retirement). OPPX_ERR_ARG: an index of the instance out of range. */
+int oppx_arena_admit(oppx_arena* arena, const oppx_instance* inst, int32_t* slot);
+
+/* One step of every resident instance: batched push + direct hop + walk
+ * (oppx_fempic_batch_step_fused), Sort_All finalize (oppx_sort_by_cell), FP32 compensated deposit
+ * (oppx_fempic_deposit_sorted_f32). Returns when the step is complete. */
+int oppx_arena_step(oppx_arena* arena);
+
+/* Hand an instance back and free its ranges: *n_out particles remain; pos/vel (3 per particle),
+ * lc (4), p2c (local cell indices) receive them in the instance's order alone (each nullable,
+ * capacity `cap` particles), charge_f32 (nullable) its n_nodes FP32 node charges of the last step.
+ * Call after a step (OPPX_ERR_STATE when an instance was admitted since the last step). */
+int oppx_arena_retire(oppx_arena* arena, int32_t slot, int64_t cap, int64_t* n_out, double* pos, double* vel,
+                      double* lc, int32_t* p2c, float* charge_f32);
+
+/* Host copy of the arena's move counters (faults and hop statistics of the last step). */
+int oppx_arena_counters(const oppx_arena* arena, oppx_move_counters* out);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,15 @@ class BatchInst(Structure):
                 ("k", c_double), ("dt", c_double)]
 
 
+class Instance(Structure):
+    """oppx_instance: one FEM-PIC instance as host arrays (the oppx_arena_admit input)."""
+    _fields_ = [("n_cells", c_int64), ("n_nodes", c_int64), ("n_part", c_int64), ("c_det", c_void_p),
+                ("c_vol", c_void_p), ("c2c", c_void_p), ("c2n", c_void_p), ("c_ef", c_void_p), ("pos", c_void_p),
+                ("vel", c_void_p), ("p2c", c_void_p), ("overlay", c_void_p), ("dh_dims", c_int64 * 3),
+                ("dh_min", c_double * 3), ("dh_inv_gs", c_double), ("charge", c_double), ("mass", c_double),
+                ("dt", c_double)]
+
+
 class Inlet(Structure):
     _fields_ = [("n_faces", c_int64), ("face_cell", c_void_p), ("u", c_void_p), ("v", c_void_p), ("normal", c_void_p),
                 ("node_pos", c_void_p)]
@@ -133,6 +142,10 @@ _SIGS = {
                                               c_void_p, c_void_p, ctypes.c_size_t, c_void_p, c_void_p, c_void_p]),
     "oppx_gather_dats": (c_int, [c_int64, c_int64, c_void_p, c_void_p, POINTER(DatDesc), POINTER(DatDesc), c_int,
                                  c_void_p]),
+    "oppx_fempic_step_fused_rec": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, POINTER(FempicParts), c_void_p,
+                                           POINTER(TetMesh), POINTER(DhGrid), c_double, c_double, c_void_p, c_void_p,
+                                           c_void_p, c_size_t, c_void_p, c_void_p, c_void_p]),
+    "oppx_gather_records": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "oppx_fempic_batch_step_fused": (c_int, [c_int64, POINTER(FempicParts), POINTER(TetMesh), c_void_p, c_void_p,
                                              c_int32, c_void_p, c_void_p, c_void_p, ctypes.c_size_t, c_void_p,
                                              c_void_p, c_void_p]),
@@ -176,6 +189,13 @@ _SIGS = {
     "oppx_holefill": (c_int, [c_int64, c_int64, c_void_p, c_void_p, POINTER(DatDesc), c_int, c_void_p,
                               c_size_t, c_void_p]),
     "oppx_holefill_noalloc": (c_int, [c_int64, c_int64, c_void_p, c_void_p, POINTER(DatDesc), c_int, c_void_p]),
+    "oppx_arena_create": (c_int, [c_int64, c_int64, c_int64, c_int64, c_int32, c_void_p, POINTER(c_void_p)]),
+    "oppx_arena_destroy": (c_int, [c_void_p]),
+    "oppx_arena_admit": (c_int, [c_void_p, POINTER(Instance), POINTER(c_int32)]),
+    "oppx_arena_step": (c_int, [c_void_p]),
+    "oppx_arena_retire": (c_int, [c_void_p, c_int32, c_int64, POINTER(c_int64), c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p]),
+    "oppx_arena_counters": (c_int, [c_void_p, POINTER(MoveCounters)]),
 }
 
 _lib = None
@@ -478,6 +498,8 @@ class FempicDevice:
         self.dats = (DatDesc * 4)(DatDesc(self.pos.data_ptr(), 3, 8), DatDesc(self.vel.data_ptr(), 3, 8),
                                   DatDesc(self.lc.data_ptr(), 4, 8), DatDesc(self.p2c.data_ptr(), 1, 4))
         self.n_max = n  # upper bound of the live count (sizes only shrink without injection)
+        self.pending = False  # a lazy sort's permutation waits for the next pass (or materialize)
+        self.staged = False   # positions/velocities live in a 48-byte record buffer (pass_fused(stage=True))
 
     def stream(self):
         return self.torch.cuda.current_stream(self.dev).cuda_stream
@@ -521,12 +543,22 @@ class FempicDevice:
         assert len(tensors) == len(shapes)
         return (DatDesc * len(tensors))(*[DatDesc(t.data_ptr(), d, e) for t, (d, e) in zip(tensors, shapes)])
 
+    def _records(self):
+        """The two 48-byte position/velocity record buffers (6 doubles per particle slot)."""
+        if not hasattr(self, "_rec"):
+            self._rec = [self.torch.empty(6 * self.cap, dtype=self.torch.float64, device=self.dev) for _ in range(2)]
+            self._rec_cur = 0
+        return self._rec
+
     def sort(self, drop_removed=False, perm=None, lazy=False):
         """Stable sort of the particle dats by cell (removed last); drop_removed applies the finalize.
         lazy=True computes only the permutation (and the sorted p2c, written by the sort's last pass into
         the swap buffer) and leaves the rest of the record gather pending: the next fused step reads its
-        inputs through it (pass_fused), or materialize() applies it."""
-        self.materialize()
+        inputs through it (pass_fused), or materialize() applies it. A lazy sort after a staged pass
+        (pass_fused(stage=True)) keeps the position/velocity records: the next step reads them through
+        the permutation."""
+        if not (lazy and getattr(self, "staged", False) and not getattr(self, "pending", False)):
+            self.materialize()
         swap = self._swap_buffers()
         cur = [self.pos, self.vel, self.lc, self.p2c]
         out_perm = perm if perm is not None else (self.perm if lazy else None)
@@ -545,19 +577,36 @@ class FempicDevice:
         self._rebind()
 
     def materialize(self):
-        """Apply a pending permutation to the particle dats."""
-        if not getattr(self, "pending", False):
+        """Apply a pending permutation to the particle dats, and bring staged position/velocity records
+        back into the SoA dats."""
+        pending, staged = getattr(self, "pending", False), getattr(self, "staged", False)
+        if staged and not pending:  # records of a staged pass, no sort since: same order, into pos/vel
+            check(lib().oppx_gather_records(self.n_max, self.cap, self.counters.data_ptr(), None,
+                                            self._rec[self._rec_cur].data_ptr(), self.pos.data_ptr(),
+                                            self.vel.data_ptr(), self.stream()), "gather_records")
+            self.staged = False
+            return
+        if not pending:
             return
         swap = self._swap_buffers()
         cur = [self.pos, self.vel, self.lc, self.p2c]
         # the sort already wrote the sorted p2c into the swap buffer
-        check(lib().oppx_gather_dats(self.n_max, self.cap, self.counters.data_ptr(), self.perm.data_ptr(),
-                                     self._descs(cur[:3], ("pos", "vel", "lc")), self._descs(swap[:3], ("pos", "vel", "lc")),
-                                     3, self.stream()), "gather_dats")
+        if staged:
+            check(lib().oppx_gather_records(self.n_max, self.cap, self.counters.data_ptr(), self.perm.data_ptr(),
+                                            self._rec[self._rec_cur].data_ptr(), swap[0].data_ptr(),
+                                            swap[1].data_ptr(), self.stream()), "gather_records")
+            check(lib().oppx_gather_dats(self.n_max, self.cap, self.counters.data_ptr(), self.perm.data_ptr(),
+                                         self._descs(cur[2:3], ("lc",)), self._descs(swap[2:3], ("lc",)), 1,
+                                         self.stream()), "gather_dats")
+        else:
+            check(lib().oppx_gather_dats(self.n_max, self.cap, self.counters.data_ptr(), self.perm.data_ptr(),
+                                         self._descs(cur[:3], ("pos", "vel", "lc")),
+                                         self._descs(swap[:3], ("pos", "vel", "lc")), 3, self.stream()), "gather_dats")
         self.pos, self.vel, self.lc, self.p2c = swap
         self._swap = cur
         self._rebind()
         self.pending = False
+        self.staged = False
 
     def deposit_sorted_f32(self):
         """FP32 Kahan deposit over the (cell-sorted) particles; returns the FP32 node charge tensor."""
@@ -595,9 +644,20 @@ class FempicDevice:
         check(lib().oppx_fempic_step_begin(self.counters.data_ptr(), self.charge.data_ptr(), self.nn, self.stream()),
               "fempic_step_begin")
 
-    def pass_fused(self, charge, mass, dt, record_hops=False, deposit=True):
+    def pass_fused(self, charge, mass, dt, record_hops=False, deposit=True, stage=False):
+        """push + direct hop + walk (+ FP64 deposit). stage=True (for a step that a lazy sort follows,
+        with the two-pass move): the new positions and velocities go to 48-byte records instead of the
+        SoA dats, so that the step after the sort reads one record per particle through the permutation."""
         k = charge / mass * dt
-        if getattr(self, "pending", False):  # the step reads its inputs through the pending sort
+        pending, staged = getattr(self, "pending", False), getattr(self, "staged", False)
+        stage = stage and get_tuning()["inline_hops"] >= 1
+        if staged and not pending:
+            self.materialize()
+            staged = False
+        if stage or (pending and staged):
+            self._pass_records(k, dt, record_hops, deposit, stage, pending)
+            return
+        if pending:  # the step reads its inputs through the pending sort
             swap = self._swap_buffers()
             # p2c None: the sort wrote the sorted p2c into the target already
             src = FempicParts(self.cap, self.pos.data_ptr(), self.vel.data_ptr(), self.lc.data_ptr(), None)
@@ -620,6 +680,36 @@ class FempicDevice:
                                            self.hops.data_ptr() if record_hops else None,
                                            self.charge.data_ptr() if deposit else None, self.stream()),
               "fempic_step_fused")
+
+    def _pass_records(self, k, dt, record_hops, deposit, stage, pending):
+        """The fused step through oppx_fempic_step_fused_rec: reads the staged records through the pending
+        permutation (pending) and/or writes this step's records (stage)."""
+        recs = self._records()
+        rec_in = recs[self._rec_cur] if pending else None
+        out_idx = (1 - self._rec_cur) if pending else self._rec_cur  # never the buffer being read
+        rec_out = recs[out_idx] if stage else None
+        if pending:
+            swap = self._swap_buffers()
+            dst = FempicParts(self.cap, swap[0].data_ptr(), swap[1].data_ptr(), swap[2].data_ptr(),
+                              swap[3].data_ptr())
+        else:
+            dst = self.parts
+        check(lib().oppx_fempic_step_fused_rec(
+            self.n_max, self.perm.data_ptr() if pending else None, rec_in.data_ptr() if pending else None,
+            self.lc.data_ptr() if pending else None, ctypes.byref(dst),
+            rec_out.data_ptr() if rec_out is not None else None, ctypes.byref(self.mesh), self._dh_ref(), k, dt,
+            self.counters.data_ptr(), self.hole_bits.data_ptr(), self.work.data_ptr(), self.work_bytes,
+            self.hops.data_ptr() if record_hops else None, self.charge.data_ptr() if deposit else None,
+            self.stream()), "fempic_step_fused_rec")
+        if pending:
+            cur = [self.pos, self.vel, self.lc, self.p2c]
+            self.pos, self.vel, self.lc, self.p2c = swap
+            self._swap = cur
+            self._rebind()
+            self.pending = False
+        if stage:
+            self._rec_cur = out_idx
+        self.staged = stage
 
     def step_fused(self, charge, mass, dt, record_hops=False, fill="holefill"):
         """push + direct hop + walk + deposit in one pass, then hole-fill (or a sort finalize)."""
