@@ -772,6 +772,16 @@ def bench_serve(args):
     large = [serve.Template.of(workloads.box_instance(nx, n, 5_000_000 + i, 0.01))
              for i, (nx, n) in enumerate(args.serve_large)]
     log(f"serve templates in {time.time() - t0:.1f}s: {len(small)} small, {len(large)} large")
+    # the template pool is resident in HBM before the stream starts (admissions copy device to device);
+    # the plain path gets the same: a prototype per template, cloned on the device for every request
+    t0 = time.time()
+    for tpl in small + large:
+        tpl.make_resident()
+        tpl.prototype()
+    import torch
+    torch.cuda.synchronize()
+    pool_gb = sum(t.resident_bytes() for t in small + large) / 1e9
+    log(f"template pool resident in {time.time() - t0:.1f}s: {pool_gb:.2f} GB (+ the plain path's prototypes)")
     rng = np.random.default_rng(5)
     reqs, t = [], 0.0
     while True:
@@ -788,7 +798,6 @@ def bench_serve(args):
     _, st = serve.serve(arena, reqs, budget, log=log)
     log(f"continuous batching: {st}")
     del arena
-    import torch
     torch.cuda.empty_cache()
     reqs_plain = [serve.Request(r.rid, r.arrival, r.tpl, r.steps) for r in reqs]
     serve.serve_plain([serve.Request(i, 0.0, small[i % len(small)], 10) for i in range(min(4, len(small)))])
@@ -809,8 +818,8 @@ def bench_serve(args):
         "dtype": "f64 move, f32 compensated deposit",
         "data": "synthetic",
         "config": {"workload": f"Poisson {args.serve_rate}/s for {args.serve_seconds}s, {len(reqs)} requests of 10 "
-                               f"steps, {len(small)} small + {len(large)} large instance templates, budget "
-                               f"{args.serve_budget_gb} GB"},
+                               f"steps, {len(small)} small + {len(large)} large instance templates (resident in "
+                               f"HBM, {pool_gb:.2f} GB), budget {args.serve_budget_gb} GB"},
         # latency = queueing (arrival to admission) + service (admission to results on the host)
         **{k: st[k] for k in LATENCY_KEYS},
         "offered_load": offered_load(reqs, args.serve_seconds, st),

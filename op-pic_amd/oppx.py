@@ -504,6 +504,29 @@ class FempicDevice:
     def stream(self):
         return self.torch.cuda.current_stream(self.dev).cuda_stream
 
+    # attributes a clone gets afresh (the rest, the read-only mesh tensors and host metadata, it shares)
+    _PER_INSTANCE = ("pos", "vel", "lc", "p2c", "counters", "hole_bits", "work", "scratch", "charge", "hops", "_swap",
+                     "perm", "sort_scratch", "_rec", "_rec_cur", "cell_partials", "charge_f32", "dep_scratch", "parts",
+                     "dats")
+
+    def clone(self):
+        """Another instance over this one's mesh -- its device tensors shared, read-only -- with a device
+        copy of this particle state and its own counters, charge and work buffers (no host upload)."""
+        torch = self.torch
+        self.materialize()
+        g = object.__new__(type(self))
+        g.__dict__.update({k: v for k, v in self.__dict__.items() if k not in self._PER_INSTANCE})
+        g.pos, g.vel, g.lc, g.p2c = (t.clone() for t in (self.pos, self.vel, self.lc, self.p2c))
+        g.counters = self.counters.clone()
+        for name in ("hole_bits", "work", "scratch", "charge", "hops"):
+            setattr(g, name, torch.zeros_like(getattr(self, name)))
+        if hasattr(self, "node_start"):  # the FP32 deposit's node rows are shared, its outputs are not
+            for name in ("cell_partials", "charge_f32", "dep_scratch"):
+                setattr(g, name, torch.zeros_like(getattr(self, name)))
+        g.pending = g.staged = False
+        g._rebind()
+        return g
+
     def _dh_ref(self):
         return ctypes.byref(self.dh) if self.dh is not None else None
 

@@ -90,6 +90,8 @@ class Template:
     n_part: int
     overlay_words: int
     host: dict = field(default_factory=dict)
+    dev: dict = field(default_factory=dict)  # the same arrays resident in HBM (make_resident)
+    proto: object = None                     # a FempicDevice of the instance, never stepped (prototype)
 
     @staticmethod
     def of(inst: dict) -> "Template":
@@ -113,6 +115,25 @@ class Template:
         if ov is not None:
             host["overlay"] = pin(ov.astype(np.int32))
         return Template(inst, nc, nn, len(inst["p2c"]), 0 if ov is None else int(ov.size), host)
+
+    def make_resident(self, device="cuda"):
+        """Keep the staged arrays in HBM as well: an admission then copies device to device instead of
+        over the host link (the template pool stays resident across requests)."""
+        if not self.dev:
+            self.dev = {k: v.to(device, non_blocking=True) for k, v in self.host.items()}
+        return self
+
+    def resident_bytes(self) -> int:
+        return sum(v.numel() * v.element_size() for v in self.dev.values())
+
+    def prototype(self):
+        """The instance as its own FempicDevice (built once, never stepped): the plain per-request path
+        clones it (device copies of the particle state, the mesh shared read-only) for every request."""
+        if self.proto is None:
+            d = self.inst
+            self.proto = oppx.FempicDevice(d["mesh"], d["ef"], d["pos"], d["vel"], d["p2c"], overlay=d.get("overlay"),
+                                           bmin=d.get("bmin"), bmax=d.get("bmax"), gs=d.get("gs"))
+        return self.proto
 
     def device_bytes(self) -> int:
         """HBM a resident request holds: mesh records, node rows, overlay, particles (+ sort buffers)."""
@@ -183,7 +204,7 @@ class FempicArena:
             return None
         L, s = lib(), self.stream()
         torch = self.torch
-        h, d = tpl.host, tpl.inst
+        h, d = (tpl.dev or tpl.host), tpl.inst  # device to device when the template is resident
         nc, nn, n = tpl.n_cells, tpl.n_nodes, tpl.n_part
         cb, nb = self.cells.alloc(nc), self.nodes.alloc(nn)
         ob = self.ovl.alloc(tpl.overlay_words) if tpl.overlay_words else -1
@@ -463,8 +484,11 @@ def serve_plain(requests, keep_results: bool = False):
             time.sleep(max(0.0, min(r.arrival - (clock() - t0), 0.01)))
         r.t_admit = clock() - t0  # this path's admission: the request's turn on the device
         d = r.tpl.inst
-        g = oppx.FempicDevice(d["mesh"], d["ef"], d["pos"], d["vel"], d["p2c"], overlay=d.get("overlay"),
-                              bmin=d.get("bmin"), bmax=d.get("bmax"), gs=d.get("gs"))
+        if r.tpl.proto is not None:  # a resident template: device copies of its state
+            g = r.tpl.proto.clone()
+        else:
+            g = oppx.FempicDevice(d["mesh"], d["ef"], d["pos"], d["vel"], d["p2c"], overlay=d.get("overlay"),
+                                  bmin=d.get("bmin"), bmax=d.get("bmax"), gs=d.get("gs"))
         for _ in range(r.steps):
             part_steps += g.n_max if _ == 0 else g.sync_size()
             g.step_begin()

@@ -89,3 +89,28 @@ def test_admission_waits_for_budget():
     assert all(r.t_done >= r.t_retire for r in done)
     for r in done:
         assert same(r.result, alone(r.tpl, r.steps)), f"request {r.rid}"
+
+
+def test_resident_templates_and_plain_clones_give_the_same_results():
+    """Templates kept in HBM (admission copies device to device) and the plain path's prototypes cloned on
+    the device for every request: the same bits as the host-staged admission and a fresh alone run."""
+    tpls = make_templates([(3, 4000), (5, 20000), (4, 9000), (2, 700)], seed=63)
+    for t in tpls:
+        t.make_resident()
+        t.prototype()
+    steps = [3, 5, 4, 2]
+    reqs = [serve.Request(i, 0.0 if i < 2 else 1e-3 * i, t, s) for i, (t, s) in enumerate(zip(tpls + tpls, steps * 2))]
+    arena = serve.FempicArena(max_cells=20000, max_nodes=5000, max_particles=120000, max_overlay=40000, max_insts=8)
+    done, stats = serve.serve(arena, reqs, budget_bytes=1e12, keep_results=True)
+    assert stats["requests"] == len(reqs)
+    plain = [serve.Request(r.rid, 0.0, r.tpl, r.steps) for r in reqs]
+    serve.serve_plain(plain, keep_results=True)
+    for r, p in zip(sorted(done, key=lambda r: r.rid), plain):
+        ref = alone(r.tpl, r.steps)
+        assert same(r.result, ref), f"request {r.rid} (batched, resident template)"
+        assert same(p.result, ref), f"request {r.rid} (plain, cloned prototype)"
+    # the prototypes were never stepped: a clone still starts from the template's initial state
+    for t in tpls:
+        g = t.prototype().clone()
+        out = g.download()
+        assert out["n"] == t.n_part and np.array_equal(out["pos"], t.inst["pos"])
