@@ -96,3 +96,85 @@ def test_cpp_arena_serving_loop_equals_alone():
                 for key in ("pos", "vel", "lc", "p2c"):
                     assert np.array_equal(got[key], e[key]), f"instance {i} {key} vs oracle"
         assert waited >= 1, "the arena was large enough for every arrival: no admission waited"
+
+
+def _desc(d, keep):
+    """oppx_instance of a workloads.box_instance dict (the numpy arrays kept alive in `keep`)."""
+    m = d["mesh"]
+    arrs = [np.ascontiguousarray(a) for a in (m["c_det"], m["c_vol"], m["c2c"].astype(np.int32),
+                                               m["c2n"].astype(np.int32), d["ef"], d["pos"], d["vel"],
+                                               d["p2c"].astype(np.int32), d["overlay"].astype(np.int32))]
+    keep.extend(arrs)
+    p = [a.ctypes.data for a in arrs]
+    dims = oppx.dh_dims(d["bmin"], d["bmax"], d["gs"])
+    import ctypes
+    return oppx.Instance(int(m["n_cells"]), int(m["n_nodes"]), len(d["p2c"]), *p, (ctypes.c_int64 * 3)(*dims),
+                         (ctypes.c_double * 3)(*d["bmin"]), 1.0 / d["gs"], d["charge"], d["mass"], d["dt"])
+
+
+def test_arena_error_paths_leave_it_usable():
+    """Out-of-range indices are refused before anything is copied; a retirement before the step that
+    follows an admission is refused (the set is not in cell order yet); an instance that does not fit is
+    refused with OPPX_ERR_NOMEM until a retirement frees room; a retired slot cannot be retired again; an
+    instance with no particles is served. After each refusal the arena still serves correctly."""
+    import ctypes
+
+    L = oppx.lib()
+    keep = []
+    a_inst = workloads.box_instance(4, 5000, 401, 0.02)
+    b_inst = workloads.box_instance(5, 9000, 402, 0.01)
+    empty = workloads.box_instance(3, 1, 403, 0.02)
+    empty = dict(empty, pos=empty["pos"][:0], vel=empty["vel"][:0], p2c=empty["p2c"][:0])
+    nc = {k: int(d["mesh"]["n_cells"]) for k, d in (("a", a_inst), ("b", b_inst), ("e", empty))}
+    cells = nc["e"] + nc["b"]  # b fits only where a was, once a has gone
+    arena = ctypes.c_void_p()
+    assert L.oppx_arena_create(cells, cells, 20000, 20000, 4, None, ctypes.byref(arena)) == oppx.OK
+
+    def retire(slot, d):
+        cap = max(1, len(d["p2c"]))
+        n = ctypes.c_int64()
+        out = {k: np.zeros((cap, w)) for k, w in (("pos", 3), ("vel", 3), ("lc", 4))}
+        p2c, q = np.zeros(cap, np.int32), np.zeros(int(d["mesh"]["n_nodes"]), np.float32)
+        rc = L.oppx_arena_retire(arena, slot, cap, ctypes.byref(n), out["pos"].ctypes.data, out["vel"].ctypes.data,
+                                 out["lc"].ctypes.data, p2c.ctypes.data, q.ctypes.data)
+        k = n.value
+        return rc, dict(n=k, pos=out["pos"][:k], vel=out["vel"][:k], lc=out["lc"][:k], p2c=p2c[:k], charge_f32=q)
+
+    def admit(d):
+        slot = ctypes.c_int32(-1)
+        return L.oppx_arena_admit(arena, ctypes.byref(_desc(d, keep)), ctypes.byref(slot)), slot.value
+
+    try:
+        bad = dict(a_inst, p2c=np.where(np.arange(len(a_inst["p2c"])) == 7, 10 ** 6, a_inst["p2c"]).astype(np.int32))
+        assert admit(bad)[0] == oppx.ERR_ARG
+        rc, se = admit(empty)
+        assert rc == oppx.OK
+        rc, sa = admit(a_inst)
+        assert rc == oppx.OK
+        assert admit(b_inst)[0] == oppx.ERR_NOMEM  # no room beside a: refused, nothing changed
+        assert retire(sa, a_inst)[0] == oppx.ERR_STATE  # admitted since the last step
+        for _ in range(3):
+            assert L.oppx_arena_step(arena) == oppx.OK
+        rc, got = retire(sa, a_inst)
+        assert rc == oppx.OK
+        assert retire(sa, a_inst)[0] == oppx.ERR_ARG  # already retired
+        exp = _alone(a_inst, 3)
+        assert got["n"] == exp["n"]
+        for key in ("pos", "vel", "lc", "p2c", "charge_f32"):
+            assert np.array_equal(got[key], exp[key]), key
+        rc, sb = admit(b_inst)  # a's cells are free again
+        assert rc == oppx.OK
+        for _ in range(2):
+            assert L.oppx_arena_step(arena) == oppx.OK
+        rc, got = retire(sb, b_inst)
+        assert rc == oppx.OK
+        exp = _alone(b_inst, 2)
+        assert got["n"] == exp["n"]
+        for key in ("pos", "vel", "lc", "p2c", "charge_f32"):
+            assert np.array_equal(got[key], exp[key]), f"b {key}"
+        rc, got = retire(se, empty)
+        assert rc == oppx.OK and got["n"] == 0 and not got["charge_f32"].any()
+        c = oppx.MoveCounters()
+        assert L.oppx_arena_counters(arena, ctypes.byref(c)) == oppx.OK and c.faults == 0
+    finally:
+        assert L.oppx_arena_destroy(arena) == oppx.OK
