@@ -822,6 +822,9 @@ def bench_serve(args):
                                f"HBM, {pool_gb:.2f} GB), budget {args.serve_budget_gb} GB"},
         # latency = queueing (arrival to admission) + service (admission to results on the host)
         **{k: st[k] for k in LATENCY_KEYS},
+        # one batched step over every resident request (host-synchronised)
+        "p50_step_ms": st["p50_step_ms"],
+        "p99_step_ms": st["p99_step_ms"],
         "offered_load": offered_load(reqs, args.serve_seconds, st),
         "requests": len(reqs),
         "iterations": st["iterations"],
@@ -920,7 +923,7 @@ ENSEMBLE_KEYS = ("value", "plain_value", "speedup_vs_plain", "ms_per_step", "p50
                  "graph_equals_eager", "fp32_deposit_err_vs_gpu_fp64_atomic", "phase_ms_median", "autotune_ms_fastest_slowest",
                  "config")
 SERVE_KEYS = ("value", "plain_value", "speedup_vs_plain", "requests", "iterations", "peak_resident_gb",
-              "offered_load", "config") + LATENCY_KEYS
+              "offered_load", "p50_step_ms", "p99_step_ms", "config") + LATENCY_KEYS
 
 
 def batched_summaries(args):

@@ -414,6 +414,7 @@ def serve(arena: FempicArena, requests, budget_bytes: float, keep_results: bool 
     resident = 0.0
     peak_resident, iters, part_steps = 0.0, 0, 0
     spent = dict(admit_s=0.0, step_s=0.0, retire_s=0.0)
+    step_ms = []  # wall time of every batched step (all resident requests), host-synchronised
     prev = None  # (results still copying, their requests): resolved after the next step
 
     def finish(pr):
@@ -448,8 +449,9 @@ def serve(arena: FempicArena, requests, budget_bytes: float, keep_results: bool 
             continue
         part_steps += arena.size - arena.dropping  # retired particles still in the set are not moved
         ts = clock()
-        arena.step()
-        spent["step_s"] += clock() - ts
+        arena.step()  # returns once the step is complete (it reads the new live count)
+        step_ms.append(1e3 * (clock() - ts))
+        spent["step_s"] += step_ms[-1] * 1e-3
         iters += 1
         finished = [k for k, r in active.items() if r.left == 1]
         tr = clock()
@@ -469,7 +471,9 @@ def serve(arena: FempicArena, requests, budget_bytes: float, keep_results: bool 
         finish(prev)
     wall = clock() - t0
     return done, dict(wall_s=wall, iterations=iters, particle_steps=part_steps, value=part_steps / wall,
-                      **latency_split(done), peak_resident_bytes=peak_resident, requests=len(done), **spent)
+                      **latency_split(done), p50_step_ms=float(np.percentile(step_ms, 50)) if step_ms else 0.0,
+                      p99_step_ms=float(np.percentile(step_ms, 99)) if step_ms else 0.0,
+                      peak_resident_bytes=peak_resident, requests=len(done), **spent)
 
 
 def serve_plain(requests, keep_results: bool = False):
