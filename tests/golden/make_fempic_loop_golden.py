@@ -71,6 +71,37 @@ def loop_inputs(nx: int, seed: int, per_face_max: int, sigma: float = 0.02):
                 dt=dt, ion_velocity=0.4 * h / dt, charge=1e-3, mass=1.0, spwt=2.0, wall_potential=1.5)
 
 
+def loop_inputs_mesh12(nx: int, ny: int, nz: int, seed: int, per_face_max: int, sigma: float = 0.02):
+    """The 12-tet box of nx x ny x nz cubes written in the app's text format and read back through the
+    loader (oppx.box12_mesh: the geometry, node volumes, node types and inlet faces the reference app's
+    LoadVolumeMesh / LoadSurfaceMesh compute from those files); the field-side dats and the inlet
+    distribution synthetic as in loop_inputs."""
+    m = oppx.box12_mesh(nx, ny, nz, 1.0, ny / nx, nz / nx)
+    ef = oppx.box_field_sines(m)
+    rng = np.random.default_rng(seed)
+    npos, c2n = m["node_pos"].reshape(-1, 3), m["c2n"].reshape(-1, 4)
+    det, vol = m["c_det"].reshape(-1, 4, 4), m["c_vol"]
+    s = 1.0 / (6.0 * vol)
+    sd = np.stack([-det[:, :, 1] * s[:, None], det[:, :, 2] * s[:, None], -det[:, :, 3] * s[:, None]], axis=2)
+    sd = np.ascontiguousarray(sd.reshape(-1, 12))
+    n_pot = np.sin(math.pi * npos[:, 0]) * np.sin(math.pi * npos[:, 1]) * np.sin(math.pi * npos[:, 2])
+    n_pot = n_pot + 0.25 * npos[:, 2]
+    nf = len(m["if2c"])
+    counts = rng.integers(0, per_face_max + 1, nf)
+    counts[1] = 0  # a face without particles: the distribution's cumulative count repeats
+    dist = np.cumsum(counts).astype(np.int32)
+    inject = int(dist[-1])
+    rand = rng.random((inject + INJ_EXCESS, 2))
+    h = 1.0 / nx
+    dt = 0.5 * h / (sigma * math.sqrt(3.0))
+    if_npos = np.ascontiguousarray(npos[m["if2n"].reshape(-1, 3)].reshape(-1, 9))
+    return dict(mesh=m, ef=ef, sd=sd, n_vol=m["node_vol"], n_type=m["node_type"], n_pot=n_pot, if2c=m["if2c"],
+                if_u=np.ascontiguousarray(m["if_u"]), if_v=np.ascontiguousarray(m["if_v"]),
+                if_norm=np.ascontiguousarray(m["if_normal"]), if_npos=if_npos, if_dist=dist, rand=rand,
+                inject=inject, dt=dt, ion_velocity=0.4 * h / dt, charge=1e-3, mass=1.0, spwt=2.0, wall_potential=1.5,
+                bmax=(1.0, ny / nx, nz / nx), mesh12=(nx, ny, nz))
+
+
 def run_ref(d, *, steps, dh, gs):
     m = d["mesh"]
     with tempfile.TemporaryDirectory() as td:
@@ -89,7 +120,7 @@ def run_ref(d, *, steps, dh, gs):
                f"REAL spwt = {d['spwt']!r}", f"REAL ion_velocity = {d['ion_velocity']!r}",
                f"REAL wall_potential = {d['wall_potential']!r}", f"REAL grid_spacing = {gs!r}",
                "REAL bminx = 0.0", "REAL bminy = 0.0", "REAL bminz = 0.0",
-               "REAL bmaxx = 1.0", "REAL bmaxy = 1.0", "REAL bmaxz = 1.0",
+               *[f"REAL bmax{a} = {v!r}" for a, v in zip("xyz", d.get("bmax", (1.0, 1.0, 1.0)))],
                f"BOOL opp_global_move = {'true' if dh else 'false'}", "BOOL opp_dh_data_generate = true",
                "REAL opp_allocation_multiple = 1.1", "BOOL opp_profile_all = false",
                "STRING opp_fill = HoleFill_All", "INT opp_fill_period = 4"]
@@ -113,8 +144,8 @@ def run_ref(d, *, steps, dh, gs):
         return out
 
 
-def make_case(name, nx, seed, per_face_max, steps, dh):
-    d = loop_inputs(nx, seed, per_face_max)
+def make_case(name, nx, seed, per_face_max, steps, dh, mesh12=None):
+    d = loop_inputs(nx, seed, per_face_max) if mesh12 is None else loop_inputs_mesh12(*mesh12, seed, per_face_max)
     gs = 1.0 / (2 * nx)
     out = run_ref(d, steps=steps, dh=dh, gs=gs)
     arrays = {f"meta_{k}": np.array(v) for k, v in dict(nx=nx, seed=seed, per_face_max=per_face_max, steps=steps,
@@ -122,6 +153,8 @@ def make_case(name, nx, seed, per_face_max, steps, dh):
                                                         charge=d["charge"], mass=d["mass"], spwt=d["spwt"],
                                                         wall_potential=d["wall_potential"],
                                                         inject=d["inject"]).items()}
+    if mesh12 is not None:  # the 12-tet box read through the mesh-file loader
+        arrays.update(meta_mesh12=np.array(mesh12), meta_bmax=np.array(d["bmax"]))
     for k in ("sd", "n_vol", "n_type", "n_pot", "if2c", "if_u", "if_v", "if_norm", "if_npos", "if_dist", "rand"):
         arrays[f"in_{k}"] = np.ascontiguousarray(d[k])
     arrays.update({f"out_{k}": v for k, v in out.items()})
@@ -136,6 +169,7 @@ def main():
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "_ref/fempic_loop_ref_seq"], check=True)
     make_case("small", nx=4, seed=31, per_face_max=12, steps=8, dh=False)
     make_case("dh", nx=6, seed=32, per_face_max=20, steps=12, dh=True)
+    make_case("mesh12", nx=6, seed=33, per_face_max=20, steps=12, dh=True, mesh12=(6, 6, 5))
 
 
 if __name__ == "__main__":

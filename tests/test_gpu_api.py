@@ -171,16 +171,17 @@ def test_neso_api_app_matches_reference(name):
         assert np.array_equal(rd("final_mdir", np.int32), g["out_final_mdir"].ravel())
 
 
-@pytest.mark.parametrize("name", ["fempic_loop_small.npz", "fempic_loop_dh.npz"])
+@pytest.mark.parametrize("name", ["fempic_loop_small.npz", "fempic_loop_dh.npz", "fempic_loop_mesh12.npz"])
 def test_fempic_loop_app_matches_reference(name):
     """The main loop minus the field solve through the drop-in (injection, push, move, deposit, node
     charge density, electric field, OPP_MAX globals) against the reference seq build: particles,
     injected records, boundary potential, field and the field maxima bit-exact; charge density and
-    its maximum within 1e-12 relative (FP64 atomic deposit)."""
+    its maximum within 1e-12 relative (FP64 atomic deposit). The mesh12 case runs on a 12-tet box read
+    through the mesh-file loader, with its node volumes, node types and inlet faces."""
+    from test_oracle_loop import loop_mesh
+
     g = np.load(golden_path(name), allow_pickle=False)
-    nx = int(g["meta_nx"])
-    m = oppx.box_mesh(nx, nx, nx)
-    ef = oppx.box_field_sines(m)
+    m, ef, bmax = loop_mesh(g)
     with tempfile.TemporaryDirectory() as td:
         ind, outd = os.path.join(td, "in"), os.path.join(td, "out")
         os.makedirs(ind)
@@ -197,7 +198,7 @@ def test_fempic_loop_app_matches_reference(name):
                  f"REAL charge = {f('charge')!r}", f"REAL spwt = {f('spwt')!r}",
                  f"REAL ion_velocity = {f('ion_velocity')!r}", f"REAL wall_potential = {f('wall_potential')!r}",
                  f"REAL grid_spacing = {f('gs')!r}", "REAL bminx = 0.0", "REAL bminy = 0.0", "REAL bminz = 0.0",
-                 "REAL bmaxx = 1.0", "REAL bmaxy = 1.0", "REAL bmaxz = 1.0",
+                 *[f"REAL bmax{a} = {v!r}" for a, v in zip("xyz", bmax)],
                  f"BOOL opp_global_move = {'true' if int(g['meta_dh']) else 'false'}",
                  "BOOL opp_dh_data_generate = true", "REAL opp_allocation_multiple = 1.1",
                  "BOOL opp_profile_all = false", "STRING opp_fill = HoleFill_All", "INT opp_fill_period = 4"]

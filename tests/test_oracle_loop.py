@@ -3,7 +3,8 @@ node charge density, electric field, OPP_MAX globals) against the reference seq 
 (tests/golden/make_fempic_loop_golden.py): every array bit-exact, the maxima exact. This pins the
 oracle's restatement of inject_ions, the distribution fill, compute_node_charge_density,
 compute_electric_field, init_boundary_pot and the max folds, including the FMA contractions of
-the reference's GCC build."""
+the reference's GCC build. The third case runs on a 12-tet box read through the mesh-file loader, with
+the loader's node volumes, node types and inlet faces (the reference app's own mesh input path)."""
 
 import numpy as np
 import pytest
@@ -13,23 +14,34 @@ import oracle
 from conftest import golden_path
 
 
+def loop_mesh(g):
+    """The fixture's mesh: the Kuhn box, or the 12-tet box read through the mesh-file loader."""
+    if "meta_mesh12" in g:
+        nx, ny, nz = (int(v) for v in g["meta_mesh12"])
+        m = oppx.box12_mesh(nx, ny, nz, 1.0, ny / nx, nz / nx)
+        bmax = tuple(float(v) for v in g["meta_bmax"])
+    else:
+        nx = int(g["meta_nx"])
+        m = oppx.box_mesh(nx, nx, nx)
+        bmax = (1.0, 1.0, 1.0)
+    return m, oppx.box_field_sines(m), bmax
+
+
 def loop_case(name):
     g = np.load(golden_path(name), allow_pickle=False)
-    nx = int(g["meta_nx"])
-    m = oppx.box_mesh(nx, nx, nx)
-    ef = oppx.box_field_sines(m)
+    m, ef, bmax = loop_mesh(g)
     inlet = {k: g[f"in_{k}"] for k in ("if2c", "if_u", "if_v", "if_norm", "if_npos", "if_dist")}
     inject, steps = int(g["meta_inject"]), int(g["meta_steps"])
     kw = {}
     if int(g["meta_dh"]):
         gs = float(g["meta_gs"])
-        kw = dict(overlay=oppx.dh_overlay(m, (0, 0, 0), (1, 1, 1), gs), bmin=(0, 0, 0), bmax=(1, 1, 1), gs=gs)
+        kw = dict(overlay=oppx.dh_overlay(m, (0, 0, 0), bmax, gs), bmin=(0, 0, 0), bmax=bmax, gs=gs)
     o = oracle.FempicLoop(m, ef, g["in_sd"], g["in_n_vol"], g["in_n_type"], g["in_n_pot"], inlet, g["in_rand"],
                           capacity=inject * (steps + 1), **kw)
     return g, o, inject, steps
 
 
-@pytest.mark.parametrize("name", ["fempic_loop_small.npz", "fempic_loop_dh.npz"])
+@pytest.mark.parametrize("name", ["fempic_loop_small.npz", "fempic_loop_dh.npz", "fempic_loop_mesh12.npz"])
 def test_oracle_main_loop_matches_reference(name):
     g, o, inject, steps = loop_case(name)
     q, ms, dt = float(g["meta_charge"]), float(g["meta_mass"]), float(g["meta_dt"])
