@@ -268,7 +268,10 @@ def bench_ensemble(args):
     """Batched ensemble: N independent tet-box instances (tets log-uniform 2k-60k, particles log-uniform
     10k-2M, sigma uniform [0.005, 0.02], seed 4) stepped 20 times as one concatenated set: batched walk,
     Sort_All finalize and FP32 compensated deposit, launch shape from the autotuner, the 20 steps
-    captured in one hipGraph and launched once. Compared with the plain path: every instance stepped
+    captured in one hipGraph and launched once. With the two-pass move each sort is lazy (permutation and
+    sorted p2c; the next pass reads 48-byte position/velocity records through the permutation, the
+    deposit a gathered lc) and the 20th sort is applied in place at the end (inside the timed graph);
+    --eager-sort gathers every dat at every sort instead. Compared with the plain path: every instance stepped
     by its own launches, one after the other."""
     import torch
 
@@ -285,42 +288,41 @@ def bench_ensemble(args):
 
     b = oppx.FempicBatch(inst)
     snap = autotune.StateSnapshot(b)
-    best, table = autotune.autotune(lambda: b.step_batch("f32"), snap, autotune.DEFAULT_CANDIDATES, steps=2,
-                                    use_cache=False, log=log)
-    log(f"autotuned launch shape {best}")
+    # two steps a trial: the second reads the first one's lazy sort (with the two-pass move)
+    best, table = autotune.autotune(lambda: b.steps_batch(2, "f32", lazy=not args.eager_sort), snap,
+                                    autotune.DEFAULT_CANDIDATES, steps=1, per_call=2, use_cache=False, log=log)
+    lazy = not args.eager_sort and best["inline_hops"] >= 1
+    log(f"autotuned launch shape {best}; lazy sorts: {lazy}")
 
     # eager steps: per-step latency and the live counts of the trajectory (deterministic)
     snap.restore()
     torch.cuda.synchronize()
-    lat, live, phases = [], [], []
-    for _ in range(S):
-        live.append(b.read_counters()["size"])
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-        ev[0].record()
-        b.step_begin()
-        b.pass_batch(deposit=False)
-        ev[1].record()
-        b.sort(drop_removed=True)
-        ev[2].record()
-        b.deposit_sorted_f32()
-        ev[3].record()
-        torch.cuda.synchronize()
-        lat.append(ev[0].elapsed_time(ev[3]))
-        phases.append([ev[k].elapsed_time(ev[k + 1]) for k in range(3)])
+    lat, live, phases, ev = [], [], [], []
+
+    def mark(phase):
+        if phase == 0:
+            live.append(b.read_counters()["size"])
+            ev[:] = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[phase].record()
+        if phase == 3:
+            torch.cuda.synchronize()
+            lat.append(ev[0].elapsed_time(ev[3]))
+            phases.append([ev[k].elapsed_time(ev[k + 1]) for k in range(3)])
+
+    b.steps_batch(S, "f32", lazy=lazy, mark=mark)
     final_eager = b.split()
     ph = np.array(phases)
     log(f"ensemble step phases (move, sort, fp32 deposit) ms: first {ph[0].round(2).tolist()}, "
         f"median {np.median(ph, axis=0).round(2).tolist()}")
 
-    # the S steps as one graph launch (S even: the sort's buffer swap returns to the start buffers)
+    # the S steps as one graph launch (S even: the sequence ends on the start buffers)
     snap.restore()
     torch.cuda.synchronize()
     side = torch.cuda.Stream()
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.stream(side):
         with torch.cuda.graph(graph, stream=side):
-            for _ in range(S):
-                b.step_batch("f32")
+            b.steps_batch(S, "f32", lazy=lazy)
     snap.restore()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -392,6 +394,7 @@ def bench_ensemble(args):
         "p50_step_ms": float(np.percentile(lat, 50)),
         "p99_step_ms": float(np.percentile(lat, 99)),
         "graph_equals_eager": bool(same),
+        "lazy_sort": bool(lazy),
         "phase_ms_median": {"move": float(np.median(ph[:, 0])), "sort": float(np.median(ph[:, 1])),
                             "fp32_deposit": float(np.median(ph[:, 2]))},
         "fp32_deposit_err_vs_gpu_fp64_atomic": f32_err,
@@ -920,8 +923,8 @@ def main():
 
 
 ENSEMBLE_KEYS = ("value", "plain_value", "speedup_vs_plain", "ms_per_step", "p50_step_ms", "p99_step_ms",
-                 "graph_equals_eager", "fp32_deposit_err_vs_gpu_fp64_atomic", "phase_ms_median", "autotune_ms_fastest_slowest",
-                 "config")
+                 "graph_equals_eager", "lazy_sort", "fp32_deposit_err_vs_gpu_fp64_atomic", "phase_ms_median",
+                 "autotune_ms_fastest_slowest", "config")
 SERVE_KEYS = ("value", "plain_value", "speedup_vs_plain", "requests", "iterations", "peak_resident_gb",
               "offered_load", "p50_step_ms", "p99_step_ms", "config") + LATENCY_KEYS
 

@@ -336,6 +336,16 @@ int oppx_fempic_batch_step_fused(int64_t n_max, const oppx_fempic_parts* parts, 
                                  oppx_move_counters* counters, uint64_t* hole_bits, void* work, size_t work_bytes,
                                  int32_t* hops, double* node_charge, void* stream);
 
+/* oppx_fempic_batch_step_fused with 48-byte position/velocity records on either side, as
+ * oppx_fempic_step_fused_rec describes them (no reference counterpart): the batched step before a lazy
+ * sort of the whole ensemble and the step after it. Per-particle results equal
+ * oppx_fempic_batch_step_fused. Needs the two-pass move (tuning.inline_hops >= 1 and its workspace). */
+int oppx_fempic_batch_step_fused_rec(int64_t n_max, const int32_t* perm, const double* src_rec, const double* src_lc,
+                                     const oppx_fempic_parts* parts, double* rec_out, const oppx_tet_mesh* mesh,
+                                     const int32_t* cell_inst, const oppx_batch_inst* insts, int32_t n_insts,
+                                     oppx_move_counters* counters, uint64_t* hole_bits, void* work, size_t work_bytes,
+                                     int32_t* hops, double* node_charge, void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * FEM-PIC injection and the loops either side of the field solve (device)
  * ------------------------------------------------------------------------------------- */

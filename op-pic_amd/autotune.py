@@ -94,9 +94,10 @@ class StateSnapshot:
 
 
 def autotune(step: Callable[[], None], snapshot: StateSnapshot, candidates: Iterable[dict] | None = None,
-             steps: int = 2, signature: str | None = None, use_cache: bool = True, log=None, repeats: int = 2):
+             steps: int = 2, signature: str | None = None, use_cache: bool = True, log=None, repeats: int = 2,
+             per_call: int = 1):
     """Time `steps` calls of `step` per candidate tuning from the same state, best of `repeats`; set and
-    return the fastest.
+    return the fastest. per_call: the number of steps one call of `step` makes (the table is per step).
 
     Returns (best_tuning, [(tuning, ms_per_step), ...]). The state is restored after the search."""
     import torch
@@ -124,7 +125,7 @@ def autotune(step: Callable[[], None], snapshot: StateSnapshot, candidates: Iter
                     step()
                 b.record()
                 torch.cuda.synchronize()
-                ms = min(ms, a.elapsed_time(b) / steps)
+                ms = min(ms, a.elapsed_time(b) / (steps * per_call))
             table.append((cand, ms))
             if log:
                 log(f"autotune {cand}: {ms:.3f} ms/step")

@@ -1247,6 +1247,60 @@ void launch_pass(const StepArgs& P, bool dh, bool stats, int64_t n_max, hipStrea
     }
 }
 
+template <bool DH, bool BATCH, bool GATHER>
+void launch_rec(const StepArgs& P, bool deposit, int64_t n_max, void* work, size_t work_bytes, hipStream_t s) {
+    if (deposit) launch_two_pass<true, DH, true, BATCH, GATHER>(P, n_max, work, work_bytes, s);
+    else launch_two_pass<true, DH, false, BATCH, GATHER>(P, n_max, work, work_bytes, s);
+}
+
+// The fused step with 48-byte position/velocity records on either side (oppx_fempic_step_fused_rec and its
+// batched form): argument checks shared by both, then the two-pass move with the gather mode when a pending
+// permutation is given. P carries the mesh, particle arrays, push coefficients and (BATCH) instance tables.
+template <bool BATCH>
+int step_rec(StepArgs& P, const char* who, int64_t n_max, const int32_t* perm, const double* src_rec,
+             const double* src_lc, const oppx_fempic_parts* parts, double* rec_out, bool dh, uint64_t* hole_bits,
+             void* work, size_t work_bytes, int32_t* hops, double* node_charge, hipStream_t s) {
+    const std::string w(who);
+    OPPX_ARG_CHECK(perm ? (src_rec && src_lc) : (!src_rec && !src_lc),
+                   w + ": src_rec and src_lc go with perm (all three or none)");
+    OPPX_ARG_CHECK(perm || rec_out, w + ": neither records in nor records out");
+    OPPX_ARG_CHECK(parts->lc && parts->p2c && (rec_out || (parts->pos && parts->vel)) &&
+                       (perm || (parts->pos && parts->vel)),
+                   w + ": particle arrays missing");
+    OPPX_ARG_CHECK(!rec_out || rec_out != src_rec, w + ": rec_out must not be src_rec");
+    OPPX_ARG_CHECK(!perm || src_lc != parts->lc, w + ": src_lc and parts->lc must be separate");
+    OPPX_ARG_CHECK(n_max >= 0 && n_max <= parts->stride, w + ": n_max beyond capacity");
+    OPPX_ARG_CHECK(hole_bits != nullptr, w + ": hole_bits required");
+    OPPX_ARG_CHECK(g_tuning.inline_hops >= 1 && work && work_bytes >= work_need(n_max),
+                   w + ": needs the two-pass move (tuning.inline_hops >= 1 and its workspace)");
+    if (n_max == 0) return OPPX_OK;
+    P.n_max = n_max;
+    P.hole_bits = hole_bits;
+    P.hops = hops;
+    P.charge = node_charge;
+    P.rec_out = rec_out;
+    attach_slots(P, work, work_bytes, s);
+    if (perm) {
+        P.perm = perm;
+        P.src_rec = src_rec;
+        P.src_lc = src_lc;
+        P.src_pos = nullptr;  // never read: records replace them
+        P.src_vel = nullptr;
+        P.src_p2c = nullptr;  // the sort wrote the sorted p2c into parts->p2c: read in place
+    }
+    // batched: each instance brings its own overlay (the instance table), so the DH form is the one built
+    if (BATCH || dh) {
+        if (perm) launch_rec<true, BATCH, true>(P, node_charge != nullptr, n_max, work, work_bytes, s);
+        else launch_rec<true, BATCH, false>(P, node_charge != nullptr, n_max, work, work_bytes, s);
+    } else if constexpr (!BATCH) {
+        if (perm) launch_rec<false, false, true>(P, node_charge != nullptr, n_max, work, work_bytes, s);
+        else launch_rec<false, false, false>(P, node_charge != nullptr, n_max, work, work_bytes, s);
+    }
+    fold_slots(P, s);
+    OPPX_LAUNCH_CHECK(who);
+    return OPPX_OK;
+}
+
 }  // namespace
 }  // namespace oppx
 
@@ -1406,51 +1460,26 @@ int oppx_fempic_step_fused_rec(int64_t n_max, const int32_t* perm, const double*
     StepArgs P;
     int rc = fill_args(P, parts, mesh, dh, counters);
     if (rc) return rc;
-    OPPX_ARG_CHECK(perm ? (src_rec && src_lc) : (!src_rec && !src_lc),
-                   "oppx_fempic_step_fused_rec: src_rec and src_lc go with perm (all three or none)");
-    OPPX_ARG_CHECK(perm || rec_out, "oppx_fempic_step_fused_rec: neither records in nor records out");
-    OPPX_ARG_CHECK(parts->lc && parts->p2c && (rec_out || (parts->pos && parts->vel)) && (perm || (parts->pos && parts->vel)),
-                   "oppx_fempic_step_fused_rec: particle arrays missing");
-    OPPX_ARG_CHECK(!rec_out || rec_out != src_rec, "oppx_fempic_step_fused_rec: rec_out must not be src_rec");
-    OPPX_ARG_CHECK(!perm || src_lc != parts->lc, "oppx_fempic_step_fused_rec: src_lc and parts->lc must be separate");
-    OPPX_ARG_CHECK(n_max >= 0 && n_max <= parts->stride, "oppx_fempic_step_fused_rec: n_max beyond capacity");
-    OPPX_ARG_CHECK(hole_bits != nullptr, "oppx_fempic_step_fused_rec: hole_bits required");
-    OPPX_ARG_CHECK(g_tuning.inline_hops >= 1 && work && work_bytes >= work_need(n_max),
-                   "oppx_fempic_step_fused_rec: needs the two-pass move (tuning.inline_hops >= 1 and its workspace)");
-    hipStream_t s = as_stream(stream);
-    if (n_max == 0) return OPPX_OK;
-    P.n_max = n_max;
     P.k = k;
     P.dt = dt;
-    P.hole_bits = hole_bits;
-    P.hops = hops;
-    P.charge = node_charge;
-    P.rec_out = rec_out;
-    attach_slots(P, work, work_bytes, s);
-    if (perm) {
-        P.perm = perm;
-        P.src_rec = src_rec;
-        P.src_lc = src_lc;
-        P.src_pos = nullptr;  // never read: records replace them
-        P.src_vel = nullptr;
-        P.src_p2c = nullptr;  // the sort wrote the sorted p2c into parts->p2c: read in place
-        if (dh) {
-            if (node_charge) launch_two_pass<true, true, true, false, true>(P, n_max, work, work_bytes, s);
-            else launch_two_pass<true, true, false, false, true>(P, n_max, work, work_bytes, s);
-        } else {
-            if (node_charge) launch_two_pass<true, false, true, false, true>(P, n_max, work, work_bytes, s);
-            else launch_two_pass<true, false, false, false, true>(P, n_max, work, work_bytes, s);
-        }
-    } else if (dh) {
-        if (node_charge) launch_two_pass<true, true, true, false>(P, n_max, work, work_bytes, s);
-        else launch_two_pass<true, true, false, false>(P, n_max, work, work_bytes, s);
-    } else {
-        if (node_charge) launch_two_pass<true, false, true, false>(P, n_max, work, work_bytes, s);
-        else launch_two_pass<true, false, false, false>(P, n_max, work, work_bytes, s);
-    }
-    fold_slots(P, s);
-    OPPX_LAUNCH_CHECK("fempic_step_fused_rec");
-    return OPPX_OK;
+    return step_rec<false>(P, "oppx_fempic_step_fused_rec", n_max, perm, src_rec, src_lc, parts, rec_out, dh != nullptr,
+                           hole_bits, work, work_bytes, hops, node_charge, as_stream(stream));
+}
+
+int oppx_fempic_batch_step_fused_rec(int64_t n_max, const int32_t* perm, const double* src_rec, const double* src_lc,
+                                     const oppx_fempic_parts* parts, double* rec_out, const oppx_tet_mesh* mesh,
+                                     const int32_t* cell_inst, const oppx_batch_inst* insts, int32_t n_insts,
+                                     oppx_move_counters* counters, uint64_t* hole_bits, void* work, size_t work_bytes,
+                                     int32_t* hops, double* node_charge, void* stream) {
+    StepArgs P;
+    int rc = fill_args(P, parts, mesh, nullptr, counters);
+    if (rc) return rc;
+    OPPX_ARG_CHECK(cell_inst && insts && n_insts > 0, "oppx_fempic_batch_step_fused_rec: instance tables required");
+    P.cell_inst = cell_inst;
+    P.insts = insts;
+    P.n_insts = n_insts;
+    return step_rec<true>(P, "oppx_fempic_batch_step_fused_rec", n_max, perm, src_rec, src_lc, parts, rec_out, true,
+                          hole_bits, work, work_bytes, hops, node_charge, as_stream(stream));
 }
 
 int oppx_gather_records(int64_t n_max, int64_t stride, oppx_move_counters* counters, const int32_t* perm,

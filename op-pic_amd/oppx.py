@@ -149,6 +149,9 @@ _SIGS = {
     "oppx_fempic_batch_step_fused": (c_int, [c_int64, POINTER(FempicParts), POINTER(TetMesh), c_void_p, c_void_p,
                                              c_int32, c_void_p, c_void_p, c_void_p, ctypes.c_size_t, c_void_p,
                                              c_void_p, c_void_p]),
+    "oppx_fempic_batch_step_fused_rec": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, POINTER(FempicParts),
+                                                 c_void_p, POINTER(TetMesh), c_void_p, c_void_p, c_int32, c_void_p,
+                                                 c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p]),
     "oppx_quad_cells_build": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "oppx_neso_step": (c_int, [c_int64, POINTER(NesoParts), c_void_p, c_int64, c_double, c_double, c_double, c_double,
                                c_int, c_void_p, c_void_p, c_void_p]),
@@ -632,9 +635,20 @@ class FempicDevice:
         self.staged = False
 
     def deposit_sorted_f32(self):
-        """FP32 Kahan deposit over the (cell-sorted) particles; returns the FP32 node charge tensor."""
-        self.materialize()
+        """FP32 Kahan deposit over the (cell-sorted) particles; returns the FP32 node charge tensor.
+        Over a pending (lazy) sort the particle dats stay as they are: the sort left the sorted p2c in the
+        swap buffer, the sorted lc is gathered into the swap lc buffer (which the next pass overwrites),
+        and the deposit reads those two."""
         torch = self.torch
+        if getattr(self, "pending", False):
+            swap = self._swap_buffers()
+            check(lib().oppx_gather_dats(self.n_max, self.cap, self.counters.data_ptr(), self.perm.data_ptr(),
+                                         self._descs([self.lc], ("lc",)), self._descs(swap[2:3], ("lc",)), 1,
+                                         self.stream()), "gather_dats")
+            p2c, lc = swap[3], swap[2]
+        else:
+            self.materialize()
+            p2c, lc = self.p2c, self.lc
         if not hasattr(self, "node_start"):
             st, ref = node_cell_csr(self._mesh_host)
             self.node_start = torch.from_numpy(st).to(self.dev)
@@ -643,8 +657,8 @@ class FempicDevice:
             self.charge_f32 = torch.zeros(self.nn, dtype=torch.float32, device=self.dev)
             self.dep_bytes = int(lib().oppx_fempic_deposit_scratch_bytes(self.cap))
             self.dep_scratch = torch.zeros(self.dep_bytes, dtype=torch.uint8, device=self.dev)
-        check(lib().oppx_fempic_deposit_sorted_f32(self.n_max, self.cap, self.counters.data_ptr(), self.p2c.data_ptr(),
-                                                   self.lc.data_ptr(), self.nc, self.node_start.data_ptr(), None,
+        check(lib().oppx_fempic_deposit_sorted_f32(self.n_max, self.cap, self.counters.data_ptr(), p2c.data_ptr(),
+                                                   lc.data_ptr(), self.nc, self.node_start.data_ptr(), None,
                                                    self.node_ref.data_ptr(), self.nn, self.cell_partials.data_ptr(),
                                                    None, self.charge_f32.data_ptr(), self.dep_scratch.data_ptr(),
                                                    self.dep_bytes, self.stream()),
@@ -717,13 +731,10 @@ class FempicDevice:
                               swap[3].data_ptr())
         else:
             dst = self.parts
-        check(lib().oppx_fempic_step_fused_rec(
-            self.n_max, self.perm.data_ptr() if pending else None, rec_in.data_ptr() if pending else None,
-            self.lc.data_ptr() if pending else None, ctypes.byref(dst),
-            rec_out.data_ptr() if rec_out is not None else None, ctypes.byref(self.mesh), self._dh_ref(), k, dt,
-            self.counters.data_ptr(), self.hole_bits.data_ptr(), self.work.data_ptr(), self.work_bytes,
-            self.hops.data_ptr() if record_hops else None, self.charge.data_ptr() if deposit else None,
-            self.stream()), "fempic_step_fused_rec")
+        self._rec_call(self.perm.data_ptr() if pending else None, rec_in.data_ptr() if pending else None,
+                       self.lc.data_ptr() if pending else None, dst,
+                       rec_out.data_ptr() if rec_out is not None else None, k, dt,
+                       self.hops.data_ptr() if record_hops else None, self.charge.data_ptr() if deposit else None)
         if pending:
             cur = [self.pos, self.vel, self.lc, self.p2c]
             self.pos, self.vel, self.lc, self.p2c = swap
@@ -733,6 +744,12 @@ class FempicDevice:
         if stage:
             self._rec_cur = out_idx
         self.staged = stage
+
+    def _rec_call(self, perm, rec_in, src_lc, dst, rec_out, k, dt, hops, charge):
+        check(lib().oppx_fempic_step_fused_rec(
+            self.n_max, perm, rec_in, src_lc, ctypes.byref(dst), rec_out, ctypes.byref(self.mesh), self._dh_ref(),
+            k, dt, self.counters.data_ptr(), self.hole_bits.data_ptr(), self.work.data_ptr(), self.work_bytes, hops,
+            charge, self.stream()), "fempic_step_fused_rec")
 
     def step_fused(self, charge, mass, dt, record_hops=False, fill="holefill"):
         """push + direct hop + walk + deposit in one pass, then hole-fill (or a sort finalize)."""
@@ -856,13 +873,33 @@ class FempicBatch(FempicDevice):
         raw = np.frombuffer(bytes(table), np.uint8).copy()
         self.inst_table = torch.from_numpy(raw).to(self.dev)
 
-    def pass_batch(self, deposit=False, record_hops=False):
-        """push + direct hop + walk (+ FP64 deposit into the concatenated node charge) of every instance."""
+    def pass_batch(self, deposit=False, record_hops=False, stage=False):
+        """push + direct hop + walk (+ FP64 deposit into the concatenated node charge) of every instance.
+        stage=True (with the two-pass move): the new positions and velocities go to 48-byte records, and a
+        pass after a lazy sort of staged records reads them through the permutation, as
+        FempicDevice.pass_fused does for one instance."""
+        pending, staged = self.pending, self.staged
+        stage = stage and get_tuning()["inline_hops"] >= 1
+        if staged and not pending:
+            self.materialize()
+            staged = False
+        if stage or (pending and staged):
+            self._pass_records(0.0, 0.0, record_hops, deposit, stage, pending)
+            return
+        self.materialize()
         check(lib().oppx_fempic_batch_step_fused(
             self.n_max, ctypes.byref(self.parts), ctypes.byref(self.mesh), self.cell_inst.data_ptr(),
             self.inst_table.data_ptr(), self.n_inst, self.counters.data_ptr(), self.hole_bits.data_ptr(),
             self.work.data_ptr(), self.work_bytes, self.hops.data_ptr() if record_hops else None,
             self.charge.data_ptr() if deposit else None, self.stream()), "fempic_batch_step_fused")
+
+    def _rec_call(self, perm, rec_in, src_lc, dst, rec_out, k, dt, hops, charge):
+        # k and dt come from the instance table
+        check(lib().oppx_fempic_batch_step_fused_rec(
+            self.n_max, perm, rec_in, src_lc, ctypes.byref(dst), rec_out, ctypes.byref(self.mesh),
+            self.cell_inst.data_ptr(), self.inst_table.data_ptr(), self.n_inst, self.counters.data_ptr(),
+            self.hole_bits.data_ptr(), self.work.data_ptr(), self.work_bytes, hops, charge, self.stream()),
+            "fempic_batch_step_fused_rec")
 
     def step_batch(self, deposit="f32"):
         """One ensemble step: reset, batched push/move, Sort_All finalize, deposit.
@@ -873,6 +910,29 @@ class FempicBatch(FempicDevice):
         self.sort(drop_removed=True)
         if deposit == "f32":
             self.deposit_sorted_f32()
+
+    def steps_batch(self, steps, deposit="f32", lazy=True, mark=None):
+        """`steps` ensemble steps, each one step_batch's work, ending with the particle dats sorted in place.
+        lazy (with the two-pass move): every sort computes only the permutation and the sorted p2c; each
+        pass writes positions and velocities as 48-byte records and the next pass reads them through the
+        permutation; the FP32 deposit reads the sorted p2c and a gathered lc; after the last step the
+        permutation is applied (materialize). The particle order, cells, positions, velocities, lc and
+        charges after each step equal step_batch's. An even `steps` ends on the starting buffers, so the
+        sequence can be captured in a graph and replayed. mark(phase) (optional) is called before each
+        step (0), after its begin and pass (1), after its sort (2) and after its deposit (3)."""
+        lazy = lazy and get_tuning()["inline_hops"] >= 1
+        note = mark or (lambda phase: None)
+        for _ in range(steps):
+            note(0)
+            self.step_begin()
+            self.pass_batch(deposit=(deposit == "f64"), stage=lazy)
+            note(1)
+            self.sort(drop_removed=True, lazy=lazy)
+            note(2)
+            if deposit == "f32":
+                self.deposit_sorted_f32()
+            note(3)
+        self.materialize()
 
     def split(self):
         """Per-instance state (host): the concatenated particles are sorted by concatenated cell."""

@@ -144,3 +144,65 @@ def test_batch_graph_replays_match_eager():
             assert x["n"] == y["n"], f"replay {rep} instance {i}"
             for key in ("p2c", "pos", "vel", "lc", "charge_f32"):
                 assert np.array_equal(x[key], y[key]), f"replay {rep} instance {i} {key}"
+
+
+@pytest.mark.parametrize("deposit", ["f32", "f64"])
+def test_lazy_sequence_equals_eager_steps(tuning, deposit):
+    """steps_batch (lazy sorts, 48-byte records read through the permutation, the FP32 deposit over the
+    pending sort) against step_batch step by step: every step's charge, and the state after the sequence.
+    The tile-walk tunings take the eager path inside steps_batch (the records need the two-pass move)."""
+    inst = make_instances(SPEC, seed=43)
+    lazy, eager = oppx.FempicBatch(inst), oppx.FempicBatch(inst)
+    for seq, n in enumerate((3, 1, 4)):
+        charges = []
+
+        def keep(phase):
+            if phase == 3:
+                charges.append((lazy.charge_f32 if deposit == "f32" else lazy.charge).clone())
+
+        print(f"[lazy] {tuning} {deposit} sequence {seq}: {n} steps", flush=True)
+        lazy.steps_batch(n, deposit=deposit, mark=keep)
+        assert not lazy.pending and not lazy.staged
+        for s in range(n):
+            eager.step_batch(deposit=deposit)
+            got = charges[s].cpu().numpy()
+            if deposit == "f32":
+                assert np.array_equal(got, eager.charge_f32.cpu().numpy()), f"sequence {seq} step {s}"
+            else:
+                exp = eager.charge.cpu().numpy()
+                assert np.abs(got - exp).max() <= F64_RTOL * max(np.abs(exp).max(), 1e-300), f"sequence {seq} step {s}"
+        assert lazy.read_counters()["faults"] == 0
+        a, e = lazy.download(), eager.download()
+        assert a["n"] == e["n"], f"sequence {seq}"
+        for key in ("p2c", "pos", "vel", "lc"):
+            assert np.array_equal(a[key], e[key]), f"sequence {seq} {key}"
+
+
+def test_lazy_sequence_graph_replays_match_eager():
+    """Two lazy ensemble steps captured as one graph (they end on the starting buffers), replayed several
+    times with eager work and synchronisation between replays, each replay against two eager steps."""
+    inst = make_instances(SPEC, seed=44)
+    eager = oppx.FempicBatch(inst)
+    g = oppx.FempicBatch(inst)
+    g.steps_batch(2)  # warm-up outside capture: allocations (records, swap buffers), occupancy queries
+    eager.step_batch()
+    eager.step_batch()
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph, stream=side):
+            g.steps_batch(2)
+    torch.cuda.synchronize()
+    for rep in range(4):
+        graph.replay()
+        torch.cuda.synchronize()
+        eager.step_batch()
+        eager.step_batch()
+        _ = torch.ones(1000, device="cuda").sum().item()  # unrelated eager work between replays
+        torch.cuda.synchronize()
+        a, e = g.split(), eager.split()
+        for i, (x, y) in enumerate(zip(a, e)):
+            assert x["n"] == y["n"], f"replay {rep} instance {i}"
+            for key in ("p2c", "pos", "vel", "lc", "charge_f32"):
+                assert np.array_equal(x[key], y[key]), f"replay {rep} instance {i} {key}"
