@@ -241,11 +241,12 @@ __device__ __forceinline__ void table_flush(DepositTable& T, double* __restrict_
 
 template <bool PUSH, bool MOVE, bool DH, bool DEPOSIT, bool STATS>
 __global__ void __launch_bounds__(kBlock) fempic_pass_kernel(StepArgs P, uint32_t nblocks) {
-    const uint32_t b = xcd_block(blockIdx.x, nblocks);
-    const int64_t i = (int64_t)b * kBlock + threadIdx.x;
     int64_t n = P.ctr->size;
     n = n < 0 ? 0 : (n > P.n_max ? P.n_max : n);  // never beyond the host's bound
-    if ((int64_t)b * kBlock >= n) return;         // whole block past the live count
+    const uint32_t nlive = live_blocks(n, kBlock, nblocks);
+    if (blockIdx.x >= nlive) return;  // no block of live particles left for this workgroup
+    const uint32_t b = xcd_block(blockIdx.x, nlive);
+    const int64_t i = (int64_t)b * kBlock + threadIdx.x;
     const bool live = i < n;
     const int64_t S = P.stride;
 
@@ -371,13 +372,14 @@ struct WorkList {
 
 template <int BS, bool PUSH, bool DH, bool DEPOSIT, bool BATCH, bool GATHER>
 __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t nblocks, int inline_hops, WorkList W) {
-    const uint32_t b = xcd_block(blockIdx.x, nblocks);
-    const int64_t i = (int64_t)b * BS + threadIdx.x;
     int64_t n = P.ctr->size;
     n = n < 0 ? 0 : (n > P.n_max ? P.n_max : n);
-    // blocks past the live count (the grid is sized by the host's bound) leave at once; the
-    // removal-bitmap words past the live count are never read
-    if ((int64_t)b * BS >= n) return;
+    // workgroups past the blocks of live particles (the grid is sized by the host's bound) leave at
+    // once; the removal-bitmap words past the live count are never read
+    const uint32_t nlive = live_blocks(n, BS, nblocks);
+    if (blockIdx.x >= nlive) return;
+    const uint32_t b = xcd_block(blockIdx.x, nlive);
+    const int64_t i = (int64_t)b * BS + threadIdx.x;
     const bool live = i < n;
     const int64_t S = P.stride;
     const int lane = threadIdx.x & (kWave - 1);
@@ -637,12 +639,13 @@ __device__ __forceinline__ int quad_hop(const StepArgs& P, int r, double x, doub
 
 template <bool PUSH, bool DH, bool DEPOSIT, bool BATCH, bool GATHER>
 __global__ void __launch_bounds__(kQuadBlock) fempic_quad_kernel(StepArgs P, uint32_t nblocks, int inline_hops, WorkList W) {
-    const uint32_t b = xcd_block(blockIdx.x, nblocks);
-    const int r = threadIdx.x & 3;
-    const int64_t i = (int64_t)b * kQuadParts + (threadIdx.x >> 2);
     int64_t n = P.ctr->size;
     n = n < 0 ? 0 : (n > P.n_max ? P.n_max : n);
-    if ((int64_t)b * kQuadParts >= n) return;  // whole block past the live count
+    const uint32_t nlive = live_blocks(n, kQuadParts, nblocks);
+    if (blockIdx.x >= nlive) return;  // no block of live particles left for this workgroup
+    const uint32_t b = xcd_block(blockIdx.x, nlive);
+    const int r = threadIdx.x & 3;
+    const int64_t i = (int64_t)b * kQuadParts + (threadIdx.x >> 2);
     const bool live = i < n;                     // quad-uniform
     const int64_t S = P.stride;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -873,8 +876,10 @@ __global__ void __launch_bounds__(kBlock) fempic_tile_kernel(StepArgs P, uint32_
     // Block tiles (kWavesPerBlock consecutive wave tiles) are split into one contiguous range per
     // XCD. A plain launch maps block ids onto those ranges; a persistent launch keeps a resident grid
     // whose blocks take the next block tile of their XCD's range from a queue in the counters.
+    // Only the block tiles that hold live particles are split, so every XCD gets 1/8 of them.
     const uint32_t xcd = blockIdx.x % kXcds;
-    const uint32_t q8 = nblocks / kXcds, r8 = nblocks % kXcds;
+    const uint32_t nlive = live_blocks(n, (int64_t)kWavesPerBlock * tile, nblocks);
+    const uint32_t q8 = nlive / kXcds, r8 = nlive % kXcds;
     const uint32_t range0 = xcd * q8 + (xcd < r8 ? xcd : r8), range_n = q8 + (xcd < r8 ? 1u : 0u);
 
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -891,8 +896,8 @@ __global__ void __launch_bounds__(kBlock) fempic_tile_kernel(StepArgs P, uint32_
             if (t >= range_n) break;  // uniform over the block: every wave leaves here
             b = range0 + t;
         } else {
-            if (round > 0) break;
-            b = xcd_block(blockIdx.x, nblocks);
+            if (round > 0 || blockIdx.x >= nlive) break;
+            b = xcd_block(blockIdx.x, nlive);
         }
         // block tiles past the live count hold no particle (and later ones of this range neither)
         if ((int64_t)b * kWavesPerBlock * tile >= n) break;

@@ -66,6 +66,15 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t pid, uint32_t nblocks) {
     return x * q + (x < r ? x : r) + k;
 }
 
+// Blocks of per_block particles that hold live particles (n of them), at most nblocks. A grid sized by
+// the host's upper bound on the live count lets workgroups 0 .. live_blocks - 1, which the hardware
+// dispatches first, take those blocks through xcd_block(pid, live_blocks): every XCD then gets 1/8 of the
+// live particles however far the count has fallen below the bound, and the remaining workgroups leave.
+__device__ __forceinline__ uint32_t live_blocks(int64_t n, int64_t per_block, uint32_t nblocks) {
+    const int64_t l = (n + per_block - 1) / per_block;
+    return l < (int64_t)nblocks ? (uint32_t)l : nblocks;
+}
+
 __device__ __forceinline__ int wave_sum_i32(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
