@@ -28,6 +28,7 @@ extern "C" {
 #define OPPX_ERR_HIP 2    /* a HIP runtime call failed */
 #define OPPX_ERR_NOMEM 3  /* device or host allocation failed, or admission budget exceeded */
 #define OPPX_ERR_STATE 4  /* call out of order (library not initialised, handle busy, ...) */
+#define OPPX_ERR_NOTFOUND 5  /* a named item (an HDF5 dataset) is not in the file */
 
 /* MAX_CELL_INDEX of opp_lib/include/opp_defs.h:150: the p2c value of a removed particle. */
 #define OPPX_MAX_CELL_INDEX 2147483647
@@ -215,6 +216,23 @@ int oppx_mesh_file_load(const char* mesh_path, const char* inlet_path, const cha
                         int32_t* node_type, double* node_vol, int32_t* c2n, int32_t* c2c, double* c_det,
                         double* c_vol, int32_t* if2c, int32_t* if2n, double* if_u, double* if_v, double* if_normal,
                         double* if_area);
+
+/* HDF5 files as OP-PIC's opp_decl_set_hdf5 / opp_decl_map_hdf5 / opp_decl_dat_hdf5 read them
+ * (opp_lib/src/hdf5/opp_mpi_hdf5.cpp:126-517), without an HDF5 library: superblock version 0 or 1,
+ * symbol-table groups, version-1 object headers, contiguous or compact datasets of little-endian 4- or
+ * 8-byte integers or floats, no filters (anything else returns OPPX_ERR_ARG with the reason in
+ * oppx_last_error). `name` is a path from the root group ("c_det", "group/name"); a missing dataset
+ * returns OPPX_ERR_NOTFOUND. oppx_h5_dataset_info gives the stored element type and the extents as
+ * the reference takes them (rows = first extent, dim = second extent or 1; a scalar is 1 x 1).
+ * oppx_h5_read copies the first `rows` rows (dim must match) into `out` as `type`, converting like
+ * HDF5's conversion to a memory type (integers to floats round, floats to integers truncate; a value
+ * out of the target's range is an error). Host-side; thread-safe. */
+#define OPPX_H5_INT32 1
+#define OPPX_H5_INT64 2
+#define OPPX_H5_FLOAT32 3
+#define OPPX_H5_FLOAT64 4
+int oppx_h5_dataset_info(const char* path, const char* name, int32_t* type, int64_t* rows, int64_t* dim);
+int oppx_h5_read(const char* path, const char* name, int32_t type, int64_t rows, int64_t dim, void* out);
 /* Overlay dims of CellMapper::init_host (opp_lib/include/opp_direct_hop_core.h:317-336). */
 int oppx_dh_grid_dims(const double bmin[3], const double bmax[3], double gs, int64_t dims[3]);
 /* Overlay contents of CellMapper::generateStructuredMesh (opp_lib/src/seq/opp_direct_hop_seq.cpp:134-265),

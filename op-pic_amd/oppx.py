@@ -188,6 +188,8 @@ _SIGS = {
                                      c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "oppx_mesh_file_sizes": (c_int, [ctypes.c_char_p, ctypes.c_char_p, c_void_p, c_void_p, c_void_p]),
     "oppx_mesh_file_load": (c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p] + [c_void_p] * 13),
+    "oppx_h5_dataset_info": (c_int, [ctypes.c_char_p, ctypes.c_char_p, c_void_p, c_void_p, c_void_p]),
+    "oppx_h5_read": (c_int, [ctypes.c_char_p, ctypes.c_char_p, c_int32, c_int64, c_int64, c_void_p]),
     "oppx_holefill_scratch_bytes": (c_size_t, [c_int64]),
     "oppx_holefill": (c_int, [c_int64, c_int64, c_void_p, c_void_p, POINTER(DatDesc), c_int, c_void_p,
                               c_size_t, c_void_p]),
@@ -294,6 +296,27 @@ def box12_particles(n: int, seed: int, sigma: float, m: dict):
                                      _ptr(m["c_det"]), _ptr(m["c_vol"]), _ptr(pos), _ptr(vel), _ptr(p2c)),
           "box12_particles")
     return pos, vel, p2c
+
+
+H5_TYPES = {1: np.int32, 2: np.int64, 3: np.float32, 4: np.float64}  # OPPX_H5_* element types
+H5_CODES = {np.dtype(v).name: k for k, v in H5_TYPES.items()}
+
+
+def h5_dataset(path: str, name: str, dtype=None):
+    """One dataset of an OP-PIC HDF5 file (oppx_h5_read) as a (rows, dim) array, in its stored element
+    type or converted to dtype (int32, int64, float32 or float64). None when the file has no such
+    dataset."""
+    L = lib()
+    t, rows, dim = c_int32(), c_int64(), c_int64()
+    rc = L.oppx_h5_dataset_info(path.encode(), name.encode(), ctypes.byref(t), ctypes.byref(rows), ctypes.byref(dim))
+    if rc == 5:  # OPPX_ERR_NOTFOUND
+        return None
+    check(rc, "h5_dataset_info")
+    want = np.dtype(dtype) if dtype is not None else np.dtype(H5_TYPES[t.value])
+    code = {np.dtype(v): k for k, v in H5_TYPES.items()}[want]
+    out = np.zeros((rows.value, dim.value), want)
+    check(L.oppx_h5_read(path.encode(), name.encode(), code, rows.value, dim.value, _ptr(out)), "h5_read")
+    return out
 
 
 def mesh_from_files(mesh_path: str, inlet_path: str | None = None, wall_path: str | None = None) -> dict:

@@ -5,12 +5,14 @@
 // the loop's OPP_MAX globals. Built against op-pic_amd/include/opp_lib.h and linked with
 // libopp_mi355x.so: the drop-in check of the injection and field-side loop symbols.
 //
-// usage: fempic_loop_app <params> <in_dir> <out_dir>   (same files as oracle/ref/fempic_loop_ref_main.cpp)
+// usage: fempic_loop_app <params> <in_dir> <out_dir>   (same files as oracle/ref/fempic_loop_ref_main.cpp;
+// with `STRING hdf_filename = <file>` in the params the mesh comes from that HDF5 file instead)
 
 #include <cstdio>
 #include <string>
 #include <vector>
 
+#include "opp_hdf5.h"
 #include "opp_lib.h"
 
 template <typename T>
@@ -54,54 +56,102 @@ int main(int argc, char** argv) {
     const opp_point bmax(opp_params->get<OPP_REAL>("bmaxx"), opp_params->get<OPP_REAL>("bmaxy"),
                          opp_params->get<OPP_REAL>("bmaxz"));
 
-    auto c2n = read_arr<int>(in + "/c2n.bin", 4 * (size_t)n_cells);
-    auto c2c = read_arr<int>(in + "/c2c.bin", 4 * (size_t)n_cells);
-    auto c_det = read_arr<double>(in + "/c_det.bin", 16 * (size_t)n_cells);
-    auto c_vol = read_arr<double>(in + "/c_vol.bin", (size_t)n_cells);
-    auto c_ef = read_arr<double>(in + "/c_ef.bin", 3 * (size_t)n_cells);
-    auto c_sd = read_arr<double>(in + "/c_sd.bin", 12 * (size_t)n_cells);
-    auto n_vol = read_arr<double>(in + "/n_vol.bin", (size_t)n_nodes);
-    auto n_type = read_arr<int>(in + "/n_type.bin", (size_t)n_nodes);
-    auto n_pot = read_arr<double>(in + "/n_pot.bin", (size_t)n_nodes);
-    auto if2c = read_arr<int>(in + "/if2c.bin", (size_t)n_faces);
-    auto if_u = read_arr<double>(in + "/if_u.bin", 3 * (size_t)n_faces);
-    auto if_v = read_arr<double>(in + "/if_v.bin", 3 * (size_t)n_faces);
-    auto if_norm = read_arr<double>(in + "/if_norm.bin", 3 * (size_t)n_faces);
-    auto if_npos = read_arr<double>(in + "/if_npos.bin", 9 * (size_t)n_faces);
-    auto if_dist = read_arr<int>(in + "/if_dist.bin", (size_t)n_faces);
-    auto rnd = read_arr<double>(in + "/rand.bin", 2 * (size_t)n_rand);
-    std::vector<int> c_id(n_cells);
-    for (int i = 0; i < n_cells; i++) c_id[i] = i;
-    std::vector<double> zeros(n_nodes, 0.0);
+    // the mesh and the particle sets: from an OP-PIC HDF5 file when hdf_filename is given, with the
+    // declarations of the generated HDF5 main (apps_codegen/app_fempic_cg/fempic_hdf5_opp.cpp:79-118),
+    // otherwise from the binary arrays in in_dir
+    std::string h5 = opp_params->get<OPP_STRING>("hdf_filename", false);
+    if (h5 == "NOT FOUND") h5.clear();  // the value Params::get gives for an absent STRING key
+    opp_set node_set, cell_set, iface_set, particle_set, dummy_part_set;
+    opp_map c2n_map, c2c_map, if2c_map, p2c_map;
+    opp_dat c_det_d, c_vol_d, c_ef_d, c_sd_d, c_gbl_id, n_volume, n_potential, n_charge_den, n_type_d, n_bnd_pot;
+    opp_dat if_v_norm, if_u_norm, if_norm_d, if_distrib, if_n_pos, p_pos, p_vel, p_lc, dp_rand;
+    if (!h5.empty()) {
+        const char* file = h5.c_str();
+        node_set = opp_decl_set_hdf5(file, "mesh_nodes");
+        cell_set = opp_decl_set_hdf5(file, "mesh_cells");
+        iface_set = opp_decl_set_hdf5(file, "inlet_faces_cells");
+        particle_set = opp_decl_particle_set_hdf5(file, "particles", cell_set);
+        dummy_part_set = opp_decl_particle_set_hdf5(file, "dummy particles", cell_set);
+        c2n_map = opp_decl_map_hdf5(cell_set, node_set, 4, file, "c_v_n_map");
+        c2c_map = opp_decl_map_hdf5(cell_set, cell_set, 4, file, "c_v_c_map");
+        if2c_map = opp_decl_map_hdf5(iface_set, cell_set, 1, file, "if_v_c_map");
+        opp_decl_map_hdf5(iface_set, node_set, 3, file, "if_v_n_map");
+        c_det_d = opp_decl_dat_hdf5(cell_set, 16, DT_REAL, file, "c_det");
+        c_vol_d = opp_decl_dat_hdf5(cell_set, 1, DT_REAL, file, "c_volume");
+        c_ef_d = opp_decl_dat_hdf5(cell_set, 3, DT_REAL, file, "c_ef");
+        c_sd_d = opp_decl_dat_hdf5(cell_set, 12, DT_REAL, file, "c_shape_deri");
+        c_gbl_id = opp_decl_dat_hdf5(cell_set, 1, DT_INT, file, "c_gbl_id");
+        opp_decl_dat_hdf5(cell_set, 1, DT_INT, file, "c_colors");
+        opp_decl_dat_hdf5(cell_set, 3, DT_REAL, file, "c_centroids");
+        n_volume = opp_decl_dat_hdf5(node_set, 1, DT_REAL, file, "n_vol");
+        n_potential = opp_decl_dat_hdf5(node_set, 1, DT_REAL, file, "n_potential");
+        n_charge_den = opp_decl_dat_hdf5(node_set, 1, DT_REAL, file, "n_charge_den");
+        opp_decl_dat_hdf5(node_set, 3, DT_REAL, file, "n_pos");
+        n_type_d = opp_decl_dat_hdf5(node_set, 1, DT_INT, file, "n_type");
+        n_bnd_pot = opp_decl_dat_hdf5(node_set, 1, DT_REAL, file, "n_bnd_pot");
+        if_v_norm = opp_decl_dat_hdf5(iface_set, 3, DT_REAL, file, "iface_v_norm");
+        if_u_norm = opp_decl_dat_hdf5(iface_set, 3, DT_REAL, file, "iface_u_norm");
+        if_norm_d = opp_decl_dat_hdf5(iface_set, 3, DT_REAL, file, "iface_norm");
+        opp_decl_dat_hdf5(iface_set, 1, DT_REAL, file, "iface_area");
+        if_distrib = opp_decl_dat_hdf5(iface_set, 1, DT_INT, file, "iface_dist");
+        if_n_pos = opp_decl_dat_hdf5(iface_set, 9, DT_REAL, file, "iface_n_pos");
+        p_pos = opp_decl_dat_hdf5(particle_set, 3, DT_REAL, file, "part_position");
+        p_vel = opp_decl_dat_hdf5(particle_set, 3, DT_REAL, file, "part_velocity");
+        p_lc = opp_decl_dat_hdf5(particle_set, 4, DT_REAL, file, "part_lc");
+        p2c_map = opp_decl_map(particle_set, cell_set, 1, nullptr, "part_mesh_rel");
+        dp_rand = opp_decl_dat_hdf5(dummy_part_set, 2, DT_REAL, file, "dummy_part_rand");
+        if (node_set->size != n_nodes || cell_set->size != n_cells || iface_set->size != n_faces)
+            opp_abort("HDF5 set sizes differ from the parameters");
+    } else {
+        auto c2n = read_arr<int>(in + "/c2n.bin", 4 * (size_t)n_cells);
+        auto c2c = read_arr<int>(in + "/c2c.bin", 4 * (size_t)n_cells);
+        auto c_det = read_arr<double>(in + "/c_det.bin", 16 * (size_t)n_cells);
+        auto c_vol = read_arr<double>(in + "/c_vol.bin", (size_t)n_cells);
+        auto c_ef = read_arr<double>(in + "/c_ef.bin", 3 * (size_t)n_cells);
+        auto c_sd = read_arr<double>(in + "/c_sd.bin", 12 * (size_t)n_cells);
+        auto n_vol = read_arr<double>(in + "/n_vol.bin", (size_t)n_nodes);
+        auto n_type = read_arr<int>(in + "/n_type.bin", (size_t)n_nodes);
+        auto n_pot = read_arr<double>(in + "/n_pot.bin", (size_t)n_nodes);
+        auto if2c = read_arr<int>(in + "/if2c.bin", (size_t)n_faces);
+        auto if_u = read_arr<double>(in + "/if_u.bin", 3 * (size_t)n_faces);
+        auto if_v = read_arr<double>(in + "/if_v.bin", 3 * (size_t)n_faces);
+        auto if_norm = read_arr<double>(in + "/if_norm.bin", 3 * (size_t)n_faces);
+        auto if_npos = read_arr<double>(in + "/if_npos.bin", 9 * (size_t)n_faces);
+        auto if_dist = read_arr<int>(in + "/if_dist.bin", (size_t)n_faces);
+        std::vector<int> c_id(n_cells);
+        for (int i = 0; i < n_cells; i++) c_id[i] = i;
+        std::vector<double> zeros(n_nodes, 0.0);
 
-    opp_set node_set = opp_decl_set(n_nodes, "mesh_nodes");
-    opp_set cell_set = opp_decl_set(n_cells, "mesh_cells");
-    opp_set iface_set = opp_decl_set(n_faces, "inlet_faces_cells");
-    opp_set particle_set = opp_decl_particle_set("particles", cell_set);
-    opp_set dummy_part_set = opp_decl_particle_set("dummy particles", cell_set);
-    opp_map c2n_map = opp_decl_map(cell_set, node_set, 4, c2n.data(), "c_v_n_map");
-    opp_map c2c_map = opp_decl_map(cell_set, cell_set, 4, c2c.data(), "c_v_c_map");
-    opp_map if2c_map = opp_decl_map(iface_set, cell_set, 1, if2c.data(), "if_v_c_map");
-    opp_dat c_det_d = opp_decl_dat(cell_set, 16, DT_REAL, c_det.data(), "c_det");
-    opp_dat c_vol_d = opp_decl_dat(cell_set, 1, DT_REAL, c_vol.data(), "c_volume");
-    opp_dat c_ef_d = opp_decl_dat(cell_set, 3, DT_REAL, c_ef.data(), "c_ef");
-    opp_dat c_sd_d = opp_decl_dat(cell_set, 12, DT_REAL, c_sd.data(), "c_shape_deri");
-    opp_dat c_gbl_id = opp_decl_dat(cell_set, 1, DT_INT, c_id.data(), "c_gbl_id");
-    opp_dat n_volume = opp_decl_dat(node_set, 1, DT_REAL, n_vol.data(), "n_vol");
-    opp_dat n_potential = opp_decl_dat(node_set, 1, DT_REAL, n_pot.data(), "n_potential");
-    opp_dat n_charge_den = opp_decl_dat(node_set, 1, DT_REAL, zeros.data(), "n_charge_den");
-    opp_dat n_type_d = opp_decl_dat(node_set, 1, DT_INT, n_type.data(), "n_type");
-    opp_dat n_bnd_pot = opp_decl_dat(node_set, 1, DT_REAL, zeros.data(), "n_bnd_pot");
-    opp_dat if_v_norm = opp_decl_dat(iface_set, 3, DT_REAL, if_v.data(), "iface_v_norm");
-    opp_dat if_u_norm = opp_decl_dat(iface_set, 3, DT_REAL, if_u.data(), "iface_u_norm");
-    opp_dat if_norm_d = opp_decl_dat(iface_set, 3, DT_REAL, if_norm.data(), "iface_norm");
-    opp_dat if_distrib = opp_decl_dat(iface_set, 1, DT_INT, if_dist.data(), "iface_dist");
-    opp_dat if_n_pos = opp_decl_dat(iface_set, 9, DT_REAL, if_npos.data(), "iface_n_pos");
-    opp_dat p_pos = opp_decl_dat(particle_set, 3, DT_REAL, nullptr, "p_position");
-    opp_dat p_vel = opp_decl_dat(particle_set, 3, DT_REAL, nullptr, "p_velocity");
-    opp_dat p_lc = opp_decl_dat(particle_set, 4, DT_REAL, nullptr, "p_lc");
-    opp_map p2c_map = opp_decl_map(particle_set, cell_set, 1, nullptr, "p2c_map");
-    opp_dat dp_rand = opp_decl_dat(dummy_part_set, 2, DT_REAL, nullptr, "dummy_part_rand");
+        node_set = opp_decl_set(n_nodes, "mesh_nodes");
+        cell_set = opp_decl_set(n_cells, "mesh_cells");
+        iface_set = opp_decl_set(n_faces, "inlet_faces_cells");
+        particle_set = opp_decl_particle_set("particles", cell_set);
+        dummy_part_set = opp_decl_particle_set("dummy particles", cell_set);
+        c2n_map = opp_decl_map(cell_set, node_set, 4, c2n.data(), "c_v_n_map");
+        c2c_map = opp_decl_map(cell_set, cell_set, 4, c2c.data(), "c_v_c_map");
+        if2c_map = opp_decl_map(iface_set, cell_set, 1, if2c.data(), "if_v_c_map");
+        c_det_d = opp_decl_dat(cell_set, 16, DT_REAL, c_det.data(), "c_det");
+        c_vol_d = opp_decl_dat(cell_set, 1, DT_REAL, c_vol.data(), "c_volume");
+        c_ef_d = opp_decl_dat(cell_set, 3, DT_REAL, c_ef.data(), "c_ef");
+        c_sd_d = opp_decl_dat(cell_set, 12, DT_REAL, c_sd.data(), "c_shape_deri");
+        c_gbl_id = opp_decl_dat(cell_set, 1, DT_INT, c_id.data(), "c_gbl_id");
+        n_volume = opp_decl_dat(node_set, 1, DT_REAL, n_vol.data(), "n_vol");
+        n_potential = opp_decl_dat(node_set, 1, DT_REAL, n_pot.data(), "n_potential");
+        n_charge_den = opp_decl_dat(node_set, 1, DT_REAL, zeros.data(), "n_charge_den");
+        n_type_d = opp_decl_dat(node_set, 1, DT_INT, n_type.data(), "n_type");
+        n_bnd_pot = opp_decl_dat(node_set, 1, DT_REAL, zeros.data(), "n_bnd_pot");
+        if_v_norm = opp_decl_dat(iface_set, 3, DT_REAL, if_v.data(), "iface_v_norm");
+        if_u_norm = opp_decl_dat(iface_set, 3, DT_REAL, if_u.data(), "iface_u_norm");
+        if_norm_d = opp_decl_dat(iface_set, 3, DT_REAL, if_norm.data(), "iface_norm");
+        if_distrib = opp_decl_dat(iface_set, 1, DT_INT, if_dist.data(), "iface_dist");
+        if_n_pos = opp_decl_dat(iface_set, 9, DT_REAL, if_npos.data(), "iface_n_pos");
+        p_pos = opp_decl_dat(particle_set, 3, DT_REAL, nullptr, "p_position");
+        p_vel = opp_decl_dat(particle_set, 3, DT_REAL, nullptr, "p_velocity");
+        p_lc = opp_decl_dat(particle_set, 4, DT_REAL, nullptr, "p_lc");
+        p2c_map = opp_decl_map(particle_set, cell_set, 1, nullptr, "p2c_map");
+        dp_rand = opp_decl_dat(dummy_part_set, 2, DT_REAL, nullptr, "dummy_part_rand");
+    }
+    auto rnd = read_arr<double>(in + "/rand.bin", 2 * (size_t)n_rand);
 
     opp_decl_const<OPP_REAL>(1, &spwt, "CONST_spwt");
     opp_decl_const<OPP_REAL>(1, &ion_velocity, "CONST_ion_velocity");

@@ -171,13 +171,16 @@ def test_neso_api_app_matches_reference(name):
         assert np.array_equal(rd("final_mdir", np.int32), g["out_final_mdir"].ravel())
 
 
+@pytest.mark.parametrize("source", ["bin", "hdf5"])
 @pytest.mark.parametrize("name", ["fempic_loop_small.npz", "fempic_loop_dh.npz", "fempic_loop_mesh12.npz"])
-def test_fempic_loop_app_matches_reference(name):
+def test_fempic_loop_app_matches_reference(name, source):
     """The main loop minus the field solve through the drop-in (injection, push, move, deposit, node
     charge density, electric field, OPP_MAX globals) against the reference seq build: particles,
     injected records, boundary potential, field and the field maxima bit-exact; charge density and
     its maximum within 1e-12 relative (FP64 atomic deposit). The mesh12 case runs on a 12-tet box read
-    through the mesh-file loader, with its node volumes, node types and inlet faces."""
+    through the mesh-file loader, with its node volumes, node types and inlet faces. source "hdf5":
+    the mesh and particle sets come from an OP-PIC HDF5 file holding the same arrays, declared as the
+    generated HDF5 main declares them (opp_decl_set_hdf5, opp_decl_map_hdf5, opp_decl_dat_hdf5)."""
     from test_oracle_loop import loop_mesh
 
     g = np.load(golden_path(name), allow_pickle=False)
@@ -202,6 +205,24 @@ def test_fempic_loop_app_matches_reference(name):
                  f"BOOL opp_global_move = {'true' if int(g['meta_dh']) else 'false'}",
                  "BOOL opp_dh_data_generate = true", "REAL opp_allocation_multiple = 1.1",
                  "BOOL opp_profile_all = false", "STRING opp_fill = HoleFill_All", "INT opp_fill_period = 4"]
+        if source == "hdf5":
+            import h5write
+
+            nc, nn, nf = m["n_cells"], m["n_nodes"], g["in_if2c"].size
+            one = lambda v: np.array([v], np.int32)
+            col = lambda a, d: np.ascontiguousarray(a).reshape(-1, d)
+            h5 = os.path.join(td, "mesh.h5")
+            h5write.write(h5, {
+                "mesh_nodes": one(nn), "mesh_cells": one(nc), "inlet_faces_cells": one(nf), "particles": one(0),
+                "dummy particles": one(0), "c_v_n_map": col(m["c2n"], 4), "c_v_c_map": col(m["c2c"], 4),
+                "if_v_c_map": col(g["in_if2c"], 1), "c_det": col(m["c_det"], 16), "c_volume": col(m["c_vol"], 1),
+                "c_ef": col(ef, 3), "c_shape_deri": col(g["in_sd"], 12), "c_gbl_id": col(np.arange(nc, dtype=np.int32), 1),
+                "n_vol": col(g["in_n_vol"], 1), "n_potential": col(g["in_n_pot"], 1),
+                "n_charge_den": np.zeros((nn, 1)), "n_pos": col(m["node_pos"], 3), "n_type": col(g["in_n_type"], 1),
+                "n_bnd_pot": np.zeros((nn, 1)), "iface_v_norm": col(g["in_if_v"], 3),
+                "iface_u_norm": col(g["in_if_u"], 3), "iface_norm": col(g["in_if_norm"], 3),
+                "iface_dist": col(g["in_if_dist"], 1), "iface_n_pos": col(g["in_if_npos"], 9)})
+            lines.append(f"STRING hdf_filename = {h5}")
         par = os.path.join(td, "case.param")
         with open(par, "w") as fh:
             fh.write("\n".join(lines) + "\n")
