@@ -366,7 +366,7 @@ int oppx_h5_read(const char* path, const char* name, int32_t type, int64_t rows,
         memset(out, 0, (size_t)n * (size_t)out_size);
         return OPPX_OK;
     }
-    if (!r.in(ds.data, (uint64_t)total * (uint64_t)ds.elem)) {
+    if ((uint64_t)total > m.n / (uint64_t)ds.elem || !r.in(ds.data, (uint64_t)total * (uint64_t)ds.elem)) {
         oppx::set_error(std::string("'") + name + "' data past the end of " + path);
         return OPPX_ERR_ARG;
     }
