@@ -10,6 +10,8 @@
 // c_det + a 32 B aux record per hop). Blocks are remapped so each XCD walks one contiguous
 // eighth of the particle range.
 
+#include <type_traits>
+
 #include "oppx_internal.h"
 
 namespace oppx {
@@ -335,13 +337,25 @@ enum HopResult { kHopInside = 0, kHopMove = 1, kHopExit = 2, kHopBad = 3 };
 
 // One evaluation of apps/app_fempic/kernels.h:106-161 at `cell`: inside -> kHopInside; otherwise
 // cell becomes the face neighbour of the most negative weight (first index wins ties), or
-// kMaxCell when that face is on the boundary.
-__device__ __forceinline__ int hop_eval(const StepArgs& P, double x, double y, double z, int& cell, double& l0,
-                                        double& l1, double& l2, double& l3) {
+// kMaxCell when that face is on the boundary. In two steps, the cell's records (its 128 B c_det line
+// and 32 B aux record) and the evaluation, so that the records can also come from LDS.
+struct HopRec {
+    double2 a[8];  // c_det: alpha, beta ; gamma, delta of vertices 0..3
+    double4 au;    // {c2, c2c[0..1], c2c[2..3], pad}
+};
+
+__device__ __forceinline__ void hop_load(const StepArgs& P, int cell, HopRec& R) {
     const double2* __restrict__ d = reinterpret_cast<const double2*>(P.det + 16 * (int64_t)cell);
-    const double2 a0 = d[0], a1 = d[1], a2 = d[2], a3 = d[3], a4 = d[4], a5 = d[5], a6 = d[6], a7 = d[7];
-    const double4 au = *reinterpret_cast<const double4*>(P.aux + cell);
-    const double c2 = au.x;
+#pragma unroll
+    for (int q = 0; q < 8; q++) R.a[q] = d[q];
+    R.au = *reinterpret_cast<const double4*>(P.aux + cell);
+}
+
+__device__ __forceinline__ int hop_calc(const StepArgs& P, const HopRec& R, double x, double y, double z, int& cell,
+                                        double& l0, double& l1, double& l2, double& l3) {
+    const double2 a0 = R.a[0], a1 = R.a[1], a2 = R.a[2], a3 = R.a[3], a4 = R.a[4], a5 = R.a[5], a6 = R.a[6],
+                  a7 = R.a[7];
+    const double c2 = R.au.x;
     l0 = c2 * fma(-a1.y, z, fma(a1.x, y, fma(-a0.y, x, a0.x)));
     l1 = c2 * fma(-a3.y, z, fma(a3.x, y, fma(-a2.y, x, a2.x)));
     l2 = c2 * fma(-a5.y, z, fma(a5.x, y, fma(-a4.y, x, a4.x)));
@@ -349,7 +363,7 @@ __device__ __forceinline__ int hop_eval(const StepArgs& P, double x, double y, d
     if ((l0 > -1e-10) && ((l0 - 1.0) < 1e-10) && (l1 > -1e-10) && ((l1 - 1.0) < 1e-10) && (l2 > -1e-10) &&
         ((l2 - 1.0) < 1e-10) && (l3 > -1e-10) && ((l3 - 1.0) < 1e-10))
         return kHopInside;
-    const int2 n01 = __builtin_bit_cast(int2, au.y), n23 = __builtin_bit_cast(int2, au.z);
+    const int2 n01 = __builtin_bit_cast(int2, R.au.y), n23 = __builtin_bit_cast(int2, R.au.z);
     int nb = n01.x;
     double ml = l0;
     if (l1 < ml) { ml = l1; nb = n01.y; }
@@ -363,6 +377,78 @@ __device__ __forceinline__ int hop_eval(const StepArgs& P, double x, double y, d
     cell = nb;
     return kHopMove;
 }
+
+__device__ __forceinline__ int hop_eval(const StepArgs& P, double x, double y, double z, int& cell, double& l0,
+                                        double& l1, double& l2, double& l3) {
+    HopRec R;
+    hop_load(P, cell, R);
+    return hop_calc(P, R, x, y, z, cell, l0, l1, l2, l3);
+}
+
+// How an evaluation's result ends a walk of the first pass (h evaluations so far): kWalkOn (still moving),
+// kWalkIn (inside its cell), kWalkOut (removed), kWalkBad (fault) or kWalkHandOff (to the worklist).
+enum WalkState { kWalkOn = 0, kWalkIn = 1, kWalkOut = 2, kWalkBad = 3, kWalkHandOff = 4 };
+__device__ __forceinline__ int walk_state(int r, int h, int inline_hops) {
+    if (r == kHopInside) return kWalkIn;
+    if (r == kHopExit) return kWalkOut;
+    if (r == kHopBad || h >= OPPX_MAX_HOPS) return kWalkBad;
+    return h >= inline_hops ? kWalkHandOff : kWalkOn;
+}
+
+// Cell records shared by a block's walks. Sorted particles cluster in a few cells, and a wave's
+// record loads are bound by the addresses the vector memory unit processes (one per lane per 16-byte
+// load), so in the first kCacheRounds evaluations each distinct cell of the block is fetched once,
+// 160 bytes by ten lanes, into LDS, and every lane of that cell reads it from there. A lane whose cell
+// finds no slot (kCacheProbe probes) loads its own records. One round of 64 slots (10.5 KB, inside the
+// deposit table's LDS) measured best: a second round's barriers cost more than its loads save.
+constexpr int kCacheBits = 6;
+constexpr int kCacheSlots = 1 << kCacheBits;
+constexpr int kCacheProbe = 16;
+constexpr int kCacheRounds = 1;
+struct CellCache {
+    int key[kCacheSlots];
+    double2 rec[kCacheSlots][10];  // c_det (8 pairs), aux (2 pairs)
+};
+
+__device__ __forceinline__ int cache_insert(CellCache& C, int cell) {
+    uint32_t h = ((uint32_t)cell * 2654435761u) >> (32 - kCacheBits);
+    for (int p = 0; p < kCacheProbe; p++) {
+        const int k = C.key[h];
+        if (k == cell) return (int)h;
+        if (k == -1) {
+            const int prev = atomicCAS(&C.key[h], -1, cell);
+            if (prev == -1 || prev == cell) return (int)h;
+        }
+        h = (h + 1) & (kCacheSlots - 1);
+    }
+    return -1;
+}
+
+__device__ __forceinline__ void cache_fill(const StepArgs& P, CellCache& C) {
+    for (int c = threadIdx.x; c < kCacheSlots * 10; c += blockDim.x) {
+        const int sl = c / 10, q = c - 10 * sl;
+        const int key = C.key[sl];
+        if (key >= 0) {
+            const double2* src = q < 8 ? reinterpret_cast<const double2*>(P.det + 16 * (int64_t)key) + q
+                                       : reinterpret_cast<const double2*>(P.aux + key) + (q - 8);
+            C.rec[sl][q] = *src;
+        }
+    }
+}
+
+__device__ __forceinline__ void cache_read(const CellCache& C, int sl, HopRec& R) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) R.a[q] = C.rec[sl][q];
+    const double2 u0 = C.rec[sl][8], u1 = C.rec[sl][9];
+    R.au = double4{u0.x, u0.y, u1.x, u1.y};
+}
+
+// LDS of a 256-thread first-pass block: the record staging, the cell cache and the deposit table in turn.
+union FirstLds {
+    DepositTable dep;
+    CellCache cache;
+};
+static_assert(sizeof(CellCache) <= sizeof(DepositTable), "the cell cache lives inside the deposit table's LDS");
 
 // Worklist of the second pass: one list per XCD, `cap` entries each, {particle, hops so far}.
 struct WorkList {
@@ -409,7 +495,10 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
     bool removed = false, pending = false, pushed = false, p2c_done = false, lc_done = false;
     int h = 0;
     double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
-    __shared__ DepositTable s_table;
+    constexpr bool CACHE = BS == kBlock;
+    using Lds = std::conditional_t<CACHE, FirstLds, DepositTable>;
+    __shared__ Lds s_lds;
+    DepositTable& s_table = *reinterpret_cast<DepositTable*>(&s_lds);
 
     // push: the particle's new position and velocity (or, for a particle this pass does not push, the
     // ones it carries over)
@@ -483,9 +572,9 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
     // wave's 64 records pass through LDS (the deposit table's storage, not yet in use) so that each of
     // the three 16-byte stores of a lane lands next to its neighbours' (1 KB contiguous per instruction)
     if (P.rec_out) {
-        if constexpr (BS * 48 <= (int)sizeof(DepositTable)) {
+        if constexpr (BS * 48 <= (int)sizeof(Lds)) {
             const int wave = threadIdx.x / kWave;
-            f64x2* slice = reinterpret_cast<f64x2*>(&s_table) + wave * (3 * kWave);
+            f64x2* slice = reinterpret_cast<f64x2*>(&s_lds) + wave * (3 * kWave);
             if (live) {
                 slice[3 * lane] = f64x2{x, y};
                 slice[3 * lane + 1] = f64x2{z, vx};
@@ -504,25 +593,61 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
             st_record(P.rec_out + 6 * i, x, y, z, vx, vy, vz);
         }
     }
-    if (DEPOSIT) table_clear(s_table);
+    if constexpr (!CACHE) {
+        if (DEPOSIT) table_clear(s_table);
+    }
 
-    // direct hop and walk
-    if (go) {
-        if (!fault) {
-            if (DH && (!BATCH || D.cell != nullptr)) {
-                cell = dh_lookup(D, x, y, z);
-                if (cell == kMaxCell) removed = true;
-                else if (!((unsigned)cell < (unsigned)P.n_cells)) fault = true;
-            }
-            while (!removed && !fault) {
-                const int r = hop_eval(P, x, y, z, cell, l0, l1, l2, l3);
+    // direct hop
+    bool moving = false;
+    int walk = kWalkOn;
+    if (go && !fault) {
+        if (DH && (!BATCH || D.cell != nullptr)) {
+            cell = dh_lookup(D, x, y, z);
+            if (cell == kMaxCell) removed = true;
+            else if (!((unsigned)cell < (unsigned)P.n_cells)) fault = true;
+        }
+        moving = !removed && !fault;
+    }
+    // walk: the block's first evaluations through the cell cache (every thread takes part in its barriers),
+    // then each lane on its own
+    if constexpr (CACHE) {
+        CellCache& C = s_lds.cache;
+        for (int round = 0; round < kCacheRounds; round++) {
+            __syncthreads();  // the LDS's last users (record staging, the previous round's readers) are done
+            for (int q = threadIdx.x; q < kCacheSlots; q += BS) C.key[q] = -1;
+            __syncthreads();
+            if (!__syncthreads_or(moving)) break;  // the same on every thread
+            const int sl = moving ? cache_insert(C, cell) : -1;
+            __syncthreads();
+            cache_fill(P, C);
+            __syncthreads();
+            if (moving) {
+                HopRec R;
+                if (sl >= 0) cache_read(C, sl, R);
+                else hop_load(P, cell, R);
+                const int r = hop_calc(P, R, x, y, z, cell, l0, l1, l2, l3);
                 h++;
-                if (r == kHopInside) break;
-                if (r == kHopExit) { removed = true; break; }
-                if (r == kHopBad || h >= OPPX_MAX_HOPS) { fault = true; break; }
-                if (h >= inline_hops) { pending = true; break; }
+                walk = walk_state(r, h, inline_hops);
+                moving = walk == kWalkOn;
             }
         }
+    }
+    while (moving) {
+        const int r = hop_eval(P, x, y, z, cell, l0, l1, l2, l3);
+        h++;
+        walk = walk_state(r, h, inline_hops);
+        moving = walk == kWalkOn;
+    }
+    removed = removed || walk == kWalkOut;
+    fault = fault || walk == kWalkBad;
+    pending = walk == kWalkHandOff;
+    if constexpr (CACHE) {
+        if (DEPOSIT) {
+            __syncthreads();  // the cache's readers are done with the LDS
+            table_clear(s_table);
+        }
+    }
+    if (go) {
         if (!fault) {
             if (!pending && !removed && h > 0) {
                 st_stream(P.lc + i, l0); st_stream(P.lc + S + i, l1);
