@@ -615,8 +615,7 @@ __global__ void __launch_bounds__(BS) fempic_first_kernel(StepArgs P, uint32_t n
         for (int round = 0; round < kCacheRounds; round++) {
             __syncthreads();  // the LDS's last users (record staging, the previous round's readers) are done
             for (int q = threadIdx.x; q < kCacheSlots; q += BS) C.key[q] = -1;
-            __syncthreads();
-            if (!__syncthreads_or(moving)) break;  // the same on every thread
+            if (!__syncthreads_or(moving)) break;  // a barrier (the keys are clear), the same on every thread
             const int sl = moving ? cache_insert(C, cell) : -1;
             __syncthreads();
             cache_fill(P, C);
