@@ -443,7 +443,8 @@ __device__ __forceinline__ void cache_read(const CellCache& C, int sl, HopRec& R
     R.au = double4{u0.x, u0.y, u1.x, u1.y};
 }
 
-// LDS of a 256-thread first-pass block: the record staging, the cell cache and the deposit table in turn.
+// LDS of a 256-thread first-pass block: the record staging, the cell cache and the deposit table in turn
+// (side by side, 22.8 KB instead of 12 KB, the pass measured 1-3% slower).
 union FirstLds {
     DepositTable dep;
     CellCache cache;
@@ -926,7 +927,18 @@ __global__ void __launch_bounds__(kBlock) fempic_walk_kernel(StepArgs P, WorkLis
 
     long long hsum = 0;
     int hmax = 0, nrem = 0, nfault = 0;
-    for (int64_t e = (int64_t)k_q * kBlock + threadIdx.x; e < m; e += (int64_t)nb_q * kBlock) {
+    // block-uniform rounds of kBlock entries; with the deposit, the LDS table is flushed after every
+    // round so that it never fills (a full table sends every further add to a global atomic)
+    for (int64_t base = (int64_t)k_q * kBlock; base < m; base += (int64_t)nb_q * kBlock) {
+        const int64_t e = base + threadIdx.x;
+        if (DEPOSIT && base != (int64_t)k_q * kBlock) {
+            __syncthreads();  // the previous round's adds are in the table
+            table_flush(s_table, P.charge);
+            __syncthreads();
+            table_clear(s_table);
+            __syncthreads();
+        }
+        if (e >= m) continue;
         const int2 w = W.entry[q * W.cap + e];
         const int64_t i = w.x;
         int h = w.y;
