@@ -46,28 +46,6 @@ __device__ __forceinline__ T ld_nt(const T* p) { return __builtin_nontemporal_lo
 template <typename T>
 __device__ __forceinline__ void st_nt(T* p, T v) { __builtin_nontemporal_store(v, p); }
 
-// The cell's 18 interpolation coefficients for the active lanes. In cell order nearly every wave's
-// particles share one cell: the wave then reads the coefficients once with scalar loads (its address
-// made wave-uniform by readfirstlane), which spares the vector memory unit 18 per-lane addresses;
-// a wave spanning several cells loads them lane by lane.
-__device__ __forceinline__ void load_interp(const CabArgs& P, int cell, double (&ip)[18]) {
-    const unsigned long long act = __ballot(1);
-    const int lead = __shfl(cell, __ffsll((long long)act) - 1);
-    if (__ballot(cell != lead) == 0ull) {
-        // the constant address space: read-only for the kernel, so a wave-uniform address becomes
-        // scalar loads (through the scalar cache, read only)
-        typedef const __attribute__((address_space(4))) double const_double;
-        const int uc = __builtin_amdgcn_readfirstlane(lead);
-        const const_double* src = (const const_double*)(P.interp + 18 * (int64_t)uc);
-#pragma unroll
-        for (int k = 0; k < 18; k++) ip[k] = src[k];
-    } else {
-        const double* __restrict__ src = P.interp + 18 * (int64_t)cell;
-#pragma unroll
-        for (int k = 0; k < 18; k++) ip[k] = src[k];
-    }
-}
-
 struct CabTable {
     int key[kCabTable];
     double val[kCabTable][12];
@@ -224,8 +202,7 @@ __global__ void __launch_bounds__(kBlock) cabana_move_deposit_kernel(CabArgs P) 
                 // first evaluation (OPP_DO_ONCE): Boris push, then either a move inside the cell
                 // or the first streak
                 if (!fault) {
-                    double ip[18];
-                    load_interp(P, cell, ip);
+                    const double* ip = P.interp + 18 * (int64_t)cell;
                     const double ex = ip[0], dexdy = ip[1], dexdz = ip[2], d2exdydz = ip[3];
                     const double ey = ip[4], deydz = ip[5], deydx = ip[6], d2eydzdx = ip[7];
                     const double ez = ip[8], dezdx = ip[9], dezdy = ip[10], d2ezdxdy = ip[11];
