@@ -14,8 +14,8 @@ oppx_tuning g_tuning = {256, 16, 0, 0, 16, 64, 256, 1};
 
 unsigned resident_blocks(const void* kernel, int block) {
     static int cus = 0;
-    static const void* kern_cache[64] = {nullptr};
-    static int block_cache[64] = {0}, per_cu_cache[64] = {0};
+    static const void* kern_cache[128] = {nullptr};
+    static int block_cache[128] = {0}, per_cu_cache[128] = {0};
     if (cus == 0) {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
@@ -23,7 +23,7 @@ unsigned resident_blocks(const void* kernel, int block) {
             cus = 256;
     }
     int per_cu = 0;
-    for (int i = 0; i < 64; i++) {
+    for (int i = 0; i < 128; i++) {
         if (kern_cache[i] == kernel && block_cache[i] == block) { per_cu = per_cu_cache[i]; break; }
         if (kern_cache[i] == nullptr) {
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu <= 0)

@@ -121,17 +121,17 @@ struct TileLds {
 };
 
 template <int ITEMS>
-__device__ __forceinline__ int tile_elem(int r) {
-    return (int)(threadIdx.x / kWave) * (kWave * ITEMS) + r * kWave + (int)(threadIdx.x & (kWave - 1));
+__device__ __forceinline__ int tile_elem(int tid, int r) {
+    return (int)(tid / kWave) * (kWave * ITEMS) + r * kWave + (int)(tid & (kWave - 1));
 }
 
 template <int ITEMS, bool FIRST>
 __device__ __forceinline__ void tile_load(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                          const int32_t* __restrict__ p2c, uint32_t n_cells, int64_t base, int m,
+                                          const int32_t* __restrict__ p2c, uint32_t n_cells, int64_t base, int m, int tid,
                                           uint32_t (&key)[ITEMS], uint32_t (&val)[ITEMS]) {
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
-        const int li = tile_elem<ITEMS>(r);
+        const int li = tile_elem<ITEMS>(tid, r);
         const bool ok = li < m;
         if (FIRST) {  // keys straight from p2c, the particle index as the value
             key[r] = ok ? cell_key(p2c[base + li], n_cells) : 0u;
@@ -147,13 +147,13 @@ __device__ __forceinline__ void tile_load(const uint32_t* __restrict__ kin, cons
 // L.run and the exclusive prefix over the waves in L.wave. L.wave must be zeroed (and that made
 // visible) by the caller.
 template <int ITEMS>
-__device__ __forceinline__ void tile_rank(TileLds<ITEMS>& L, const uint32_t (&key)[ITEMS], int m, int shift,
+__device__ __forceinline__ void tile_rank(TileLds<ITEMS>& L, int tid, const uint32_t (&key)[ITEMS], int m, int shift,
                                           uint32_t (&rank)[ITEMS]) {
-    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int w = tid / kWave, lane = tid & (kWave - 1);
     const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
-        const bool ok = tile_elem<ITEMS>(r) < m;
+        const bool ok = tile_elem<ITEMS>(tid, r) < m;
         const uint32_t d = (key[r] >> shift) & 0xFFu;
         const unsigned long long peers = digit_peers(d, __ballot(ok));
         // every lane reads the wave's count before the lowest lane of each digit group adds the group
@@ -165,19 +165,19 @@ __device__ __forceinline__ void tile_rank(TileLds<ITEMS>& L, const uint32_t (&ke
     __syncthreads();
     uint32_t run = 0;
     for (int q = 0; q < kB / kWave; q++) {
-        const uint32_t c = L.wave[q][threadIdx.x];
-        L.wave[q][threadIdx.x] = run;
+        const uint32_t c = L.wave[q][tid];
+        L.wave[q][tid] = run;
         run += c;
     }
-    L.run[threadIdx.x] = run;
+    L.run[tid] = run;
     __syncthreads();
 }
 
 // Digit starts inside the tile: exclusive scan of L.run (one digit per thread), in place.
 template <int ITEMS>
-__device__ __forceinline__ void tile_digit_starts(TileLds<ITEMS>& L) {
-    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    const uint32_t c = L.run[threadIdx.x];
+__device__ __forceinline__ void tile_digit_starts(TileLds<ITEMS>& L, int tid) {
+    const int w = tid / kWave, lane = tid & (kWave - 1);
+    const uint32_t c = L.run[tid];
     uint32_t inc = c;
 #pragma unroll
     for (int dd = 1; dd < kWave; dd <<= 1) {
@@ -188,7 +188,7 @@ __device__ __forceinline__ void tile_digit_starts(TileLds<ITEMS>& L) {
     __syncthreads();
     uint32_t off = 0;
     for (int q = 0; q < w; q++) off += L.tot[q];
-    L.run[threadIdx.x] = off + inc - c;
+    L.run[tid] = off + inc - c;
     __syncthreads();
 }
 
@@ -196,14 +196,14 @@ __device__ __forceinline__ void tile_digit_starts(TileLds<ITEMS>& L) {
 // `limit` are dropped (never taken by a correct sort; a bound on every store). The last pass writes no
 // keys, except as sorted p2c into p2c_out (nullable; key n_cells back to MAX_CELL_INDEX).
 template <int ITEMS, bool LAST>
-__device__ __forceinline__ void tile_write(TileLds<ITEMS>& L, const uint32_t (&key)[ITEMS], const uint32_t (&val)[ITEMS],
+__device__ __forceinline__ void tile_write(TileLds<ITEMS>& L, int tid, const uint32_t (&key)[ITEMS], const uint32_t (&val)[ITEMS],
                                            const uint32_t (&rank)[ITEMS], int m, int shift, uint64_t limit,
                                            uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                            int32_t* __restrict__ p2c_out, uint32_t n_cells) {
-    const int w = threadIdx.x / kWave;
+    const int w = tid / kWave;
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
-        if (tile_elem<ITEMS>(r) < m) {
+        if (tile_elem<ITEMS>(tid, r) < m) {
             const uint32_t d = (key[r] >> shift) & 0xFFu;
             const uint32_t pos = L.run[d] + L.wave[w][d] + rank[r];
             L.key[pos] = key[r];
@@ -213,7 +213,7 @@ __device__ __forceinline__ void tile_write(TileLds<ITEMS>& L, const uint32_t (&k
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
-        const int p = r * kB + threadIdx.x;
+        const int p = r * kB + tid;
         if (p < m) {
             const uint32_t k = L.key[p];
             const uint32_t d = (k >> shift) & 0xFFu;
@@ -228,26 +228,37 @@ __device__ __forceinline__ void tile_write(TileLds<ITEMS>& L, const uint32_t (&k
 }
 
 // Scatter of one digit pass with the (digit, block) offsets from the scanned per-block histograms.
+// Four waves per SIMD: the occupancy the 16-key tile's LDS allows (four 39 KB blocks per CU), reached
+// without spilling.
 template <int ITEMS, bool LAST, bool FIRST>
-__global__ void __launch_bounds__(kB) sort_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                   const int32_t* __restrict__ p2c, uint32_t n_cells,
-                                                   const oppx_move_counters* ctr, int64_t n_max, int shift,
-                                                   const uint32_t* __restrict__ offs, int64_t nblocks,
-                                                   uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                   int32_t* __restrict__ p2c_out) {
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4)))
+sort_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, const int32_t* __restrict__ p2c,
+             uint32_t n_cells, const oppx_move_counters* ctr, int64_t n_max, int shift,
+             const uint32_t* __restrict__ offs, int64_t nblocks, uint32_t* __restrict__ kout,
+             uint32_t* __restrict__ vout, int32_t* __restrict__ p2c_out) {
     __shared__ TileLds<ITEMS> L;
-    const int64_t base = (int64_t)blockIdx.x * (kB * ITEMS);
     const int64_t n = live_n(ctr, n_max);
-    // elements of this tile (<= 0 for a tile past the live count: nothing is loaded or written)
-    const int m = (int)((n - base) < kB * ITEMS ? (n - base) : kB * ITEMS);
-    for (int q = 0; q < kB / kWave; q++) L.wave[q][threadIdx.x] = 0;
-    L.dst[threadIdx.x] = offs[(int64_t)threadIdx.x * nblocks + blockIdx.x];
-    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
-    tile_load<ITEMS, FIRST>(kin, vin, p2c, n_cells, base, m, key, val);
-    __syncthreads();
-    tile_rank<ITEMS>(L, key, m, shift, rank);
-    tile_digit_starts<ITEMS>(L);
-    tile_write<ITEMS, LAST>(L, key, val, rank, m, shift, (uint64_t)n, kout, vout, p2c_out, n_cells);
+    // the grid (at most the blocks that fit the chip at once) strides over the tiles that hold live
+    // elements: a tile past the live count is never dispatched, where each such block would still have
+    // had to be given its LDS (40 KB) before it could leave
+    const int64_t tiles = (n + kB * ITEMS - 1) / (kB * ITEMS);
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t base = t * (kB * ITEMS);
+        const int m = (int)((n - base) < kB * ITEMS ? (n - base) : kB * ITEMS);  // elements of the tile
+        // the thread index, opaque to the compiler inside each tile: the per-element indexes and LDS
+        // addresses derived from it are recomputed per tile instead of held in registers across the loop
+        int tid = (int)threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        for (int q = 0; q < kB / kWave; q++) L.wave[q][tid] = 0;
+        L.dst[tid] = offs[(int64_t)tid * nblocks + t];
+        uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+        tile_load<ITEMS, FIRST>(kin, vin, p2c, n_cells, base, m, tid, key, val);
+        __syncthreads();
+        tile_rank<ITEMS>(L, tid, key, m, shift, rank);
+        tile_digit_starts<ITEMS>(L, tid);
+        tile_write<ITEMS, LAST>(L, tid, key, val, rank, m, shift, (uint64_t)n, kout, vout, p2c_out, n_cells);
+        __syncthreads();  // every thread is done with the tile's LDS before the next tile fills it
+    }
 }
 
 // ---- exclusive scan of a uint32 array (chunks of kElems) ----
@@ -469,8 +480,14 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
                 scan_sums<<<1, kB, 0, s>>>(sc.sums, sc.scan_blocks);
                 scan_apply<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
                 OPPX_LAUNCH_CHECK("sort_scan");
-#define OPPX_SCATTER(I, LST, F) sort_scatter<I, LST, F><<<(unsigned)sc.nblocks, kB, 0, s>>>( \
-    kin, vin, p2c, kc, counters, n_max, shift, sc.hist, sc.nblocks, kout, vdst, p2c_out)
+#define OPPX_SCATTER(I, LST, F)                                                                      \
+    do {                                                                                              \
+        auto kern = sort_scatter<I, LST, F>;                                                          \
+        const unsigned res = resident_blocks(reinterpret_cast<const void*>(kern), kB);               \
+        const unsigned g = (int64_t)res < sc.nblocks ? res : (unsigned)sc.nblocks;                    \
+        kern<<<g, kB, 0, s>>>(kin, vin, p2c, kc, counters, n_max, shift, sc.hist, sc.nblocks, kout, vdst, \
+                              p2c_out);                                                              \
+    } while (0)
 #define OPPX_SCATTER_LF(I)                                                     \
     do {                                                                       \
         if (last) { if (first) OPPX_SCATTER(I, true, true); else OPPX_SCATTER(I, true, false); } \
