@@ -23,6 +23,7 @@ namespace oppx {
 namespace {
 
 constexpr int kB = 256;
+constexpr unsigned kGridMult = 4;  // grids that stride over live work: this many times the resident blocks
 
 struct Kahan {
     float s = 0.0f, c = 0.0f;
@@ -93,35 +94,41 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* tot) {
 // Also checks the precondition: a valid cell right after a larger cell or after an entry that is not a
 // valid cell (a removed particle) means p2c is not sorted, and two runs of one cell would overwrite each
 // other's partials; every such entry is counted in ctr->faults.
+// The grid strides over the tiles below the live count.
 __global__ void __launch_bounds__(kB) run_count(const int32_t* __restrict__ p2c, oppx_move_counters* ctr,
                                                 int64_t n_max, int64_t n_cells, uint32_t* __restrict__ bsum) {
     __shared__ RunTile T;
     const int64_t n = live_of(ctr, n_max);
-    const int64_t base0 = (int64_t)blockIdx.x * kRunElems;
-    load_run_tile(T, p2c, base0, n);
+    const int64_t tiles = (n + kRunElems - 1) / kRunElems;
     const int e0 = (int)threadIdx.x * kRunItems;
-    uint32_t c = 0;
     int unsorted = 0;
-    for (int r = 0; r < kRunItems; r++) {
-        const int64_t j = base0 + e0 + r;
-        if (j >= n) break;
-        if (run_head(T, e0 + r, n_cells)) {
-            c++;
-            const int32_t prev = T.v[tile_at(e0 + r)], cur = T.v[tile_at(e0 + r + 1)];
-            if (j > 0 && (!((unsigned)prev < (unsigned)n_cells) || prev > cur)) unsorted++;
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t base0 = t * kRunElems;
+        load_run_tile(T, p2c, base0, n);
+        uint32_t c = 0;
+        for (int r = 0; r < kRunItems; r++) {
+            const int64_t j = base0 + e0 + r;
+            if (j >= n) break;
+            if (run_head(T, e0 + r, n_cells)) {
+                c++;
+                const int32_t prev = T.v[tile_at(e0 + r)], cur = T.v[tile_at(e0 + r + 1)];
+                if (j > 0 && (!((unsigned)prev < (unsigned)n_cells) || prev > cur)) unsorted++;
+            }
         }
+        uint32_t tot;
+        block_scan(c, &tot);  // ends with a barrier: the tile may be overwritten after it
+        if (threadIdx.x == 0) bsum[t] = tot;
     }
-    uint32_t tot;
-    block_scan(c, &tot);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
     unsorted = wave_sum_i32(unsorted);
     if ((threadIdx.x & (kWave - 1)) == 0 && unsorted) atomicAdd(&ctr->faults, unsorted);
 }
 
-// single block: exclusive scan of nb block counts in place; the total goes to n_runs[0], and the
+// single block: exclusive scan of the live tiles' counts in place; the total goes to n_runs[0], and the
 // long-run count n_runs[1] starts at zero
-__global__ void __launch_bounds__(kB) run_scan(uint32_t* __restrict__ bsum, int64_t nb, uint32_t* __restrict__ n_runs) {
+__global__ void __launch_bounds__(kB) run_scan(uint32_t* __restrict__ bsum, const oppx_move_counters* ctr,
+                                               int64_t n_max, uint32_t* __restrict__ n_runs) {
     if (threadIdx.x == 0) n_runs[1] = 0u;
+    const int64_t nb = (live_of(ctr, n_max) + kRunElems - 1) / kRunElems;
     uint32_t carry = 0;
     for (int64_t b0 = 0; b0 < nb; b0 += kB) {
         const int64_t j = b0 + threadIdx.x;
@@ -139,16 +146,20 @@ __global__ void __launch_bounds__(kB) run_compact(const int32_t* __restrict__ p2
                                                   int32_t* __restrict__ heads) {
     __shared__ RunTile T;
     const int64_t n = live_of(ctr, n_max);
-    const int64_t base0 = (int64_t)blockIdx.x * kRunElems;
-    load_run_tile(T, p2c, base0, n);
+    const int64_t tiles = (n + kRunElems - 1) / kRunElems;
     const int e0 = (int)threadIdx.x * kRunItems;
-    uint32_t c = 0;
-    for (int r = 0; r < kRunItems; r++)
-        if (base0 + e0 + r < n && run_head(T, e0 + r, n_cells)) c++;
-    uint32_t tot;
-    uint32_t pos = bsum[blockIdx.x] + block_scan(c, &tot);
-    for (int r = 0; r < kRunItems; r++)
-        if (base0 + e0 + r < n && run_head(T, e0 + r, n_cells)) heads[pos++] = (int32_t)(base0 + e0 + r);
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t base0 = t * kRunElems;
+        load_run_tile(T, p2c, base0, n);
+        uint32_t c = 0;
+        for (int r = 0; r < kRunItems; r++)
+            if (base0 + e0 + r < n && run_head(T, e0 + r, n_cells)) c++;
+        uint32_t tot;
+        uint32_t pos = bsum[t] + block_scan(c, &tot);
+        for (int r = 0; r < kRunItems; r++)
+            if (base0 + e0 + r < n && run_head(T, e0 + r, n_cells)) heads[pos++] = (int32_t)(base0 + e0 + r);
+        __syncthreads();  // every thread is done with the tile before the next one is loaded
+    }
 }
 
 constexpr int64_t kLongRun = 64;  // runs longer than this are summed by a whole wave
@@ -309,11 +320,15 @@ int oppx_fempic_deposit_sorted_f32(int64_t n_max, int64_t stride, oppx_move_coun
     OPPX_LAUNCH_CHECK("zero_partials");
     if (n_max > 0) {
         RunScratch rs = run_carve(scratch, n_max);
-        run_count<<<(unsigned)rs.nb, kB, 0, s>>>(p2c, counters, n_max, n_cells, rs.bsum);
-        run_scan<<<1, kB, 0, s>>>(rs.bsum, rs.nb, rs.n_runs);
-        run_compact<<<(unsigned)rs.nb, kB, 0, s>>>(p2c, counters, n_max, n_cells, rs.bsum, rs.heads);
-        cell_runs<<<grid_for(n_max, kB), kB, 0, s>>>(p2c, lc, stride, counters, n_max, rs.heads, rs.n_runs,
-                                                     rs.long_runs, cell_partials);
+        // grids that stride over the live tiles; one thread per run, at most one run per cell
+        const int64_t max_runs = n_max < n_cells ? n_max : n_cells;
+        run_count<<<stride_grid(reinterpret_cast<const void*>(run_count), kB, rs.nb, kGridMult), kB, 0, s>>>(
+            p2c, counters, n_max, n_cells, rs.bsum);
+        run_scan<<<1, kB, 0, s>>>(rs.bsum, counters, n_max, rs.n_runs);
+        run_compact<<<stride_grid(reinterpret_cast<const void*>(run_compact), kB, rs.nb, kGridMult), kB, 0, s>>>(
+            p2c, counters, n_max, n_cells, rs.bsum, rs.heads);
+        cell_runs<<<grid_for(max_runs, kB), kB, 0, s>>>(p2c, lc, stride, counters, n_max, rs.heads, rs.n_runs,
+                                                        rs.long_runs, cell_partials);
         OPPX_LAUNCH_CHECK("cell_runs");
         int64_t gl = grid_for(n_max, kLongRun * (kB / kWave));  // at most one long run per kLongRun particles
         if (gl > 2048) gl = 2048;

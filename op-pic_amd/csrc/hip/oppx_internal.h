@@ -96,4 +96,14 @@ __device__ __forceinline__ int wave_max_i32(int v) {
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Grid of a kernel whose blocks stride over `units` units of work (one block each): at most `mult`
+// times the blocks of `kernel` that fit the device at once. Units past a live count that the kernel
+// reads on the device then cost no dispatched blocks; each dead block would otherwise still start its
+// waves and wait for that read before leaving.
+inline unsigned stride_grid(const void* kernel, int block, int64_t units, unsigned mult) {
+    const int64_t cap = (int64_t)resident_blocks(kernel, block) * mult;
+    const int64_t g = units < cap ? units : cap;
+    return (unsigned)(g > 0 ? g : 1);
+}
+
 }  // namespace oppx

@@ -25,6 +25,7 @@ constexpr int kMinSortItems = 4;     // smallest radix block (elements per threa
 constexpr int kBins = 256;
 constexpr int kMaxDats = 16;
 constexpr int kRemSlots = 256;  // removed-particle counters (block id modulo kRemSlots)
+constexpr unsigned kGridMult = 4;  // grids that stride over live work: this many times the resident blocks
 
 struct GatherTable {
     int n;
@@ -62,44 +63,52 @@ __device__ __forceinline__ uint32_t cell_key(int32_t c, uint32_t n_cells) {
 // (independent loads in flight). Counting needs no ranks: a wave whose live lanes share one digit adds
 // once (nearly sorted keys), any other wave adds per lane.
 template <int ITEMS, bool FIRST>
-__global__ void __launch_bounds__(kB) sort_hist(const uint32_t* __restrict__ keys, const int32_t* __restrict__ p2c,
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(8))) sort_hist(const uint32_t* __restrict__ keys, const int32_t* __restrict__ p2c,
                                                 uint32_t n_cells, const oppx_move_counters* ctr, int64_t n_max,
                                                 int shift, uint32_t* __restrict__ hist, int64_t nblocks,
                                                 unsigned long long* __restrict__ n_removed) {
     __shared__ uint32_t s_h[kBins];
-    s_h[threadIdx.x] = 0;
-    __syncthreads();
     const int64_t n = live_n(ctr, n_max);
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t base = (int64_t)blockIdx.x * (kB * ITEMS);
-    uint32_t key[ITEMS];
-#pragma unroll
-    for (int r = 0; r < ITEMS; r++) {
-        const int64_t i = base + r * kB + threadIdx.x;
-        key[r] = i < n ? (FIRST ? cell_key(p2c[i], n_cells) : keys[i]) : 0u;
-    }
+    // the grid strides over the tiles below the live count (tile t's counts go to column t)
+    const int64_t tiles = (n + kB * ITEMS - 1) / (kB * ITEMS);
     unsigned long long removed = 0;
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        s_h[threadIdx.x] = 0;  // each thread clears the word it wrote out last tile
+        __syncthreads();
+        const int64_t base = t * (kB * ITEMS);
+        uint32_t key[ITEMS];
 #pragma unroll
-    for (int r = 0; r < ITEMS; r++) {
-        const bool ok = base + r * kB + threadIdx.x < n;
-        const uint32_t k = key[r];
-        if (FIRST) removed += (unsigned long long)__popcll(__ballot(ok && k == n_cells));
-        const uint32_t d = (k >> shift) & 0xFFu;
-        const unsigned long long act = __ballot(ok);
-        if (act == 0ull) continue;  // uniform over the wave
-        const int lead = __ffsll((long long)act) - 1;
-        const uint32_t d_lead = (uint32_t)__shfl((int)d, lead);
-        if (__ballot(ok && d == d_lead) == act) {
-            if (lane == lead) atomicAdd(&s_h[d], (uint32_t)__popcll(act));
-        } else if (ok) {
-            atomicAdd(&s_h[d], 1u);
+        for (int r = 0; r < ITEMS; r++) {
+            const int64_t i = base + r * kB + threadIdx.x;
+            key[r] = i < n ? (FIRST ? cell_key(p2c[i], n_cells) : keys[i]) : 0u;
         }
+#pragma unroll
+        for (int r = 0; r < ITEMS; r++) {
+            const bool ok = base + r * kB + threadIdx.x < n;
+            const uint32_t k = key[r];
+            if (FIRST) removed += (unsigned long long)__popcll(__ballot(ok && k == n_cells));
+            const uint32_t d = (k >> shift) & 0xFFu;
+            const unsigned long long act = __ballot(ok);
+            if (act == 0ull) continue;  // uniform over the wave
+            const int lead = __ffsll((long long)act) - 1;
+            const uint32_t d_lead = (uint32_t)__shfl((int)d, lead);
+            if (__ballot(ok && d == d_lead) == act) {
+                if (lane == lead) atomicAdd(&s_h[d], (uint32_t)__popcll(act));
+            } else if (ok) {
+                atomicAdd(&s_h[d], 1u);
+            }
+        }
+        __syncthreads();
+        hist[(int64_t)threadIdx.x * nblocks + t] = s_h[threadIdx.x];
     }
     // per-wave removed counts land on one of kRemSlots counters (block id modulo kRemSlots): one
     // shared counter would serialise every wave's atomic in one L2 channel
     if (FIRST && lane == 0 && removed) atomicAdd(n_removed + (blockIdx.x % kRemSlots), removed);
-    __syncthreads();
-    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = s_h[threadIdx.x];
+    // the columns of the tiles past the live count hold zero counts for the scan (row by row, with
+    // consecutive threads on consecutive columns)
+    for (int64_t c = tiles + (int64_t)blockIdx.x * kB + threadIdx.x; c < nblocks; c += (int64_t)gridDim.x * kB)
+        for (int d = 0; d < kBins; d++) hist[(int64_t)d * nblocks + c] = 0u;
 }
 
 // ---- one digit pass over a tile of 256 * ITEMS consecutive elements ----
@@ -343,7 +352,6 @@ __global__ void __launch_bounds__(kB) sort_gather(const uint32_t* __restrict__ p
     }
     if (perm_out) perm_out[j] = (int32_t)from;
 }
-
 __global__ void zero_u64(unsigned long long* p) { p[threadIdx.x] = 0ull; }  // kRemSlots threads
 
 __global__ void lower_bound_kernel(const int32_t* __restrict__ p2c, const oppx_move_counters* ctr, int64_t n_max,
@@ -469,8 +477,9 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
             const bool last = pass == passes - 1;  // the keys of the last pass are not needed
             uint32_t* vdst = (last && direct) ? reinterpret_cast<uint32_t*>(perm_out) : vout;
             {
-#define OPPX_HIST(I, F) sort_hist<I, F><<<(unsigned)sc.nblocks, kB, 0, s>>>(kin, p2c, kc, counters, n_max, shift, \
-                                                                           sc.hist, sc.nblocks, sc.n_removed)
+#define OPPX_HIST(I, F)                                                                              \
+    sort_hist<I, F><<<stride_grid(reinterpret_cast<const void*>(sort_hist<I, F>), kB, sc.nblocks, kGridMult), kB, 0, \
+                      s>>>(kin, p2c, kc, counters, n_max, shift, sc.hist, sc.nblocks, sc.n_removed)
                 if (items == 4) { if (first) OPPX_HIST(4, true); else OPPX_HIST(4, false); }
                 else if (items == 8) { if (first) OPPX_HIST(8, true); else OPPX_HIST(8, false); }
                 else { if (first) OPPX_HIST(16, true); else OPPX_HIST(16, false); }
