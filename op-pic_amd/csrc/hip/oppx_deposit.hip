@@ -174,8 +174,52 @@ __device__ __forceinline__ int64_t run_end(const int32_t* __restrict__ p2c, cons
     return e;
 }
 
+// Kahan sums of lc[0..3] of particles [j, e) in index order; with PERM the particle at sorted position q
+// has its lc at perm[q] (lc left in the order before the sort). Four particles at a time: their indexes,
+// then their 16 lc words, are loaded before any of them is added, so one lane keeps 16 loads in flight;
+// the additions keep index order.
+// A permutation entry outside [0, stride) is read as particle 0 (never outside the dat) and counted in
+// the return value.
+template <bool PERM>
+__device__ __forceinline__ int perm_at(const int32_t* __restrict__ perm, int64_t q, int64_t stride, int& bad) {
+    if (!PERM) return (int)q;
+    const int32_t p = perm[q];
+    if ((uint64_t)(int64_t)p < (uint64_t)stride) return p;
+    bad++;
+    return 0;
+}
+
+template <bool PERM>
+__device__ __forceinline__ int run_sums(const double* __restrict__ lc, const int32_t* __restrict__ perm,
+                                        int64_t stride, int64_t j, int64_t e, int64_t step, Kahan (&k)[4]) {
+    int bad = 0;
+    int64_t q = j;
+    for (; q + 3 * step < e; q += 4 * step) {
+        int64_t p[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) p[u] = PERM ? (int64_t)perm_at<PERM>(perm, q + u * step, stride, bad) : q + u * step;
+        float v[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int d = 0; d < 4; d++) v[u][d] = (float)lc[d * stride + p[u]];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int d = 0; d < 4; d++) k[d].add(v[u][d]);
+    }
+    for (; q < e; q += step) {
+        const int64_t pq = PERM ? (int64_t)perm_at<PERM>(perm, q, stride, bad) : q;
+#pragma unroll
+        for (int d = 0; d < 4; d++) k[d].add((float)lc[d * stride + pq]);
+    }
+    return bad;
+}
+
+template <bool PERM>
 __global__ void __launch_bounds__(kB) cell_runs(const int32_t* __restrict__ p2c, const double* __restrict__ lc,
-                                                int64_t stride, const oppx_move_counters* ctr, int64_t n_max,
+                                                const int32_t* __restrict__ perm, int64_t stride,
+                                                oppx_move_counters* ctr, int64_t n_max,
                                                 const int32_t* __restrict__ heads, uint32_t* __restrict__ n_runs,
                                                 int32_t* __restrict__ long_runs, float* __restrict__ part) {
     const int64_t r = (int64_t)blockIdx.x * kB + threadIdx.x;
@@ -189,22 +233,20 @@ __global__ void __launch_bounds__(kB) cell_runs(const int32_t* __restrict__ p2c,
         long_runs[atomicAdd(n_runs + 1, 1u)] = (int32_t)r;
         return;
     }
-    Kahan k0, k1, k2, k3;
-    for (int64_t q = j; q < e; q++) {
-        k0.add((float)lc[q]);
-        k1.add((float)lc[stride + q]);
-        k2.add((float)lc[2 * stride + q]);
-        k3.add((float)lc[3 * stride + q]);
-    }
+    Kahan k[4];
+    const int bad = run_sums<PERM>(lc, perm, stride, j, e, 1, k);
+    if (PERM && bad) atomicAdd(&ctr->faults, bad);
     float4 v;
-    v.x = k0.s; v.y = k1.s; v.z = k2.s; v.w = k3.s;
+    v.x = k[0].s; v.y = k[1].s; v.z = k[2].s; v.w = k[3].s;
     *reinterpret_cast<float4*>(part + 4 * (int64_t)c) = v;
 }
 
 // One wave per long run (waves stride over the list): lane l sums particles head + l + 64 t in index
 // order (Kahan), then lane 0 adds the 64 lane sums in lane order (Kahan).
+template <bool PERM>
 __global__ void __launch_bounds__(kB) cell_runs_long(const int32_t* __restrict__ p2c, const double* __restrict__ lc,
-                                                     int64_t stride, const oppx_move_counters* ctr, int64_t n_max,
+                                                     const int32_t* __restrict__ perm, int64_t stride,
+                                                     oppx_move_counters* ctr, int64_t n_max,
                                                      const int32_t* __restrict__ heads,
                                                      const uint32_t* __restrict__ n_runs,
                                                      const int32_t* __restrict__ long_runs, float* __restrict__ part) {
@@ -218,19 +260,15 @@ __global__ void __launch_bounds__(kB) cell_runs_long(const int32_t* __restrict__
         const int64_t j = heads[r];
         const int32_t c = p2c[j];
         const int64_t e = run_end(p2c, heads, r, nr, n, c);
-        Kahan k0, k1, k2, k3;
-        for (int64_t q = j + lane; q < e; q += kWave) {
-            k0.add((float)lc[q]);
-            k1.add((float)lc[stride + q]);
-            k2.add((float)lc[2 * stride + q]);
-            k3.add((float)lc[3 * stride + q]);
-        }
+        Kahan ks[4];
+        const int bad = run_sums<PERM>(lc, perm, stride, j + lane, e, kWave, ks);
+        if (PERM && bad) atomicAdd(&ctr->faults, bad);
         Kahan t0, t1, t2, t3;
         for (int l = 0; l < kWave; l++) {  // the same on every lane: lane l's sums, in lane order
-            t0.add(__shfl(k0.s, l));
-            t1.add(__shfl(k1.s, l));
-            t2.add(__shfl(k2.s, l));
-            t3.add(__shfl(k3.s, l));
+            t0.add(__shfl(ks[0].s, l));
+            t1.add(__shfl(ks[1].s, l));
+            t2.add(__shfl(ks[2].s, l));
+            t3.add(__shfl(ks[3].s, l));
         }
         if (lane == 0) {
             float4 v;
@@ -292,26 +330,20 @@ __global__ void __launch_bounds__(kB) node_gather(const int32_t* __restrict__ no
     if (charge_f32) charge_f32[v] = k.s;
 }
 
-}  // namespace
-}  // namespace oppx
-
-using namespace oppx;
-
-extern "C" {
-
-size_t oppx_fempic_deposit_scratch_bytes(int64_t n_max) { return n_max < 0 ? 0 : run_scratch_need(n_max); }
-
-int oppx_fempic_deposit_sorted_f32(int64_t n_max, int64_t stride, oppx_move_counters* counters, const int32_t* p2c,
-                                   const double* lc, int64_t n_cells, const int32_t* node_start, const int32_t* node_end,
-                                   const int32_t* node_ref, int64_t n_nodes, float* cell_partials, double* node_charge,
-                                   float* node_charge_f32, void* scratch, size_t scratch_bytes, void* stream) {
+// The deposit over cell-sorted p2c, with lc in the same order (perm null) or in the order before the sort
+// (perm[q]: the source index of sorted slot q).
+int deposit_f32(const char* name, int64_t n_max, int64_t stride, oppx_move_counters* counters, const int32_t* p2c,
+                const double* lc, const int32_t* perm, int64_t n_cells, const int32_t* node_start,
+                const int32_t* node_end, const int32_t* node_ref, int64_t n_nodes, float* cell_partials,
+                double* node_charge, float* node_charge_f32, void* scratch, size_t scratch_bytes, void* stream) {
+    const std::string fn(name);
     OPPX_ARG_CHECK(counters && p2c && lc && node_start && node_ref && cell_partials && scratch,
-                   "oppx_fempic_deposit_sorted_f32: null pointer");
-    OPPX_ARG_CHECK(node_charge || node_charge_f32, "oppx_fempic_deposit_sorted_f32: no output");
+                   fn + ": null pointer");
+    OPPX_ARG_CHECK(node_charge || node_charge_f32, fn + ": no output");
     OPPX_ARG_CHECK(n_max >= 0 && stride >= n_max && n_cells > 0 && n_cells < kMaxCell && n_nodes > 0,
-                   "oppx_fempic_deposit_sorted_f32: bad sizes");
-    OPPX_ARG_CHECK(n_max < (int64_t)0x7FFFFFFF, "oppx_fempic_deposit_sorted_f32: n_max beyond 32-bit indices");
-    OPPX_ARG_CHECK(scratch_bytes >= run_scratch_need(n_max), "oppx_fempic_deposit_sorted_f32: scratch too small");
+                   fn + ": bad sizes");
+    OPPX_ARG_CHECK(n_max < (int64_t)0x7FFFFFFF, fn + ": n_max beyond 32-bit indices");
+    OPPX_ARG_CHECK(scratch_bytes >= run_scratch_need(n_max), fn + ": scratch too small");
     hipStream_t s = as_stream(stream);
     const int64_t np = 4 * n_cells;
     int64_t gz = grid_for(np, kB);
@@ -327,19 +359,58 @@ int oppx_fempic_deposit_sorted_f32(int64_t n_max, int64_t stride, oppx_move_coun
         run_scan<<<1, kB, 0, s>>>(rs.bsum, counters, n_max, rs.n_runs);
         run_compact<<<stride_grid(reinterpret_cast<const void*>(run_compact), kB, rs.nb, kGridMult), kB, 0, s>>>(
             p2c, counters, n_max, n_cells, rs.bsum, rs.heads);
-        cell_runs<<<grid_for(max_runs, kB), kB, 0, s>>>(p2c, lc, stride, counters, n_max, rs.heads, rs.n_runs,
-                                                        rs.long_runs, cell_partials);
-        OPPX_LAUNCH_CHECK("cell_runs");
+        const unsigned gr = grid_for(max_runs, kB);
         int64_t gl = grid_for(n_max, kLongRun * (kB / kWave));  // at most one long run per kLongRun particles
         if (gl > 2048) gl = 2048;
-        cell_runs_long<<<(unsigned)gl, kB, 0, s>>>(p2c, lc, stride, counters, n_max, rs.heads, rs.n_runs,
-                                                    rs.long_runs, cell_partials);
-        OPPX_LAUNCH_CHECK("cell_runs_long");
+        if (perm) {
+            cell_runs<true><<<gr, kB, 0, s>>>(p2c, lc, perm, stride, counters, n_max, rs.heads, rs.n_runs,
+                                               rs.long_runs, cell_partials);
+            cell_runs_long<true><<<(unsigned)gl, kB, 0, s>>>(p2c, lc, perm, stride, counters, n_max, rs.heads,
+                                                              rs.n_runs, rs.long_runs, cell_partials);
+        } else {
+            cell_runs<false><<<gr, kB, 0, s>>>(p2c, lc, nullptr, stride, counters, n_max, rs.heads, rs.n_runs,
+                                                rs.long_runs, cell_partials);
+            cell_runs_long<false><<<(unsigned)gl, kB, 0, s>>>(p2c, lc, nullptr, stride, counters, n_max, rs.heads,
+                                                               rs.n_runs, rs.long_runs, cell_partials);
+        }
+        OPPX_LAUNCH_CHECK("cell_runs");
     }
     node_gather<<<grid_for(n_nodes, kB), kB, 0, s>>>(node_start, node_end, node_ref, n_nodes, cell_partials,
                                                      node_charge, node_charge_f32);
     OPPX_LAUNCH_CHECK("node_gather");
     return OPPX_OK;
+}
+
+}  // namespace
+}  // namespace oppx
+
+using namespace oppx;
+
+extern "C" {
+
+size_t oppx_fempic_deposit_scratch_bytes(int64_t n_max) { return n_max < 0 ? 0 : run_scratch_need(n_max); }
+
+int oppx_fempic_deposit_sorted_f32(int64_t n_max, int64_t stride, oppx_move_counters* counters, const int32_t* p2c,
+                                   const double* lc, int64_t n_cells, const int32_t* node_start, const int32_t* node_end,
+                                   const int32_t* node_ref, int64_t n_nodes, float* cell_partials, double* node_charge,
+                                   float* node_charge_f32, void* scratch, size_t scratch_bytes, void* stream) {
+    return deposit_f32("oppx_fempic_deposit_sorted_f32", n_max, stride, counters, p2c, lc, nullptr, n_cells,
+                       node_start, node_end, node_ref, n_nodes, cell_partials, node_charge, node_charge_f32, scratch,
+                       scratch_bytes, stream);
+}
+
+int oppx_fempic_deposit_sorted_f32_perm(int64_t n_max, int64_t stride, oppx_move_counters* counters,
+                                        const int32_t* p2c, const double* lc, const int32_t* perm, int64_t n_cells,
+                                        const int32_t* node_start, const int32_t* node_end, const int32_t* node_ref,
+                                        int64_t n_nodes, float* cell_partials, double* node_charge,
+                                        float* node_charge_f32, void* scratch, size_t scratch_bytes, void* stream) {
+    if (!perm) {
+        set_error("oppx_fempic_deposit_sorted_f32_perm: null perm");
+        return OPPX_ERR_ARG;
+    }
+    return deposit_f32("oppx_fempic_deposit_sorted_f32_perm", n_max, stride, counters, p2c, lc, perm, n_cells,
+                       node_start, node_end, node_ref, n_nodes, cell_partials, node_charge, node_charge_f32, scratch,
+                       scratch_bytes, stream);
 }
 
 }  // extern "C"

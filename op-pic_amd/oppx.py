@@ -177,6 +177,9 @@ _SIGS = {
     "oppx_fempic_deposit_sorted_f32": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
                                                c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
                                                ctypes.c_size_t, c_void_p]),
+    "oppx_fempic_deposit_sorted_f32_perm": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
+                                                    c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                                    c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
     "oppx_lower_bound": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "oppx_sort_scratch_bytes": (c_size_t, [c_int64]),
     "oppx_sort_by_cell": (c_int, [c_int64, c_int64, c_void_p, c_int64, c_void_p, POINTER(DatDesc), POINTER(DatDesc),
@@ -660,15 +663,12 @@ class FempicDevice:
     def deposit_sorted_f32(self):
         """FP32 Kahan deposit over the (cell-sorted) particles; returns the FP32 node charge tensor.
         Over a pending (lazy) sort the particle dats stay as they are: the sort left the sorted p2c in the
-        swap buffer, the sorted lc is gathered into the swap lc buffer (which the next pass overwrites),
-        and the deposit reads those two."""
+        swap buffer, and the deposit reads it with lc through the sort's permutation
+        (oppx_fempic_deposit_sorted_f32_perm, the same sums bit for bit)."""
         torch = self.torch
-        if getattr(self, "pending", False):
-            swap = self._swap_buffers()
-            check(lib().oppx_gather_dats(self.n_max, self.cap, self.counters.data_ptr(), self.perm.data_ptr(),
-                                         self._descs([self.lc], ("lc",)), self._descs(swap[2:3], ("lc",)), 1,
-                                         self.stream()), "gather_dats")
-            p2c, lc = swap[3], swap[2]
+        pending = getattr(self, "pending", False)
+        if pending:
+            p2c, lc = self._swap_buffers()[3], self.lc
         else:
             self.materialize()
             p2c, lc = self.p2c, self.lc
@@ -680,12 +680,16 @@ class FempicDevice:
             self.charge_f32 = torch.zeros(self.nn, dtype=torch.float32, device=self.dev)
             self.dep_bytes = int(lib().oppx_fempic_deposit_scratch_bytes(self.cap))
             self.dep_scratch = torch.zeros(self.dep_bytes, dtype=torch.uint8, device=self.dev)
-        check(lib().oppx_fempic_deposit_sorted_f32(self.n_max, self.cap, self.counters.data_ptr(), p2c.data_ptr(),
-                                                   lc.data_ptr(), self.nc, self.node_start.data_ptr(), None,
-                                                   self.node_ref.data_ptr(), self.nn, self.cell_partials.data_ptr(),
-                                                   None, self.charge_f32.data_ptr(), self.dep_scratch.data_ptr(),
-                                                   self.dep_bytes, self.stream()),
-              "deposit_sorted_f32")
+        tail = (self.nc, self.node_start.data_ptr(), None, self.node_ref.data_ptr(), self.nn,
+                self.cell_partials.data_ptr(), None, self.charge_f32.data_ptr(), self.dep_scratch.data_ptr(),
+                self.dep_bytes, self.stream())
+        if pending:
+            check(lib().oppx_fempic_deposit_sorted_f32_perm(self.n_max, self.cap, self.counters.data_ptr(),
+                                                            p2c.data_ptr(), lc.data_ptr(), self.perm.data_ptr(), *tail),
+                  "deposit_sorted_f32_perm")
+        else:
+            check(lib().oppx_fempic_deposit_sorted_f32(self.n_max, self.cap, self.counters.data_ptr(), p2c.data_ptr(),
+                                                       lc.data_ptr(), *tail), "deposit_sorted_f32")
         return self.charge_f32
 
     def _rebind(self):

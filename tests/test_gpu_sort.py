@@ -188,6 +188,48 @@ def test_gpu_lazy_sort_steps_equal_eager_sort_and_oracle(every):
         assert np.abs(a["charge"] - e["charge"]).max() <= 1e-12 * np.abs(e["charge"]).max()
 
 
+@pytest.mark.parametrize("nx,npart", [(3, 60000), (6, 30000)])
+def test_gpu_f32_deposit_through_perm_equals_sorted(nx, npart):
+    """The FP32 deposit over a lazy sort (lc read through the sort's permutation,
+    oppx_fempic_deposit_sorted_f32_perm) gives the bits of the deposit after the permutation is applied:
+    runs of ~190 particles per cell (nx=3, summed by whole waves) and of ~11 (nx=6, one lane each). A
+    permutation entry outside the dats reads particle 0 and counts one fault."""
+    m = oppx.box_mesh(nx, nx, nx)
+    ef = oppx.box_field_sines(m)
+    pos, vel, p2c = oppx.box_particles(npart, 13, 0.02, m)
+    gs = 1.0 / 12
+    ov = oppx.dh_overlay(m, (0, 0, 0), (1, 1, 1), gs)
+    g = oppx.FempicDevice(m, ef, pos, vel, p2c, overlay=ov, bmin=(0, 0, 0), bmax=(1, 1, 1), gs=gs)
+    g.step_begin()
+    g.pass_fused(1e-3, 1.0, 0.5 / nx / (0.02 * math.sqrt(3.0)))
+    g.sort(drop_removed=True, lazy=True)
+    assert g.pending
+    through_perm = g.deposit_sorted_f32().clone()
+    g.materialize()
+    sorted_lc = g.deposit_sorted_f32().clone()
+    assert torch.equal(through_perm, sorted_lc)
+    c = g.read_counters()
+    assert c["faults"] == 0
+    n = int(c["size"])
+    assert n > 1000
+    # a permutation with one entry past the dats (the identity otherwise: lc is now in sorted order)
+    perm = torch.arange(g.cap, dtype=torch.int32, device="cuda")
+    perm[n // 2] = g.cap + 5
+    rc = oppx.lib().oppx_fempic_deposit_sorted_f32_perm(
+        g.n_max, g.cap, g.counters.data_ptr(), g.p2c.data_ptr(), g.lc.data_ptr(), perm.data_ptr(), g.nc,
+        g.node_start.data_ptr(), None, g.node_ref.data_ptr(), g.nn, g.cell_partials.data_ptr(), None,
+        g.charge_f32.data_ptr(), g.dep_scratch.data_ptr(), g.dep_bytes, g.stream())
+    assert rc == 0
+    assert g.read_counters()["faults"] == 1
+    # the identity permutation gives the sorted deposit again
+    perm[n // 2] = n // 2
+    assert oppx.lib().oppx_fempic_deposit_sorted_f32_perm(
+        g.n_max, g.cap, g.counters.data_ptr(), g.p2c.data_ptr(), g.lc.data_ptr(), perm.data_ptr(), g.nc,
+        g.node_start.data_ptr(), None, g.node_ref.data_ptr(), g.nn, g.cell_partials.data_ptr(), None,
+        g.charge_f32.data_ptr(), g.dep_scratch.data_ptr(), g.dep_bytes, g.stream()) == 0
+    assert torch.equal(g.charge_f32, sorted_lc)
+
+
 def test_gpu_f32_deposit_reports_unsorted_p2c():
     """The FP32 deposit's precondition (p2c sorted, removed particles last) is checked: sorted input
     adds no fault; each run start that follows a larger cell or a removed particle adds one."""
