@@ -557,13 +557,16 @@ int oppx_fempic_deposit_sorted_f32(int64_t n_max, int64_t stride, oppx_move_coun
 
 /* oppx_fempic_deposit_sorted_f32 over a lazy sort (the same deposit bit for bit): p2c is sorted, while lc is
  * still in the order before the sort, and perm[q] (oppx_sort_by_cell's perm_out) is the index in lc of
- * sorted slot q. The run sums read lc through perm, so the sorted lc is never written. A perm entry outside
- * [0, stride) reads particle 0 and adds one to counters->faults. */
+ * sorted slot q. The run sums read lc through perm, so the sorted lc is never written. lc_f32 (nullable,
+ * 16-byte aligned, 4 * n_max floats of the caller's free memory) receives lc in FP32 in sorted order, from
+ * which runs longer than 64 particles are summed instead of gathering through perm again. A perm entry
+ * outside [0, stride) reads particle 0 and adds one to counters->faults. */
 int oppx_fempic_deposit_sorted_f32_perm(int64_t n_max, int64_t stride, oppx_move_counters* counters,
-                                        const int32_t* p2c, const double* lc, const int32_t* perm, int64_t n_cells,
-                                        const int32_t* node_start, const int32_t* node_end, const int32_t* node_ref,
-                                        int64_t n_nodes, float* cell_partials, double* node_charge,
-                                        float* node_charge_f32, void* scratch, size_t scratch_bytes, void* stream);
+                                        const int32_t* p2c, const double* lc, const int32_t* perm, float* lc_f32,
+                                        int64_t n_cells, const int32_t* node_start, const int32_t* node_end,
+                                        const int32_t* node_ref, int64_t n_nodes, float* cell_partials,
+                                        double* node_charge, float* node_charge_f32, void* scratch,
+                                        size_t scratch_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------------
  * Sort

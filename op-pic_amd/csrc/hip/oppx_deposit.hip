@@ -6,8 +6,9 @@
 // opp_lib/include/device_kernels/hip_inline_kernels.h:396-535). Here particles are already ordered
 // by cell (oppx_sort_by_cell), so:
 //   pass 1: the heads of the runs of equal cells are compacted (block counts, scan, scatter), then
-//           one lane per run sums lc[0..3] of its run in index order (FP32 Kahan) -- every lane of a
-//           wave owns a run instead of one lane in ~(run length) holding it. A run longer than
+//           one lane per run sums lc[0..3] of its run in index order (FP32 Kahan), from a tile of lc
+//           staged in LDS with coalesced loads -- every lane of a wave owns a run instead of one lane
+//           in ~(run length) holding it. A run longer than
 //           kLongRun particles is summed by a whole wave instead: lane l takes particles
 //           head + l, head + l + 64, ... (Kahan), and the 64 lane sums are added in lane order, so a
 //           long run no longer holds its wave for its whole length. Either way the order depends only
@@ -39,8 +40,8 @@ __global__ void __launch_bounds__(kB) zero_partials(float* __restrict__ part, in
     for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB) part[i] = 0.0f;
 }
 
-constexpr int kRunItems = 16;                 // particles per thread in the head compaction
-constexpr int kRunElems = kB * kRunItems;      // particles per block
+constexpr int kRunItems = 8;                  // particles per thread in the head compaction
+constexpr int kRunElems = kB * kRunItems;      // particles per tile
 
 __device__ __forceinline__ int64_t live_of(const oppx_move_counters* ctr, int64_t n_max) {
     const int64_t n = ctr->size;
@@ -48,12 +49,12 @@ __device__ __forceinline__ int64_t live_of(const oppx_move_counters* ctr, int64_
 }
 
 // A block's p2c tile (its kRunElems entries and the one before) staged in LDS with coalesced loads,
-// one padding word per 16 so that thread t's 16 consecutive entries sit in distinct banks.
+// one padding word per 8 so that thread t's 8 consecutive entries (from word 9t) sit in distinct banks.
 struct RunTile {
-    int32_t v[kRunElems + 1 + (kRunElems + 1) / 16 + 1];
+    int32_t v[kRunElems + 1 + (kRunElems + 1) / 8 + 1];
 };
 
-__device__ __forceinline__ int tile_at(int e) { return e + (e >> 4); }
+__device__ __forceinline__ int tile_at(int e) { return e + (e >> 3); }
 
 __device__ __forceinline__ void load_run_tile(RunTile& T, const int32_t* __restrict__ p2c, int64_t base0, int64_t n) {
     for (int t = threadIdx.x; t < kRunElems + 1; t += kB) {
@@ -69,9 +70,10 @@ __device__ __forceinline__ bool run_head(const RunTile& T, int e, int64_t n_cell
     return (unsigned)c < (unsigned)n_cells && T.v[tile_at(e)] != c;
 }
 
-// exclusive block scan of one uint32 per thread (kB threads); total in *tot
+// exclusive block scan of one uint32 per thread (NT threads); total in *tot
+template <int NT = kB>
 __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* tot) {
-    __shared__ uint32_t s_w[kB / kWave];
+    __shared__ uint32_t s_w[NT / kWave];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     uint32_t inc = v;
 #pragma unroll
@@ -82,7 +84,7 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* tot) {
     if (lane == kWave - 1) s_w[w] = inc;
     __syncthreads();
     uint32_t off = 0, all = 0;
-    for (int q = 0; q < kB / kWave; q++) {
+    for (int q = 0; q < NT / kWave; q++) {
         if (q < w) off += s_w[q];
         all += s_w[q];
     }
@@ -123,18 +125,19 @@ __global__ void __launch_bounds__(kB) run_count(const int32_t* __restrict__ p2c,
     if ((threadIdx.x & (kWave - 1)) == 0 && unsorted) atomicAdd(&ctr->faults, unsorted);
 }
 
-// single block: exclusive scan of the live tiles' counts in place; the total goes to n_runs[0], and the
-// long-run count n_runs[1] starts at zero
-__global__ void __launch_bounds__(kB) run_scan(uint32_t* __restrict__ bsum, const oppx_move_counters* ctr,
-                                               int64_t n_max, uint32_t* __restrict__ n_runs) {
+// single block of kScanThreads: exclusive scan of the live tiles' counts in place; the total goes to
+// n_runs[0], and the long-run count n_runs[1] starts at zero
+constexpr int kScanThreads = 1024;
+__global__ void __launch_bounds__(kScanThreads) run_scan(uint32_t* __restrict__ bsum, const oppx_move_counters* ctr,
+                                                         int64_t n_max, uint32_t* __restrict__ n_runs) {
     if (threadIdx.x == 0) n_runs[1] = 0u;
     const int64_t nb = (live_of(ctr, n_max) + kRunElems - 1) / kRunElems;
     uint32_t carry = 0;
-    for (int64_t b0 = 0; b0 < nb; b0 += kB) {
+    for (int64_t b0 = 0; b0 < nb; b0 += kScanThreads) {
         const int64_t j = b0 + threadIdx.x;
         const uint32_t v = j < nb ? bsum[j] : 0u;
         uint32_t tot;
-        const uint32_t ex = block_scan(v, &tot);
+        const uint32_t ex = block_scan<kScanThreads>(v, &tot);
         if (j < nb) bsum[j] = carry + ex;
         carry += tot;
     }
@@ -174,12 +177,8 @@ __device__ __forceinline__ int64_t run_end(const int32_t* __restrict__ p2c, cons
     return e;
 }
 
-// Kahan sums of lc[0..3] of particles [j, e) in index order; with PERM the particle at sorted position q
-// has its lc at perm[q] (lc left in the order before the sort). Four particles at a time: their indexes,
-// then their 16 lc words, are loaded before any of them is added, so one lane keeps 16 loads in flight;
-// the additions keep index order.
-// A permutation entry outside [0, stride) is read as particle 0 (never outside the dat) and counted in
-// the return value.
+// lc index of the particle at sorted position q: perm[q] with PERM (lc left in the order before the
+// sort), else q. An entry outside [0, stride) gives particle 0 (never outside the dat) and counts in bad.
 template <bool PERM>
 __device__ __forceinline__ int perm_at(const int32_t* __restrict__ perm, int64_t q, int64_t stride, int& bad) {
     if (!PERM) return (int)q;
@@ -189,6 +188,9 @@ __device__ __forceinline__ int perm_at(const int32_t* __restrict__ perm, int64_t
     return 0;
 }
 
+// Kahan sums of lc[0..3] of particles j, j + step, ... below e, in that order, through perm_at. Four
+// particles at a time: their indexes, then their 16 lc words, are loaded before any of them is added, so
+// one lane keeps 16 loads in flight; the additions keep index order. Returns the bad permutation entries.
 template <bool PERM>
 __device__ __forceinline__ int run_sums(const double* __restrict__ lc, const int32_t* __restrict__ perm,
                                         int64_t stride, int64_t j, int64_t e, int64_t step, Kahan (&k)[4]) {
@@ -216,37 +218,82 @@ __device__ __forceinline__ int run_sums(const double* __restrict__ lc, const int
     return bad;
 }
 
+// One block per tile of kRunElems particles (the tiles of run_count): the lc of the tile's particles and
+// of the kLongRun after it, in FP32, is staged in LDS with coalesced loads (with PERM, loads through the
+// permutation), then one lane per run that starts in the tile (heads[bsum[t] .. bsum[t + 1])) sums the
+// run from LDS in index order (Kahan); a run of at most kLongRun particles ends within kLongRun of the
+// tile's end. Longer runs are listed for cell_runs_long. The sums are those of lane-per-run loops over
+// lc in global memory, bit for bit. lc_f32 (nullable) receives the staged FP32 values of every live
+// particle in sorted order (16 B each, coalesced), from which cell_runs_long then reads instead of
+// gathering through the permutation a second time. The grid strides over the live tiles.
+constexpr int kLcThreads = 512;
+struct LcTile {
+    float v[4][kRunElems + kLongRun];
+};
+
 template <bool PERM>
-__global__ void __launch_bounds__(kB) cell_runs(const int32_t* __restrict__ p2c, const double* __restrict__ lc,
-                                                const int32_t* __restrict__ perm, int64_t stride,
-                                                oppx_move_counters* ctr, int64_t n_max,
-                                                const int32_t* __restrict__ heads, uint32_t* __restrict__ n_runs,
-                                                int32_t* __restrict__ long_runs, float* __restrict__ part) {
-    const int64_t r = (int64_t)blockIdx.x * kB + threadIdx.x;
-    const int64_t nr = (int64_t)n_runs[0];
-    if (r >= nr) return;
+__global__ void __launch_bounds__(kLcThreads) cell_runs(const int32_t* __restrict__ p2c, const double* __restrict__ lc,
+                                                        const int32_t* __restrict__ perm, int64_t stride,
+                                                        oppx_move_counters* ctr, int64_t n_max,
+                                                        const int32_t* __restrict__ heads,
+                                                        const uint32_t* __restrict__ bsum, uint32_t* __restrict__ n_runs,
+                                                        int32_t* __restrict__ long_runs, float* __restrict__ part,
+                                                        float4* __restrict__ lc_f32) {
+    __shared__ LcTile T;
     const int64_t n = live_of(ctr, n_max);
-    const int64_t j = heads[r];
-    const int32_t c = p2c[j];
-    const int64_t e = run_end(p2c, heads, r, nr, n, c);
-    if (e - j > kLongRun) {  // summed by cell_runs_long
-        long_runs[atomicAdd(n_runs + 1, 1u)] = (int32_t)r;
-        return;
+    const int64_t tiles = (n + kRunElems - 1) / kRunElems;
+    const int64_t nr = (int64_t)n_runs[0];
+    int bad = 0, bad_over = 0;  // bad permutation entries of the tile's own particles / of the overlap
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t base = t * kRunElems;
+        for (int e = threadIdx.x; e < kRunElems + kLongRun; e += kLcThreads) {
+            const int64_t q = base + e;
+            float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f, v3 = 0.0f;
+            if (q < n) {
+                const int64_t p = PERM ? (int64_t)perm_at<PERM>(perm, q, stride, e < kRunElems ? bad : bad_over) : q;
+                v0 = (float)lc[p];
+                v1 = (float)lc[stride + p];
+                v2 = (float)lc[2 * stride + p];
+                v3 = (float)lc[3 * stride + p];
+            }
+            T.v[0][e] = v0;
+            T.v[1][e] = v1;
+            T.v[2][e] = v2;
+            T.v[3][e] = v3;
+            if (lc_f32 && e < kRunElems && q < n) lc_f32[q] = make_float4(v0, v1, v2, v3);  // sorted FP32 copy
+        }
+        __syncthreads();
+        const int64_t r_end = t + 1 < tiles ? (int64_t)bsum[t + 1] : nr;
+        for (int64_t r = (int64_t)bsum[t] + threadIdx.x; r < r_end; r += kLcThreads) {
+            const int64_t j = heads[r];
+            const int32_t c = p2c[j];
+            const int64_t e = run_end(p2c, heads, r, nr, n, c);
+            if (e - j > kLongRun) {  // summed by cell_runs_long
+                long_runs[atomicAdd(n_runs + 1, 1u)] = (int32_t)r;
+                continue;
+            }
+            Kahan k0, k1, k2, k3;
+            for (int q = (int)(j - base); q < (int)(e - base); q++) {
+                k0.add(T.v[0][q]);
+                k1.add(T.v[1][q]);
+                k2.add(T.v[2][q]);
+                k3.add(T.v[3][q]);
+            }
+            float4 v;
+            v.x = k0.s; v.y = k1.s; v.z = k2.s; v.w = k3.s;
+            *reinterpret_cast<float4*>(part + 4 * (int64_t)c) = v;
+        }
+        __syncthreads();  // every lane is done with the tile before the next one is staged
     }
-    Kahan k[4];
-    const int bad = run_sums<PERM>(lc, perm, stride, j, e, 1, k);
     if (PERM && bad) atomicAdd(&ctr->faults, bad);
-    float4 v;
-    v.x = k[0].s; v.y = k[1].s; v.z = k[2].s; v.w = k[3].s;
-    *reinterpret_cast<float4*>(part + 4 * (int64_t)c) = v;
 }
 
 // One wave per long run (waves stride over the list): lane l sums particles head + l + 64 t in index
 // order (Kahan), then lane 0 adds the 64 lane sums in lane order (Kahan).
-template <bool PERM>
+template <bool PERM, bool F32>
 __global__ void __launch_bounds__(kB) cell_runs_long(const int32_t* __restrict__ p2c, const double* __restrict__ lc,
-                                                     const int32_t* __restrict__ perm, int64_t stride,
-                                                     oppx_move_counters* ctr, int64_t n_max,
+                                                     const int32_t* __restrict__ perm, const float4* __restrict__ lc_f32,
+                                                     int64_t stride, const oppx_move_counters* ctr, int64_t n_max,
                                                      const int32_t* __restrict__ heads,
                                                      const uint32_t* __restrict__ n_runs,
                                                      const int32_t* __restrict__ long_runs, float* __restrict__ part) {
@@ -261,8 +308,17 @@ __global__ void __launch_bounds__(kB) cell_runs_long(const int32_t* __restrict__
         const int32_t c = p2c[j];
         const int64_t e = run_end(p2c, heads, r, nr, n, c);
         Kahan ks[4];
-        const int bad = run_sums<PERM>(lc, perm, stride, j + lane, e, kWave, ks);
-        if (PERM && bad) atomicAdd(&ctr->faults, bad);
+        if constexpr (F32) {  // the sorted FP32 copy: the same values, read in the same order
+            for (int64_t q = j + lane; q < e; q += kWave) {
+                const float4 v = lc_f32[q];
+                ks[0].add(v.x);
+                ks[1].add(v.y);
+                ks[2].add(v.z);
+                ks[3].add(v.w);
+            }
+        } else {
+            run_sums<PERM>(lc, perm, stride, j + lane, e, kWave, ks);  // bad entries are counted by cell_runs
+        }
         Kahan t0, t1, t2, t3;
         for (int l = 0; l < kWave; l++) {  // the same on every lane: lane l's sums, in lane order
             t0.add(__shfl(ks[0].s, l));
@@ -333,7 +389,7 @@ __global__ void __launch_bounds__(kB) node_gather(const int32_t* __restrict__ no
 // The deposit over cell-sorted p2c, with lc in the same order (perm null) or in the order before the sort
 // (perm[q]: the source index of sorted slot q).
 int deposit_f32(const char* name, int64_t n_max, int64_t stride, oppx_move_counters* counters, const int32_t* p2c,
-                const double* lc, const int32_t* perm, int64_t n_cells, const int32_t* node_start,
+                const double* lc, const int32_t* perm, float* lc_f32, int64_t n_cells, const int32_t* node_start,
                 const int32_t* node_end, const int32_t* node_ref, int64_t n_nodes, float* cell_partials,
                 double* node_charge, float* node_charge_f32, void* scratch, size_t scratch_bytes, void* stream) {
     const std::string fn(name);
@@ -352,26 +408,35 @@ int deposit_f32(const char* name, int64_t n_max, int64_t stride, oppx_move_count
     OPPX_LAUNCH_CHECK("zero_partials");
     if (n_max > 0) {
         RunScratch rs = run_carve(scratch, n_max);
-        // grids that stride over the live tiles; one thread per run, at most one run per cell
-        const int64_t max_runs = n_max < n_cells ? n_max : n_cells;
+        // grids that stride over the live tiles
         run_count<<<stride_grid(reinterpret_cast<const void*>(run_count), kB, rs.nb, kGridMult), kB, 0, s>>>(
             p2c, counters, n_max, n_cells, rs.bsum);
-        run_scan<<<1, kB, 0, s>>>(rs.bsum, counters, n_max, rs.n_runs);
+        run_scan<<<1, kScanThreads, 0, s>>>(rs.bsum, counters, n_max, rs.n_runs);
         run_compact<<<stride_grid(reinterpret_cast<const void*>(run_compact), kB, rs.nb, kGridMult), kB, 0, s>>>(
             p2c, counters, n_max, n_cells, rs.bsum, rs.heads);
-        const unsigned gr = grid_for(max_runs, kB);
         int64_t gl = grid_for(n_max, kLongRun * (kB / kWave));  // at most one long run per kLongRun particles
         if (gl > 2048) gl = 2048;
+        float4* f32 = reinterpret_cast<float4*>(lc_f32);
         if (perm) {
-            cell_runs<true><<<gr, kB, 0, s>>>(p2c, lc, perm, stride, counters, n_max, rs.heads, rs.n_runs,
-                                               rs.long_runs, cell_partials);
-            cell_runs_long<true><<<(unsigned)gl, kB, 0, s>>>(p2c, lc, perm, stride, counters, n_max, rs.heads,
-                                                              rs.n_runs, rs.long_runs, cell_partials);
+            auto kern = cell_runs<true>;
+            kern<<<stride_grid(reinterpret_cast<const void*>(kern), kLcThreads, rs.nb, kGridMult), kLcThreads, 0, s>>>(
+                p2c, lc, perm, stride, counters, n_max, rs.heads, rs.bsum, rs.n_runs, rs.long_runs, cell_partials, f32);
+            if (f32)
+                cell_runs_long<true, true><<<(unsigned)gl, kB, 0, s>>>(p2c, lc, perm, f32, stride, counters, n_max,
+                                                                        rs.heads, rs.n_runs, rs.long_runs,
+                                                                        cell_partials);
+            else
+                cell_runs_long<true, false><<<(unsigned)gl, kB, 0, s>>>(p2c, lc, perm, nullptr, stride, counters,
+                                                                         n_max, rs.heads, rs.n_runs, rs.long_runs,
+                                                                         cell_partials);
         } else {
-            cell_runs<false><<<gr, kB, 0, s>>>(p2c, lc, nullptr, stride, counters, n_max, rs.heads, rs.n_runs,
-                                                rs.long_runs, cell_partials);
-            cell_runs_long<false><<<(unsigned)gl, kB, 0, s>>>(p2c, lc, nullptr, stride, counters, n_max, rs.heads,
-                                                               rs.n_runs, rs.long_runs, cell_partials);
+            auto kern = cell_runs<false>;
+            kern<<<stride_grid(reinterpret_cast<const void*>(kern), kLcThreads, rs.nb, kGridMult), kLcThreads, 0, s>>>(
+                p2c, lc, nullptr, stride, counters, n_max, rs.heads, rs.bsum, rs.n_runs, rs.long_runs, cell_partials,
+                nullptr);
+            cell_runs_long<false, false><<<(unsigned)gl, kB, 0, s>>>(p2c, lc, nullptr, nullptr, stride, counters,
+                                                                      n_max, rs.heads, rs.n_runs, rs.long_runs,
+                                                                      cell_partials);
         }
         OPPX_LAUNCH_CHECK("cell_runs");
     }
@@ -394,21 +459,26 @@ int oppx_fempic_deposit_sorted_f32(int64_t n_max, int64_t stride, oppx_move_coun
                                    const double* lc, int64_t n_cells, const int32_t* node_start, const int32_t* node_end,
                                    const int32_t* node_ref, int64_t n_nodes, float* cell_partials, double* node_charge,
                                    float* node_charge_f32, void* scratch, size_t scratch_bytes, void* stream) {
-    return deposit_f32("oppx_fempic_deposit_sorted_f32", n_max, stride, counters, p2c, lc, nullptr, n_cells,
+    return deposit_f32("oppx_fempic_deposit_sorted_f32", n_max, stride, counters, p2c, lc, nullptr, nullptr, n_cells,
                        node_start, node_end, node_ref, n_nodes, cell_partials, node_charge, node_charge_f32, scratch,
                        scratch_bytes, stream);
 }
 
 int oppx_fempic_deposit_sorted_f32_perm(int64_t n_max, int64_t stride, oppx_move_counters* counters,
-                                        const int32_t* p2c, const double* lc, const int32_t* perm, int64_t n_cells,
-                                        const int32_t* node_start, const int32_t* node_end, const int32_t* node_ref,
-                                        int64_t n_nodes, float* cell_partials, double* node_charge,
-                                        float* node_charge_f32, void* scratch, size_t scratch_bytes, void* stream) {
+                                        const int32_t* p2c, const double* lc, const int32_t* perm, float* lc_f32,
+                                        int64_t n_cells, const int32_t* node_start, const int32_t* node_end,
+                                        const int32_t* node_ref, int64_t n_nodes, float* cell_partials,
+                                        double* node_charge, float* node_charge_f32, void* scratch,
+                                        size_t scratch_bytes, void* stream) {
     if (!perm) {
         set_error("oppx_fempic_deposit_sorted_f32_perm: null perm");
         return OPPX_ERR_ARG;
     }
-    return deposit_f32("oppx_fempic_deposit_sorted_f32_perm", n_max, stride, counters, p2c, lc, perm, n_cells,
+    if (lc_f32 && (reinterpret_cast<uintptr_t>(lc_f32) & 15)) {
+        set_error("oppx_fempic_deposit_sorted_f32_perm: lc_f32 not 16-byte aligned");
+        return OPPX_ERR_ARG;
+    }
+    return deposit_f32("oppx_fempic_deposit_sorted_f32_perm", n_max, stride, counters, p2c, lc, perm, lc_f32, n_cells,
                        node_start, node_end, node_ref, n_nodes, cell_partials, node_charge, node_charge_f32, scratch,
                        scratch_bytes, stream);
 }

@@ -177,9 +177,10 @@ _SIGS = {
     "oppx_fempic_deposit_sorted_f32": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
                                                c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
                                                ctypes.c_size_t, c_void_p]),
-    "oppx_fempic_deposit_sorted_f32_perm": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
-                                                    c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
-                                                    c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
+    "oppx_fempic_deposit_sorted_f32_perm": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                                    c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
+                                                    c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_size_t,
+                                                    c_void_p]),
     "oppx_lower_bound": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "oppx_sort_scratch_bytes": (c_size_t, [c_int64]),
     "oppx_sort_by_cell": (c_int, [c_int64, c_int64, c_void_p, c_int64, c_void_p, POINTER(DatDesc), POINTER(DatDesc),
@@ -668,7 +669,9 @@ class FempicDevice:
         torch = self.torch
         pending = getattr(self, "pending", False)
         if pending:
-            p2c, lc = self._swap_buffers()[3], self.lc
+            swap = self._swap_buffers()
+            # the swap lc buffer (32 B a particle, overwritten by the next pass) takes the sorted FP32 lc
+            p2c, lc, lc_f32 = swap[3], self.lc, swap[2]
         else:
             self.materialize()
             p2c, lc = self.p2c, self.lc
@@ -685,8 +688,8 @@ class FempicDevice:
                 self.dep_bytes, self.stream())
         if pending:
             check(lib().oppx_fempic_deposit_sorted_f32_perm(self.n_max, self.cap, self.counters.data_ptr(),
-                                                            p2c.data_ptr(), lc.data_ptr(), self.perm.data_ptr(), *tail),
-                  "deposit_sorted_f32_perm")
+                                                            p2c.data_ptr(), lc.data_ptr(), self.perm.data_ptr(),
+                                                            lc_f32.data_ptr(), *tail), "deposit_sorted_f32_perm")
         else:
             check(lib().oppx_fempic_deposit_sorted_f32(self.n_max, self.cap, self.counters.data_ptr(), p2c.data_ptr(),
                                                        lc.data_ptr(), *tail), "deposit_sorted_f32")

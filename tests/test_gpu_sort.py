@@ -216,7 +216,7 @@ def test_gpu_f32_deposit_through_perm_equals_sorted(nx, npart):
     perm = torch.arange(g.cap, dtype=torch.int32, device="cuda")
     perm[n // 2] = g.cap + 5
     rc = oppx.lib().oppx_fempic_deposit_sorted_f32_perm(
-        g.n_max, g.cap, g.counters.data_ptr(), g.p2c.data_ptr(), g.lc.data_ptr(), perm.data_ptr(), g.nc,
+        g.n_max, g.cap, g.counters.data_ptr(), g.p2c.data_ptr(), g.lc.data_ptr(), perm.data_ptr(), None, g.nc,
         g.node_start.data_ptr(), None, g.node_ref.data_ptr(), g.nn, g.cell_partials.data_ptr(), None,
         g.charge_f32.data_ptr(), g.dep_scratch.data_ptr(), g.dep_bytes, g.stream())
     assert rc == 0
@@ -224,7 +224,7 @@ def test_gpu_f32_deposit_through_perm_equals_sorted(nx, npart):
     # the identity permutation gives the sorted deposit again
     perm[n // 2] = n // 2
     assert oppx.lib().oppx_fempic_deposit_sorted_f32_perm(
-        g.n_max, g.cap, g.counters.data_ptr(), g.p2c.data_ptr(), g.lc.data_ptr(), perm.data_ptr(), g.nc,
+        g.n_max, g.cap, g.counters.data_ptr(), g.p2c.data_ptr(), g.lc.data_ptr(), perm.data_ptr(), None, g.nc,
         g.node_start.data_ptr(), None, g.node_ref.data_ptr(), g.nn, g.cell_partials.data_ptr(), None,
         g.charge_f32.data_ptr(), g.dep_scratch.data_ptr(), g.dep_bytes, g.stream()) == 0
     assert torch.equal(g.charge_f32, sorted_lc)
