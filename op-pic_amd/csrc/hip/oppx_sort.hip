@@ -65,12 +65,13 @@ __device__ __forceinline__ uint32_t cell_key(int32_t c, uint32_t n_cells) {
 template <int ITEMS, bool FIRST>
 __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(8))) sort_hist(const uint32_t* __restrict__ keys, const int32_t* __restrict__ p2c,
                                                 uint32_t n_cells, const oppx_move_counters* ctr, int64_t n_max,
-                                                int shift, uint32_t* __restrict__ hist, int64_t nblocks,
+                                                int shift, uint32_t* __restrict__ hist,
                                                 unsigned long long* __restrict__ n_removed) {
     __shared__ uint32_t s_h[kBins];
     const int64_t n = live_n(ctr, n_max);
     const int lane = threadIdx.x & (kWave - 1);
-    // the grid strides over the tiles below the live count (tile t's counts go to column t)
+    // the grid strides over the tiles below the live count; tile t's count of digit d goes to
+    // hist[d * tiles + t], so the scan covers the live tiles only
     const int64_t tiles = (n + kB * ITEMS - 1) / (kB * ITEMS);
     unsigned long long removed = 0;
     for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
@@ -100,15 +101,11 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(8))) so
             }
         }
         __syncthreads();
-        hist[(int64_t)threadIdx.x * nblocks + t] = s_h[threadIdx.x];
+        hist[(int64_t)threadIdx.x * tiles + t] = s_h[threadIdx.x];  // digit-major rows of the live tiles
     }
     // per-wave removed counts land on one of kRemSlots counters (block id modulo kRemSlots): one
     // shared counter would serialise every wave's atomic in one L2 channel
     if (FIRST && lane == 0 && removed) atomicAdd(n_removed + (blockIdx.x % kRemSlots), removed);
-    // the columns of the tiles past the live count hold zero counts for the scan (row by row, with
-    // consecutive threads on consecutive columns)
-    for (int64_t c = tiles + (int64_t)blockIdx.x * kB + threadIdx.x; c < nblocks; c += (int64_t)gridDim.x * kB)
-        for (int d = 0; d < kBins; d++) hist[(int64_t)d * nblocks + c] = 0u;
 }
 
 // ---- one digit pass over a tile of 256 * ITEMS consecutive elements ----
@@ -243,7 +240,7 @@ template <int ITEMS, bool LAST, bool FIRST>
 __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4)))
 sort_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, const int32_t* __restrict__ p2c,
              uint32_t n_cells, const oppx_move_counters* ctr, int64_t n_max, int shift,
-             const uint32_t* __restrict__ offs, int64_t nblocks, uint32_t* __restrict__ kout,
+             const uint32_t* __restrict__ offs, uint32_t* __restrict__ kout,
              uint32_t* __restrict__ vout, int32_t* __restrict__ p2c_out) {
     __shared__ TileLds<ITEMS> L;
     const int64_t n = live_n(ctr, n_max);
@@ -259,7 +256,7 @@ sort_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
         int tid = (int)threadIdx.x;
         asm volatile("" : "+v"(tid));
         for (int q = 0; q < kB / kWave; q++) L.wave[q][tid] = 0;
-        L.dst[tid] = offs[(int64_t)tid * nblocks + t];
+        L.dst[tid] = offs[(int64_t)tid * tiles + t];
         uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
         tile_load<ITEMS, FIRST>(kin, vin, p2c, n_cells, base, m, tid, key, val);
         __syncthreads();
@@ -292,7 +289,16 @@ __device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* total)
     return off + inc - v;
 }
 
-__global__ void __launch_bounds__(kB) scan_reduce(const uint32_t* __restrict__ a, int64_t n, uint32_t* __restrict__ sums) {
+// Entries of the digit counts of the live tiles (kBins rows of live tiles of per_tile elements).
+__device__ __forceinline__ int64_t live_hist_n(const oppx_move_counters* ctr, int64_t n_max, int64_t per_tile) {
+    return (int64_t)kBins * ((live_n(ctr, n_max) + per_tile - 1) / per_tile);
+}
+
+// The scan of the live digit counts: blocks of chunks past them leave at once.
+__global__ void __launch_bounds__(kB) scan_reduce(const uint32_t* __restrict__ a, const oppx_move_counters* ctr,
+                                                  int64_t n_max, int64_t per_tile, uint32_t* __restrict__ sums) {
+    const int64_t n = live_hist_n(ctr, n_max, per_tile);
+    if ((int64_t)blockIdx.x * kElems >= n) return;
     uint32_t s = 0;
     for (int r = 0; r < kItems; r++) {
         const int64_t i = (int64_t)blockIdx.x * kElems + (int64_t)threadIdx.x * kItems + r;
@@ -303,7 +309,9 @@ __global__ void __launch_bounds__(kB) scan_reduce(const uint32_t* __restrict__ a
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(kB) scan_sums(uint32_t* __restrict__ sums, int64_t nb) {
+__global__ void __launch_bounds__(kB) scan_sums(uint32_t* __restrict__ sums, const oppx_move_counters* ctr,
+                                                int64_t n_max, int64_t per_tile) {
+    const int64_t nb = (live_hist_n(ctr, n_max, per_tile) + kElems - 1) / kElems;
     uint32_t carry = 0;
     for (int64_t base = 0; base < nb; base += kB) {
         const int64_t j = base + threadIdx.x;
@@ -315,7 +323,10 @@ __global__ void __launch_bounds__(kB) scan_sums(uint32_t* __restrict__ sums, int
     }
 }
 
-__global__ void __launch_bounds__(kB) scan_apply(uint32_t* __restrict__ a, int64_t n, const uint32_t* __restrict__ sums) {
+__global__ void __launch_bounds__(kB) scan_apply(uint32_t* __restrict__ a, const oppx_move_counters* ctr,
+                                                 int64_t n_max, int64_t per_tile, const uint32_t* __restrict__ sums) {
+    const int64_t n = live_hist_n(ctr, n_max, per_tile);
+    if ((int64_t)blockIdx.x * kElems >= n) return;
     uint32_t loc[kItems];
     uint32_t s = 0;
     for (int r = 0; r < kItems; r++) {
@@ -479,22 +490,23 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
             {
 #define OPPX_HIST(I, F)                                                                              \
     sort_hist<I, F><<<stride_grid(reinterpret_cast<const void*>(sort_hist<I, F>), kB, sc.nblocks, kGridMult), kB, 0, \
-                      s>>>(kin, p2c, kc, counters, n_max, shift, sc.hist, sc.nblocks, sc.n_removed)
+                      s>>>(kin, p2c, kc, counters, n_max, shift, sc.hist, sc.n_removed)
                 if (items == 4) { if (first) OPPX_HIST(4, true); else OPPX_HIST(4, false); }
                 else if (items == 8) { if (first) OPPX_HIST(8, true); else OPPX_HIST(8, false); }
                 else { if (first) OPPX_HIST(16, true); else OPPX_HIST(16, false); }
 #undef OPPX_HIST
                 OPPX_LAUNCH_CHECK("sort_hist");
-                scan_reduce<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
-                scan_sums<<<1, kB, 0, s>>>(sc.sums, sc.scan_blocks);
-                scan_apply<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, sc.hist_n, sc.sums);
+                const int64_t per_tile = (int64_t)kB * items;
+                scan_reduce<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, counters, n_max, per_tile, sc.sums);
+                scan_sums<<<1, kB, 0, s>>>(sc.sums, counters, n_max, per_tile);
+                scan_apply<<<(unsigned)sc.scan_blocks, kB, 0, s>>>(sc.hist, counters, n_max, per_tile, sc.sums);
                 OPPX_LAUNCH_CHECK("sort_scan");
 #define OPPX_SCATTER(I, LST, F)                                                                      \
     do {                                                                                              \
         auto kern = sort_scatter<I, LST, F>;                                                          \
         const unsigned res = resident_blocks(reinterpret_cast<const void*>(kern), kB);               \
         const unsigned g = (int64_t)res < sc.nblocks ? res : (unsigned)sc.nblocks;                    \
-        kern<<<g, kB, 0, s>>>(kin, vin, p2c, kc, counters, n_max, shift, sc.hist, sc.nblocks, kout, vdst, \
+        kern<<<g, kB, 0, s>>>(kin, vin, p2c, kc, counters, n_max, shift, sc.hist, kout, vdst,            \
                               p2c_out);                                                              \
     } while (0)
 #define OPPX_SCATTER_LF(I)                                                     \
