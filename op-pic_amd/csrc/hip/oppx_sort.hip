@@ -300,8 +300,8 @@ __global__ void __launch_bounds__(kB) scan_reduce(const uint32_t* __restrict__ a
     const int64_t n = live_hist_n(ctr, n_max, per_tile);
     if ((int64_t)blockIdx.x * kElems >= n) return;
     uint32_t s = 0;
-    for (int r = 0; r < kItems; r++) {
-        const int64_t i = (int64_t)blockIdx.x * kElems + (int64_t)threadIdx.x * kItems + r;
+    for (int r = 0; r < kItems; r++) {  // the chunk's total: any order, so consecutive threads on consecutive words
+        const int64_t i = (int64_t)blockIdx.x * kElems + (int64_t)r * kB + threadIdx.x;
         if (i < n) s += a[i];
     }
     uint32_t tot;
@@ -325,21 +325,35 @@ __global__ void __launch_bounds__(kB) scan_sums(uint32_t* __restrict__ sums, con
 
 __global__ void __launch_bounds__(kB) scan_apply(uint32_t* __restrict__ a, const oppx_move_counters* ctr,
                                                  int64_t n_max, int64_t per_tile, const uint32_t* __restrict__ sums) {
+    // The chunk passes through LDS so that global loads and stores are coalesced while each thread scans
+    // kItems consecutive words; word e sits at e + e / kItems (thread t's words from 17 t on: distinct
+    // banks across the wave).
+    __shared__ uint32_t s_a[kElems + kElems / kItems];
     const int64_t n = live_hist_n(ctr, n_max, per_tile);
-    if ((int64_t)blockIdx.x * kElems >= n) return;
+    const int64_t base = (int64_t)blockIdx.x * kElems;
+    if (base >= n) return;
+    for (int r = 0; r < kItems; r++) {
+        const int e = r * kB + threadIdx.x;
+        s_a[e + e / kItems] = base + e < n ? a[base + e] : 0u;
+    }
+    __syncthreads();
+    const int t0 = threadIdx.x * (kItems + 1);
     uint32_t loc[kItems];
     uint32_t s = 0;
     for (int r = 0; r < kItems; r++) {
-        const int64_t i = (int64_t)blockIdx.x * kElems + (int64_t)threadIdx.x * kItems + r;
-        loc[r] = (i < n) ? a[i] : 0u;
+        loc[r] = s_a[t0 + r];
         s += loc[r];
     }
     uint32_t tot;
     uint32_t run = sums[blockIdx.x] + block_scan_excl(s, &tot);
     for (int r = 0; r < kItems; r++) {
-        const int64_t i = (int64_t)blockIdx.x * kElems + (int64_t)threadIdx.x * kItems + r;
-        if (i < n) a[i] = run;
+        s_a[t0 + r] = run;
         run += loc[r];
+    }
+    __syncthreads();
+    for (int r = 0; r < kItems; r++) {
+        const int e = r * kB + threadIdx.x;
+        if (base + e < n) a[base + e] = s_a[e + e / kItems];
     }
 }
 
