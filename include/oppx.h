@@ -101,7 +101,7 @@ typedef struct oppx_move_counters {
  * particles (multiple of 64, <= 512) and refills its free lanes once `refill_min` of them are idle;
  * `split` = 1 runs push + overlay relocation as a separate coalesced pass before the walk;
  * `persistent` = 1 launches a resident grid whose blocks take block tiles from per-XCD queues;
- * `sort_items` (4, 8 or 16) is the elements per thread of the radix sort passes (a block sorts
+ * `sort_items` (4, 8, 16 or 32) is the elements per thread of the radix sort passes (a block sorts
  * 256 * sort_items keys); `inline_hops` >= 1 selects the two-pass move (walk evaluations done in the
  * first, one-lane-per-particle pass before a particle goes to the worklist walk), 0 the
  * cooperative tile walk. Process-wide; chosen by the autotuner (op-pic_amd/autotune.py) or

@@ -1676,8 +1676,8 @@ int oppx_set_tuning(const oppx_tuning* t) {
     OPPX_ARG_CHECK(t->refill_min >= 1 && t->refill_min <= kWave, "oppx_set_tuning: refill_min in [1, 64]");
     OPPX_ARG_CHECK((t->split == 0 || t->split == 1) && (t->persistent == 0 || t->persistent == 1),
                    "oppx_set_tuning: split and persistent are 0 or 1");
-    OPPX_ARG_CHECK(t->sort_items == 4 || t->sort_items == 8 || t->sort_items == 16,
-                   "oppx_set_tuning: sort_items must be 4, 8 or 16");
+    OPPX_ARG_CHECK(t->sort_items == 4 || t->sort_items == 8 || t->sort_items == 16 || t->sort_items == 32,
+                   "oppx_set_tuning: sort_items must be 4, 8, 16 or 32");
     OPPX_ARG_CHECK(t->inline_hops >= 0 && t->inline_hops <= 64, "oppx_set_tuning: inline_hops in [0, 64]");
     OPPX_ARG_CHECK(t->first_block == kBlock || t->first_block == kMaxBlock, "oppx_set_tuning: first_block 256 or 1024");
     OPPX_ARG_CHECK(t->lanes == 1 || t->lanes == 4, "oppx_set_tuning: lanes (per particle in the first pass) 1 or 4");

@@ -484,7 +484,8 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
     }
     hipStream_t s = as_stream(stream);
     const int items = g_tuning.sort_items;
-    OPPX_ARG_CHECK(items == 4 || items == 8 || items == 16, "oppx_sort_by_cell: tuning.sort_items must be 4, 8 or 16");
+    OPPX_ARG_CHECK(items == 4 || items == 8 || items == 16 || items == 32,
+                   "oppx_sort_by_cell: tuning.sort_items must be 4, 8, 16 or 32");
     SortScratch sc = carve(scratch, n_max, items);
     int bits = 1;
     while (bits < 32 && ((int64_t)1 << bits) <= n_cells) bits++;  // keys are in [0, n_cells]
@@ -507,7 +508,8 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
                       s>>>(kin, p2c, kc, counters, n_max, shift, sc.hist, sc.n_removed)
                 if (items == 4) { if (first) OPPX_HIST(4, true); else OPPX_HIST(4, false); }
                 else if (items == 8) { if (first) OPPX_HIST(8, true); else OPPX_HIST(8, false); }
-                else { if (first) OPPX_HIST(16, true); else OPPX_HIST(16, false); }
+                else if (items == 16) { if (first) OPPX_HIST(16, true); else OPPX_HIST(16, false); }
+                else { if (first) OPPX_HIST(32, true); else OPPX_HIST(32, false); }
 #undef OPPX_HIST
                 OPPX_LAUNCH_CHECK("sort_hist");
                 const int64_t per_tile = (int64_t)kB * items;
@@ -530,7 +532,8 @@ int oppx_sort_by_cell(int64_t n_max, int64_t stride, oppx_move_counters* counter
     } while (0)
                 if (items == 4) OPPX_SCATTER_LF(4);
                 else if (items == 8) OPPX_SCATTER_LF(8);
-                else OPPX_SCATTER_LF(16);
+                else if (items == 16) OPPX_SCATTER_LF(16);
+                else OPPX_SCATTER_LF(32);
 #undef OPPX_SCATTER_LF
 #undef OPPX_SCATTER
                 OPPX_LAUNCH_CHECK("sort_scatter");

@@ -17,7 +17,9 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("n,n_cells,frac_removed,seed,items", [(1, 5, 0.0, 0, 16), (1000, 48, 0.1, 1, 4),
                                                                (70001, 300, 0.3, 2, 8), (300000, 6_000_000, 0.05, 3, 16),
                                                                (300000, 6_000_000, 0.05, 3, 4), (5000, 7, 0.0, 4, 8),
-                                                               (4096, 100000, 1.0, 5, 16), (20000, 40_000_000, 0.1, 6, 8)])
+                                                               (4096, 100000, 1.0, 5, 16), (20000, 40_000_000, 0.1, 6, 8),
+                                                               (300000, 6_000_000, 0.05, 3, 32), (70001, 300, 0.3, 2, 32),
+                                                               (8191, 7, 0.0, 7, 32)])
 def test_gpu_sort_matches_oracle(n, n_cells, frac_removed, seed, items):
     saved = oppx.get_tuning()
     oppx.set_tuning(**dict(saved, sort_items=items))
