@@ -6,12 +6,15 @@
 // particles by p2c with removed particles (MAX_CELL_INDEX) last. Their tie order differs between backends
 // (std::sort / thrust radix); this build's contract is stable by (p2c, index).
 //
-// Each pass sorts by one 8-bit digit. A block owns 256 * sort_items consecutive elements (tuning),
-// each wave a contiguous part of them; ranks come from wave-level digit matching (8 ballots give the
-// lanes holding the same digit) against the wave's own digit counts, so equal digits keep their index
-// order and no LDS atomic is contended by nearly sorted keys. Per-block digit counts (a histogram
-// pass that needs no ranks) are scanned digit-major across blocks to give every (digit, block) its
-// output offset.
+// Each pass sorts by one 8-bit digit. A tile is 256 * sort_items consecutive elements (tuning), each
+// wave of its block a contiguous part of them; ranks come from wave-level digit matching (8 ballots
+// give the lanes holding the same digit) against the wave's own digit counts, so equal digits keep
+// their index order and no LDS atomic is contended by nearly sorted keys. Per-tile digit counts (a
+// histogram pass that needs no ranks) are scanned digit-major across the tiles to give every
+// (digit, tile) its output offset. Every kernel reads the live count on the device: the histogram and
+// scatter grids (sized by the capacity, capped at a few times the resident blocks) stride over the
+// live tiles only, the counts are laid out with the live tile count as row stride, and the scan
+// covers only those rows.
 
 #include "oppx_internal.h"
 
